@@ -1,0 +1,235 @@
+/*
+ * mcflow.h -- C-ABI of the MI355X flow-solver library (libmcflow.so).
+ *
+ * The library implements miniCOMBUST's flow-solver SIMPLE step on one GPU
+ * with hand-written HIP/CDNA4 kernels.  It replaces, below the reference's
+ * FlowSolver<T> (include/flow/gpu/FlowSolver.{hpp,inl}), the layer formed by
+ * the C_kernel_* launch wrappers (include/flow/gpu/gpu_kernels.cuh:8-83) and
+ * the AmgX C API (include/flow/gpu/FlowSolver.hpp:1008-1067,
+ * include/flow/gpu/FlowSolver.inl:716-1103).
+ *
+ * Conventions: every entry point returns MC_OK (0) or a positive error code
+ * and never exits; mc_last_error() gives the message.  All device memory is
+ * owned by the context; host pointers are plain arrays.  Work runs on the
+ * context's HIP stream; entry points that return host data synchronise it.
+ * Cell arrays have N + 6 entries: the reference's 6 boundary-side slots
+ * (phi[N + side], FlowSolver.hpp:211-213) follow the N cells.
+ */
+#ifndef MCFLOW_H
+#define MCFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MC_OK = 0,
+    MC_ERR_ARG = 1,        /* invalid argument                              */
+    MC_ERR_HIP = 2,        /* HIP runtime error (message in mc_last_error)  */
+    MC_ERR_NOMEM = 3,      /* device allocation failed                      */
+    MC_ERR_STATE = 4,      /* call order violated                           */
+    MC_ERR_NOCONV = 5,     /* linear solver did not converge                */
+    MC_ERR_NODEV = 6       /* no HIP device                                 */
+};
+
+/* Field ids: the reference phi_vector order (include/utils/utils.hpp:158-174). */
+enum { MC_U = 0, MC_V, MC_W, MC_P, MC_PP, MC_TE, MC_ED, MC_TP, MC_TEM, MC_FUL,
+       MC_PRO, MC_VARF, MC_VARP, MC_NFIELDS };
+
+/* Scalar equation types (include/utils/utils.hpp:31-37). */
+enum { MC_TERBTE = 0, MC_TERBED, MC_TEMP, MC_FUEL, MC_PROG, MC_VARFU, MC_VARPR };
+
+/* Pressure-correction semantics.
+ * MC_MODE_COMPAT: the reference CPU path as written -- the pressure-correction
+ *   system is assembled and stored only on the first call (first_pressure,
+ *   FlowSolver.inl:2118-2253) and every later solve re-solves it; the cell
+ *   correction touches cell 0 of the first loop only (FlowSolver.inl:2416-2433).
+ * MC_MODE_SIMPLE: the system is re-assembled from the current mass fluxes each
+ *   SIMPLE iteration, its current RHS is solved in every correction loop, and
+ *   every cell is corrected every loop. */
+enum { MC_MODE_COMPAT = 0, MC_MODE_SIMPLE = 1 };
+
+/* Pressure-solve forms. */
+enum {
+    MC_PSOLVE_FP64 = 0,    /* FP64 PCG + aggregation-AMG V-cycle                      */
+    MC_PSOLVE_MIXED = 1,   /* FP32 PCG-AMG (SpMV, smoothers, hierarchy) inside an FP64
+                              iterative-refinement loop; FP64 residual, update and
+                              coarsest solve                                          */
+};
+
+typedef struct mc_options {
+    int    mode;           /* MC_MODE_*                                            */
+    int    psolve;         /* MC_PSOLVE_*                                          */
+    int    learned;        /* 1: MPNN initial guess for the pressure solve         */
+    int    solve_scalars;  /* 1: the 7 scalar equations of the SIMPLE step         */
+    double delta;          /* time step, src/minicombust.cpp:120 (1e-5)            */
+    double mom_rtol;       /* BiCGStab rtol, momentum and scalars (PETSc: 1e-5)    */
+    int    mom_maxit;
+    double p_rtol;         /* pressure-correction relative residual (1e-8)         */
+    int    p_maxit;
+    int    amg_nu;         /* Jacobi sweeps before and after the coarse correction */
+    double amg_omega;      /* Jacobi damping                                       */
+    double amg_cscale;     /* coarse-correction scaling                            */
+    int    amg_coarse_max; /* dense coarsest level size bound                      */
+    double mixed_inner_rtol; /* FP32 inner solve reduction per refinement step     */
+    int    check_every;    /* Krylov iterations between host convergence checks    */
+} mc_options;
+
+typedef struct mc_solve_status {
+    int    iterations;     /* Krylov iterations (summed over RHS / refinement steps) */
+    int    converged;      /* 1 if every RHS met its tolerance                      */
+    double relres;         /* final ||b - A x|| / ||b|| (max over RHS)              */
+    int    outer;          /* refinement steps (mixed form)                         */
+    int    guess_used;     /* learned initial guess accepted                        */
+    double bnorm;          /* ||b|| (pressure: of the mean-projected RHS)           */
+} mc_solve_status;
+
+typedef struct mc_stats {
+    int64_t simple_iterations;
+    int64_t mom_iters, scalar_iters, p_iters, p_solves, p_loops, p_outer;
+    double  last_p_relres;
+    double  p_solve_ms;       /* host wall time inside pressure solves           */
+    double  last_p_max;       /* max |PP| of the last correction loop            */
+    int     last_loops;
+} mc_stats;
+
+/* Host view of a mesh, the arrays of the reference Mesh<T>
+ * (include/geometry/Mesh.hpp:70-107), one flow rank. */
+typedef struct mc_mesh_desc {
+    int64_t n_cells, n_faces, n_points;
+    const double   *points;          /* [3 n_points] x,y,z interleaved (vec<double>)   */
+    const uint64_t *cells;           /* [8 n_cells] vertex ids A..H                    */
+    const uint64_t *cell_neighbours; /* [6 n_cells], UINT64_MAX at the boundary        */
+    const uint64_t *faces;           /* [2 n_faces] (cell0, cell1); boundary cell1 =
+                                        n_cells + side                                 */
+    const uint64_t *cell_faces;      /* [6 n_cells]                                    */
+    uint64_t boundary_types[6];      /* Mesh.hpp:17 BOUNDARY_TYPES per box side        */
+} mc_mesh_desc;
+
+/* Array ids for mc_array_get / mc_array_set. */
+enum {
+    MC_ARR_PHI = 0,          /* + field id: phi[field], N+6 f64                    */
+    MC_ARR_GRAD = 16,        /* + field id: gradient, (N+6) x 3 f64 interleaved     */
+    MC_ARR_AU = 32, MC_ARR_AV, MC_ARR_AW, MC_ARR_SU, MC_ARR_SV, MC_ARR_SW, /* N+6 f64 */
+    MC_ARR_FF0 = 40, MC_ARR_FF1, MC_ARR_MFLUX,                       /* F f64      */
+    MC_ARR_FACE_AREA = 48, MC_ARR_FACE_LAMBDA, MC_ARR_FACE_RLENCOS,  /* F f64      */
+    MC_ARR_FACE_NORMAL, MC_ARR_FACE_CENTER,                          /* F x 3 f64  */
+    MC_ARR_CELL_CENTER, MC_ARR_CELL_VOLUME, MC_ARR_CELL_DENSITY,     /* N(x3) f64  */
+    MC_ARR_ROW_PTR = 64, MC_ARR_COL,                                 /* i32        */
+    MC_ARR_A_VALS, MC_ARR_PA_VALS, MC_ARR_PB,                        /* f64        */
+    MC_ARR_FACES = 72, MC_ARR_CELL_FACES, MC_ARR_CELL_NEIGHBOURS,    /* u64        */
+    MC_ARR_CELLS, MC_ARR_POINTS, MC_ARR_CELLS_PER_POINT              /* u64/f64/u8 */
+};
+
+/* Kernel classes timed with HIP events when profiling is on. */
+enum {
+    MC_KC_GRAD = 0,      /* 11-field LSQ gradient                      */
+    MC_KC_GRAD1,         /* 1-field LSQ gradient                       */
+    MC_KC_MOM_ASM,       /* momentum fluxes + sources + CSR assembly   */
+    MC_KC_P_ASM,         /* Rhie-Chow mass flux + pressure assembly    */
+    MC_KC_SCALAR_ASM,    /* scalar fluxes + sources + CSR assembly     */
+    MC_KC_SPMV64,        /* FP64 CSR SpMV (+fused dots)                */
+    MC_KC_SPMV32,        /* FP32 CSR SpMV                              */
+    MC_KC_VCYCLE,        /* AMG V-cycle (all levels)                   */
+    MC_KC_CORRECT,       /* flux / cell correction                     */
+    MC_KC_MPNN,          /* message-passing net inference              */
+    MC_KC_COUNT
+};
+
+/* ---- context: the FlowSolver constructor ----------------------------------
+ * mc_ctx_create   replaces FlowSolver<T>::FlowSolver(MPI_Config*, Mesh<T>*,
+ *                 double delta, FILE*) (include/flow/gpu/FlowSolver.hpp:274-1067):
+ *                 uploads the mesh, computes face/cell geometry (FlowSolver.hpp:
+ *                 252-398), initial fields and boundary slots (:301-485), the CSR
+ *                 pattern (:491-502) and the first assemblies (:538-539).
+ * mc_ctx_create_box  load_mesh (src/examples/mesh_examples.cpp:120-571, one flow
+ *                 rank, box of src/minicombust.cpp:123-124) generated on the GPU,
+ *                 followed by the constructor above; nothing crosses PCIe.     */
+typedef struct mc_ctx mc_ctx;
+int  mc_options_default(mc_options *opt);
+int  mc_ctx_create(const mc_mesh_desc *mesh, const mc_options *opt, mc_ctx **out);
+int  mc_ctx_create_box(int64_t modifier, const mc_options *opt, mc_ctx **out);
+int  mc_ctx_destroy(mc_ctx *ctx);
+const char *mc_last_error(mc_ctx *ctx);
+int  mc_ctx_sizes(mc_ctx *ctx, int64_t *n_cells, int64_t *n_faces, int64_t *nnz, int64_t *n_points);
+int  mc_set_options(mc_ctx *ctx, const mc_options *opt);
+
+/* ---- SIMPLE step ---------------------------------------------------------
+ * mc_timestep        FlowSolver<T>::timestep() (FlowSolver.inl:3148-3356):
+ *                    three SIMPLE iterations.
+ * mc_simple_iteration one pass of its loop body (FlowSolver.inl:3190-3267). */
+int  mc_timestep(mc_ctx *ctx);
+int  mc_simple_iteration(mc_ctx *ctx, int inner);
+
+/* ---- stages (each the counterpart of a reference kernel wrapper) ---------- */
+/* C_kernel_get_phi_gradients (gpu_kernels.cuh:30) / get_phi_gradients
+   (FlowSolver.inl:917-1063): 11-field LSQ gradients.                          */
+int  mc_get_phi_gradients(mc_ctx *ctx);
+/* C_kernel_get_phi_gradient (gpu_kernels.cuh:48) / get_phi_gradient
+   (FlowSolver.inl:738-803); pressure != 0 selects dphi = 0 on boundaries.     */
+int  mc_get_phi_gradient(mc_ctx *ctx, int field, int pressure);
+/* C_kernel_precomp_AU (gpu_kernels.cuh:38) + calculate_mass_flux:
+   set_up_field (FlowSolver.inl:1065-1113).                                   */
+int  mc_set_up_field(mc_ctx *ctx);
+/* C_kernel_calculate_flux_UVW + C_kernel_apply_forces + C_kernel_setup_sparse_
+   matrix + C_kernel_update_sparse_matrix (gpu_kernels.cuh:32,34,36,40):
+   calculate_UVW without its solves (FlowSolver.inl:1655-1758).               */
+int  mc_momentum_assemble(mc_ctx *ctx);
+/* the three AMGX_solver_solve calls of calculate_UVW (gpu/FlowSolver.inl:
+   716-782) / KSPSolve (FlowSolver.inl:1341-1375): BiCGStab, x0 = 0.          */
+int  mc_momentum_solve(mc_ctx *ctx, mc_solve_status *st);
+/* C_kernel_calculate_mass_flux + C_kernel_compute_flow_correction +
+   C_kernel_correct_flow{,2} + C_kernel_setup_pressure_matrix
+   (gpu_kernels.cuh:54,56,58,28,42): the start of calculate_pressure
+   (FlowSolver.inl:2333-2360).                                                */
+int  mc_pressure_assemble(mc_ctx *ctx);
+/* the pressure AMGX_solver_solve (gpu/FlowSolver.inl:943-969) / KSPSolve with
+   FGMRES+GAMG (FlowSolver.inl:2203-2273): PCG + aggregation-AMG, mean-zero
+   solution, into phi[PP].                                                    */
+int  mc_pressure_solve(mc_ctx *ctx, mc_solve_status *st);
+/* calculate_pressure (FlowSolver.inl:2317-2463): assembly, correction loop
+   (C_kernel_find_pressure_correction_max, _Update_P_at_boundaries,
+   _update_vel_and_flux, _update_mass_flux) and C_kernel_Update_P.            */
+int  mc_calculate_pressure(mc_ctx *ctx);
+/* Scalar_solve (FlowSolver.inl:2885-3076): C_kernel_flux_scalar,
+   _apply_pres_forces, _solve_turb_models_{cell,face}, _setup_sparse_matrix
+   (gpu_kernels.cuh:60-66,36) and the solve.                                  */
+int  mc_scalar_assemble(mc_ctx *ctx, int type);
+int  mc_scalar_solve(mc_ctx *ctx, int type);
+/* C_kernel_Update_P_at_boundaries (gpu_kernels.cuh:46) and C_kernel_Update_P
+   (gpu_kernels.cuh:52) on a field.                                          */
+int  mc_update_boundary_slots(mc_ctx *ctx, int field);
+int  mc_update_P(mc_ctx *ctx);
+
+/* ---- linear algebra on the context's CSR pattern ------------------------- */
+/* y = A x with which = 0 (momentum/scalar matrix) or 1 (pressure matrix).   */
+int  mc_spmv(mc_ctx *ctx, int which, const double *x_host, double *y_host);
+/* Solve the pressure matrix with a host RHS (tests, data generation).        */
+int  mc_solve_pressure_host(mc_ctx *ctx, const double *b_host, double *x_host, mc_solve_status *st);
+
+/* ---- state access -------------------------------------------------------- */
+int  mc_array_get(mc_ctx *ctx, int id, void *host, int64_t count);
+int  mc_array_set(mc_ctx *ctx, int id, const void *host, int64_t count);
+int  mc_get_stats(mc_ctx *ctx, mc_stats *out);
+int  mc_state_flags(mc_ctx *ctx, int *first_pressure, int64_t *timestep_count);
+int  mc_set_state_flags(mc_ctx *ctx, int first_pressure, int64_t timestep_count);
+int  mc_sync(mc_ctx *ctx);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Profiling on: each kernel class is bracketed by HIP events on the context
+   stream; mc_profile_read returns accumulated milliseconds, launch counts and
+   algorithmic bytes per class (arrays of MC_KC_COUNT).                       */
+int  mc_profile_enable(mc_ctx *ctx, int on);
+int  mc_profile_read(mc_ctx *ctx, double *ms, int64_t *calls, double *bytes);
+int  mc_profile_reset(mc_ctx *ctx);
+
+/* ---- learned pressure-correction initial guess -------------------------- */
+int  mc_mpnn_load(mc_ctx *ctx, const float *weights, int64_t count, int hidden, int layers);
+int  mc_mpnn_infer(mc_ctx *ctx, const double *rhs_dev_or_null, double *x_host_or_null);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1068 @@
+// mc_flow.hip -- the SIMPLE-step kernels and their host drivers.
+//
+// Every reference face loop that scatters into two cells (S_phi[cell0] -=,
+// S_phi[cell1] += ...) is recast as a cell-centric gather: one thread per
+// cell walks its six faces in ascending face id -- the order in which the
+// sequential face loop reaches them -- and performs the same additions in the
+// same order, so sums round exactly as in the reference, with no atomics.  The
+// owner (cell0) of an interior face writes the face's own results
+// (face_fields, mass flux).  Boundary-side slots (phi[N + side]) are written
+// by the owner of the highest-numbered face of that side: the reference's
+// last writer.  Compiled without FP contraction so each expression rounds as
+// written in FlowSolver.inl.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "mc_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mc {
+
+struct Geo {
+    int N, F;
+    const int *fc0, *fc1, *forder, *cell_faces, *nbr, *row_ptr, *diag_idx;
+    const double *cx, *cy, *cz, *fcx, *fcy, *fcz, *fnx, *fny, *fnz, *farea, *flam, *frlc, *vol, *rho;
+    const uint8_t *is_wall;
+    int btype[6];
+    int last_face[6];
+};
+
+static Geo geo(Ctx *c)
+{
+    Geo g;
+    g.N = c->N; g.F = c->F;
+    g.fc0 = c->fc0; g.fc1 = c->fc1; g.forder = c->forder; g.cell_faces = c->cell_faces; g.nbr = c->nbr;
+    g.row_ptr = c->row_ptr; g.diag_idx = c->diag_idx;
+    g.cx = c->cx; g.cy = c->cy; g.cz = c->cz; g.fcx = c->fcx; g.fcy = c->fcy; g.fcz = c->fcz;
+    g.fnx = c->fnx; g.fny = c->fny; g.fnz = c->fnz; g.farea = c->farea; g.flam = c->flam; g.frlc = c->frlc;
+    g.vol = c->vol; g.rho = c->rho; g.is_wall = c->is_wall_cell;
+    for (int s = 0; s < 6; s++) { g.btype[s] = c->btype[s]; g.last_face[s] = c->last_face[s]; }
+    return g;
+}
+
+struct dv3 { double x, y, z; };
+__device__ __forceinline__ dv3 v3(double x, double y, double z) { dv3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ dv3 operator+(dv3 a, dv3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ dv3 operator-(dv3 a, dv3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ dv3 operator*(dv3 a, double b) { return v3(a.x * b, a.y * b, a.z * b); }
+__device__ __forceinline__ dv3 operator/(dv3 a, double b) { return v3(a.x / b, a.y / b, a.z / b); }
+__device__ __forceinline__ double dot(dv3 a, dv3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double mag(dv3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ dv3 ld3(const double *x, const double *y, const double *z, int i) { return v3(x[i], y[i], z[i]); }
+
+// Position of each entry of row c in the sorted CSR row {c} u {nb_k}.
+__device__ __forceinline__ void row_positions(int c, int n_int, const int *nb, int *pos, int &dpos)
+{
+    dpos = 0;
+    for (int k = 0; k < n_int; k++) dpos += nb[k] < c;
+    for (int k = 0; k < n_int; k++) {
+        int p = (c < nb[k]);
+        for (int t = 0; t < n_int; t++) p += nb[t] < nb[k];
+        pos[k] = p;
+    }
+}
+
+// ========================================================== A4 / A5 ==
+struct GradArgs {
+    const double *phi[11];
+    double *gx[11], *gy[11], *gz[11];
+    int zero_boundary[11];
+};
+
+// get_phi_gradients / get_phi_gradient (FlowSolver.inl:738-803, 917-1063):
+// A = sum dX dX^T, b_k = sum dX dphi_k over the six faces in side order,
+// then the 3x3 Cramer solve of FlowSolver.inl:895-915.  For an interior face
+// mask*(cc1-cc0) and mask*(phi1-phi0) equal cc[nb]-cc[c] and phi[nb]-phi[c]
+// exactly (IEEE subtraction is antisymmetric).
+template <int NF>
+__global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int c = b * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
+    double ph[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) ph[k] = a.phi[k][c];
+    double a00 = 0.0, a10 = 0.0, a20 = 0.0, a11 = 0.0, a21 = 0.0, a22 = 0.0;
+    double bx[NF], by[NF], bz[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) { bx[k] = 0.0; by[k] = 0.0; bz[k] = 0.0; }
+    for (int s = 0; s < 6; s++) {
+        const int nb = g.nbr[6 * c + s];
+        dv3 dX;
+        double d[NF];
+        if (nb >= 0) {
+            dX = ld3(g.cx, g.cy, g.cz, nb) - cc;
+#pragma unroll
+            for (int k = 0; k < NF; k++) d[k] = a.phi[k][nb] - ph[k];
+        } else {
+            const int f = g.cell_faces[6 * c + s];
+            const int slot = g.fc1[f];
+            dX = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+#pragma unroll
+            for (int k = 0; k < NF; k++) d[k] = a.zero_boundary[k] ? 0.0 : a.phi[k][slot] - ph[k];
+        }
+        a00 += (dX.x * dX.x); a10 += (dX.x * dX.y); a20 += (dX.x * dX.z);
+        a11 += (dX.y * dX.y); a21 += (dX.y * dX.z); a22 += (dX.z * dX.z);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            bx[k] += (dX.x * d[k]); by[k] += (dX.y * d[k]); bz[k] += (dX.z * d[k]);
+        }
+    }
+    // A[i][j] == A[j][i] bit for bit (commutative products, same order)
+    const double A00 = a00, A01 = a10, A02 = a20, A10 = a10, A11 = a11, A12 = a21, A20 = a20, A21 = a21, A22 = a22;
+    const double det = (A00 * (A11 * A22 - A21 * A12)) - (A01 * (A10 * A22 - A20 * A12)) +
+                       (A02 * (A10 * A21 - A20 * A11));
+    double invdet = 0.0;
+    if (det != 0.0) invdet = 1 / det;
+    const double c00 = (A11 * A22 - A21 * A12), c01 = (A21 * A02 - A01 * A22), c02 = (A01 * A12 - A11 * A02);
+    const double c10 = (A12 * A20 - A10 * A22), c11 = (A00 * A22 - A20 * A02), c12 = (A10 * A02 - A00 * A12);
+    const double c20 = (A10 * A21 - A20 * A11), c21 = (A20 * A01 - A00 * A21), c22 = (A00 * A11 - A10 * A01);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+        a.gx[k][c] = (bx[k] * invdet * c00) + (by[k] * invdet * c01) + (bz[k] * invdet * c02);
+        a.gy[k][c] = (bx[k] * invdet * c10) + (by[k] * invdet * c11) + (bz[k] * invdet * c12);
+        a.gz[k][c] = (bx[k] * invdet * c20) + (by[k] * invdet * c21) + (bz[k] * invdet * c22);
+    }
+}
+
+// LSQ bytes: 11 fields read (8N) and gradients written (24N), centres, face
+// centres of boundary faces, neighbour lists.
+static double grad_bytes(Ctx *c, int nf)
+{
+    return 32.0 * nf * c->N + 48.0 * c->N + 8.0 * c->F + 24.0 * c->Fb;
+}
+
+int grad_multi(Ctx *c)
+{
+    static const int fields[11] = {U, V, W, P, TE, ED, TEM, FUL, PRO, VARF, VARP};
+    GradArgs a;
+    for (int k = 0; k < 11; k++) {
+        a.phi[k] = c->phi[fields[k]];
+        a.gx[k] = c->gx[fields[k]]; a.gy[k] = c->gy[fields[k]]; a.gz[k] = c->gz[fields[k]];
+        a.zero_boundary[k] = fields[k] == P;
+    }
+    KTimer kt(c, MC_KC_GRAD, grad_bytes(c, 11));
+    hipLaunchKernelGGL(k_grad<11>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+int grad_single(Ctx *c, int field, bool pressure)
+{
+    GradArgs a;
+    memset(&a, 0, sizeof(a));
+    a.phi[0] = c->phi[field];
+    a.gx[0] = c->gx[field]; a.gy[0] = c->gy[field]; a.gz[0] = c->gz[field];
+    a.zero_boundary[0] = pressure ? 1 : 0;
+    KTimer kt(c, MC_KC_GRAD1, grad_bytes(c, 1));
+    hipLaunchKernelGGL(k_grad<1>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// ============================================================== A6 ==
+// precomp_AU (FlowSolver.inl:1065-1104)
+__global__ void __launch_bounds__(256) k_precomp_AU(Geo g, const double *mflux, double *AU, double ev, double rdelta)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    double A = 0.0;
+    for (int k = 0; k < 6; k++) {
+        const int f = g.forder[6 * c + k];
+        const int c1 = g.fc1[f];
+        if (c1 < g.N) continue;
+        const int bt = g.btype[c1 - g.N];
+        if (bt == INLET || bt == OUTLET) {
+            const double VisFace = ev * g.frlc[f];
+            const double fb = -VisFace + dmin(mflux[f], 0.0);
+            A = A - fb;
+        }
+    }
+    const double f = g.rho[c] * g.vol[c] * rdelta;
+    A += f;
+    AU[c] = A;
+}
+
+int precomp_AU(Ctx *c)
+{
+    hipLaunchKernelGGL(k_precomp_AU, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), c->mflux, c->AU, c->k.ev,
+                       c->k.rdelta);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// ====================================================== A7 + A8 + A9 ==
+struct MomArgs {
+    double *U, *V, *W;
+    const double *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz, *gPx, *gPy, *gPz, *TEM;
+    double *mflux, *ff0, *ff1, *AU, *SU, *SV, *SW, *vals;
+    double ev, inlet_ev, rdelta, urf, gas_tem, vel_x, vel_y, vel_z;
+};
+
+// calculate_flux_UVW (FlowSolver.inl:1377-1653), the cell sources of
+// calculate_UVW (:1689-1721) and setup/update_sparse_matrix (:1186-1339).
+// A_phi.U, .V and .W receive identical updates, so one diagonal serves all
+// three momentum systems.
+__global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const int N = g.N;
+    const double GammaBlend = 0.0;
+    double SU = 0.0, SV = 0.0, SW = 0.0, A = 0.0;
+    double own_ff[6];
+    int own_nb[6];
+    int nint = 0;
+    for (int k = 0; k < 6; k++) {
+        const int f = g.forder[6 * c + k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const double mf = a.mflux[f];
+        if (c1 < N) {
+            const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
+            const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0) * lambda0 + ld3(a.gUx, a.gUy, a.gUz, c1) * lambda1;
+            const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0) * lambda0 + ld3(a.gVx, a.gVy, a.gVz, c1) * lambda1;
+            const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0) * lambda0 + ld3(a.gWx, a.gWy, a.gWz, c1) * lambda1;
+            const double Visac = a.ev * lambda0 + a.ev * lambda1;
+            const double VisFace = Visac * g.frlc[f];
+            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const double U0 = a.U[c0], U1 = a.U[c1], V0 = a.V[c0], V1 = a.V[c1], W0 = a.W[c0], W1 = a.W[c1];
+            double UFace, VFace, WFace;
+            if (mf >= 0.0) { UFace = U0; VFace = V0; WFace = W0; }
+            else { UFace = U1; VFace = V1; WFace = W1; }
+            const double fuce = mf * UFace, fvce = mf * VFace, fwce = mf * WFace;
+            const double sx = g.fnx[f], sy = g.fny[f], sz = g.fnz[f];
+            const double fude = Visac * ((dUdXac.x + dUdXac.x) * sx + (dUdXac.y + dVdXac.x) * sy + (dUdXac.z + dWdXac.x) * sz);
+            const double fvde = Visac * ((dUdXac.y + dVdXac.x) * sx + (dVdXac.y + dVdXac.y) * sy + (dVdXac.z + dWdXac.y) * sz);
+            const double fwde = Visac * ((dUdXac.z + dWdXac.x) * sx + (dWdXac.y + dVdXac.z) * sy + (dWdXac.z + dWdXac.z) * sz);
+            const double fmin = dmin(mf, 0.0), fmax = dmax(mf, 0.0);
+            const double fuci = fmin * U0 + fmax * U1;
+            const double fvci = fmin * V0 + fmax * V1;
+            const double fwci = fmin * W0 + fmax * W1;
+            const double fudi = VisFace * dot(dUdXac, Xpn);
+            const double fvdi = VisFace * dot(dVdXac, Xpn);
+            const double fwdi = VisFace * dot(dWdXac, Xpn);
+            const double ff0 = -VisFace - dmax(mf, 0.0);
+            const double ff1 = -VisFace + dmin(mf, 0.0);
+            const double blend_u = GammaBlend * (fuce - fuci);
+            const double blend_v = GammaBlend * (fvce - fvci);
+            const double blend_w = GammaBlend * (fwce - fwci);
+            if (c == c0) {
+                SU = SU - blend_u + fude - fudi;
+                SV = SV - blend_v + fvde - fvdi;
+                SW = SW - blend_w + fwde - fwdi;
+                own_ff[nint] = ff1; own_nb[nint] = c1;
+                a.ff0[f] = ff0; a.ff1[f] = ff1;
+            } else {
+                SU = SU + blend_u - fude + fudi;
+                SV = SV + blend_v - fvde + fvdi;
+                SW = SW + blend_w - fwde + fwdi;
+                own_ff[nint] = ff0; own_nb[nint] = c0;
+            }
+            nint++;
+        } else {
+            const int side = c1 - N;
+            const int bt = g.btype[side];
+            const dv3 cc = ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+            const bool last = (f == g.last_face[side]);
+            if (bt == INLET || bt == OUTLET) {
+                const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0);
+                const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0);
+                const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0);
+                double UFace, VFace, WFace, Visac;
+                if (bt == INLET) {
+                    UFace = a.vel_x; VFace = a.vel_y; WFace = a.vel_z; Visac = a.inlet_ev;
+                } else {
+                    Visac = a.ev; UFace = a.U[c0]; VFace = a.V[c0]; WFace = a.W[c0];
+                }
+                const double VisFace = Visac * g.frlc[f];
+                const double sx = g.fnx[f], sy = g.fny[f], sz = g.fnz[f];
+                const double fude = Visac * ((dUdXac.x + dUdXac.x) * sx + (dUdXac.y + dVdXac.x) * sy + (dUdXac.z + dWdXac.x) * sz);
+                const double fvde = Visac * ((dUdXac.y + dVdXac.x) * sx + (dVdXac.y + dVdXac.y) * sy + (dVdXac.z + dWdXac.y) * sz);
+                const double fwde = Visac * ((dUdXac.z + dWdXac.x) * sx + (dWdXac.y + dVdXac.z) * sy + (dWdXac.z + dWdXac.z) * sz);
+                const double fudi = VisFace * dot(dUdXac, Xpn);
+                const double fvdi = VisFace * dot(dVdXac, Xpn);
+                const double fwdi = VisFace * dot(dWdXac, Xpn);
+                double m = mf;
+                if (bt == OUTLET && m < 0.0) { m = 1e-15; a.mflux[f] = m; }   // FlowSolver.inl:1576-1580
+                const double fb = -VisFace + dmin(m, 0.0);
+                A = A - fb;
+                SU = SU - fb * UFace + fude - fudi;
+                SV = SV - fb * VFace + fvde - fvdi;
+                SW = SW - fb * WFace + fwde - fwdi;
+                if (last) { a.U[N + side] = UFace; a.V[N + side] = VFace; a.W[N + side] = WFace; }
+            } else if (bt == WALL) {
+                const double Visac = a.ev;
+                const double coef = Visac * g.frlc[f];
+                const double U0 = a.U[c0], V0 = a.V[c0], W0 = a.W[c0];
+                const dv3 Up = v3(U0 - 0., V0 - 0., W0 - 0.);
+                const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+                const dv3 nh = n / mag(n);
+                const double dp = dot(Up, nh);
+                const dv3 Ut = Up - nh * dp;
+                const double Uvel = fabs(Ut.x) + fabs(Ut.y) + fabs(Ut.z);
+                dv3 force = v3(0.0, 0.0, 0.0);
+                if (Uvel > 0.0) {
+                    const double distance_to_face = mag(Xpn);
+                    force = (Ut * (g.farea[f] * Visac)) / distance_to_face;
+                }
+                A = A + coef;
+                SU = SU + coef * U0 - force.x;
+                SV = SV + coef * V0 - force.y;
+                SW = SW + coef * W0 - force.z;
+                if (last) { a.U[N + side] = 0.; a.V[N + side] = 0.; a.W[N + side] = 0.; }
+            }
+        }
+    }
+    // buoyancy, pressure gradient, transient (FlowSolver.inl:1689-1721)
+    const double vol = g.vol[c], rho = g.rho[c];
+    const double BodyForce = -0.001 * rho * vol * (a.TEM[c] - a.gas_tem);
+    SU += 0.0 * BodyForce;
+    SV += -9.81 * BodyForce;
+    SW += 0.0 * BodyForce;
+    SU -= a.gPx[c] * vol;
+    SV -= a.gPy[c] * vol;
+    SW -= a.gPz[c] * vol;
+    const double fv = rho * vol * a.rdelta;
+    const double Uc = a.U[c], Vc = a.V[c], Wc = a.W[c];
+    SU += fv * Uc; SV += fv * Vc; SW += fv * Wc;
+    A += fv;
+    // setup_sparse_matrix / update_sparse_matrix
+    for (int k = 0; k < nint; k++) A -= own_ff[k];
+    A *= (1. / a.urf);
+    SU = SU + (1.0 - a.urf) * A * Uc;
+    SV = SV + (1.0 - a.urf) * A * Vc;
+    SW = SW + (1.0 - a.urf) * A * Wc;
+    a.AU[c] = A;
+    a.SU[c] = SU; a.SV[c] = SV; a.SW[c] = SW;
+    int pos[6], dpos;
+    row_positions(c, nint, own_nb, pos, dpos);
+    const int rp = g.row_ptr[c];
+    a.vals[rp + dpos] = A;
+    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+}
+
+static MomArgs mom_args(Ctx *c)
+{
+    MomArgs a;
+    a.U = c->phi[U]; a.V = c->phi[V]; a.W = c->phi[W];
+    a.gUx = c->gx[U]; a.gUy = c->gy[U]; a.gUz = c->gz[U];
+    a.gVx = c->gx[V]; a.gVy = c->gy[V]; a.gVz = c->gz[V];
+    a.gWx = c->gx[W]; a.gWy = c->gy[W]; a.gWz = c->gz[W];
+    a.gPx = c->gx[P]; a.gPy = c->gy[P]; a.gPz = c->gz[P];
+    a.TEM = c->phi[TEM];
+    a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1;
+    a.AU = c->AU; a.SU = c->SU; a.SV = c->SV; a.SW = c->SW; a.vals = c->A_vals;
+    a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.5; a.gas_tem = c->k.gas_tem;
+    a.vel_x = c->k.gas_vel_x; a.vel_y = c->k.gas_vel_y; a.vel_z = c->k.gas_vel_z;
+    return a;
+}
+
+int momentum_assemble(Ctx *c)
+{
+    KTimer kt(c, MC_KC_MOM_ASM, 216.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
+    hipLaunchKernelGGL(k_momentum, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), mom_args(c));
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+int momentum_solve(Ctx *c)
+{
+    double *b[3] = {c->SU, c->SV, c->SW};
+    double *x[3] = {c->phi[U], c->phi[V], c->phi[W]};
+    mc_solve_status st{};
+    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    c->st.mom_iters += st.iterations;
+    if (!st.converged) { c->err = "momentum BiCGStab did not converge"; return MC_ERR_NOCONV; }
+    return MC_OK;
+}
+
+// ============================================================= A11 ==
+// Boundary mass fluxes (FlowSolver.inl:1957-2006): fixed inlet velocity at
+// inlet and outlet, zero at walls.
+__global__ void k_boundary_mflux(Geo g, const int *bfaces, int Fb, double *mflux, double *TE, double *ED, double *TEM,
+                                 double vx, double vy, double vz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Fb) return;
+    const int f = bfaces[i];
+    const int side = g.fc1[f] - g.N, c0 = g.fc0[f];
+    const int bt = g.btype[side];
+    const double Din = 1.2;
+    if (bt == INLET) {
+        mflux[f] = Din * (vx * g.fnx[f] + vy * g.fny[f] + vz * g.fnz[f]);
+    } else if (bt == OUTLET) {
+        double m = Din * (vx * g.fnx[f] + vy * g.fny[f] + vz * g.fnz[f]);
+        if (m < 0.0) {
+            m = 1e-15;
+            if (f == g.last_face[side]) { TE[g.N + side] = TE[c0]; ED[g.N + side] = ED[c0]; TEM[g.N + side] = TEM[c0]; }
+        }
+        mflux[f] = m;
+    } else if (bt == WALL) {
+        mflux[f] = 0.0;
+    }
+}
+
+// FlowIn / FlowOut / areaout and the FlowOut < 1e-10 rescale
+// (FlowSolver.inl:2010-2079), one workgroup.
+__global__ void __launch_bounds__(1024) k_outlet_balance(Geo g, const int *bfaces, int Fb, double *mflux, double *U,
+                                                         double *V, double *W, double vx, double vy, double vz)
+{
+    __shared__ double s_in[1024], s_out[1024], s_area[1024];
+    double fin = 0.0, fout = 0.0, aout = 0.0;
+    for (int i = threadIdx.x; i < Fb; i += blockDim.x) {
+        const int f = bfaces[i];
+        const int bt = g.btype[g.fc1[f] - g.N];
+        if (bt == INLET) fin += mflux[f];
+        else if (bt == OUTLET) { fout += mflux[f]; aout += g.farea[f]; }
+    }
+    s_in[threadIdx.x] = fin; s_out[threadIdx.x] = fout; s_area[threadIdx.x] = aout;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s_in[threadIdx.x] += s_in[threadIdx.x + s];
+            s_out[threadIdx.x] += s_out[threadIdx.x + s];
+            s_area[threadIdx.x] += s_area[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    const double FlowIn = s_in[0], FlowOut = s_out[0], areaout = s_area[0];
+    if (!(FlowOut < 0.0000000001)) return;
+    const double ratearea = -FlowIn / areaout;
+    for (int i = threadIdx.x; i < Fb; i += blockDim.x) {
+        const int f = bfaces[i];
+        const int side = g.fc1[f] - g.N;
+        if (g.btype[side] == OUTLET) {
+            mflux[f] = ratearea * g.farea[f];
+            if (f == g.last_face[side]) { U[g.N + side] = vx; V[g.N + side] = vy; W[g.N + side] = vz; }
+        }
+    }
+}
+
+int boundary_mass_flux(Ctx *c)
+{
+    if (c->Fb == 0) return MC_OK;
+    hipLaunchKernelGGL(k_boundary_mflux, dim3(div_up(c->Fb, 256)), dim3(256), 0, c->stream, geo(c), c->bfaces, c->Fb,
+                       c->mflux, c->phi[TE], c->phi[ED], c->phi[TEM], c->k.gas_vel_x, c->k.gas_vel_y, c->k.gas_vel_z);
+    hipLaunchKernelGGL(k_outlet_balance, dim3(1), dim3(1024), 0, c->stream, geo(c), c->bfaces, c->Fb, c->mflux,
+                       c->phi[U], c->phi[V], c->phi[W], c->k.gas_vel_x, c->k.gas_vel_y, c->k.gas_vel_z);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+struct MassArgs {
+    const double *U, *V, *W, *P, *AU;
+    const double *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz, *gPx, *gPy, *gPz;
+    double *mflux, *ff0, *ff1, *SU, *AV, *Pb, *PAvals;
+    int assemble;      // 1: setup_pressure_matrix too
+    int into_pressure; // write the pressure matrix and RHS
+};
+
+// Rhie-Chow interior mass flux (FlowSolver.inl:1893-1950).
+__device__ __forceinline__ double rhie_chow(const Geo &g, const MassArgs &a, int f, int c0, int c1, double &ApV_out)
+{
+    const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
+    const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0) * lambda0 + ld3(a.gUx, a.gUy, a.gUz, c1) * lambda1;
+    const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0) * lambda0 + ld3(a.gVx, a.gVy, a.gVz, c1) * lambda1;
+    const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0) * lambda0 + ld3(a.gWx, a.gWy, a.gWz, c1) * lambda1;
+    const dv3 cc0 = ld3(g.cx, g.cy, g.cz, c0), cc1 = ld3(g.cx, g.cy, g.cz, c1);
+    const dv3 Xac = cc1 * lambda1 + cc0 * lambda0;
+    const dv3 fc = ld3(g.fcx, g.fcy, g.fcz, f);
+    const dv3 delta = fc - Xac;
+    const double UFace = a.U[c1] * lambda1 + a.U[c0] * lambda0 + dot(dUdXac, delta);
+    const double VFace = a.V[c1] * lambda1 + a.V[c0] * lambda0 + dot(dVdXac, delta);
+    const double WFace = a.W[c1] * lambda1 + a.W[c0] * lambda0 + dot(dWdXac, delta);
+    const double rho0 = g.rho[c0], rho1 = g.rho[c1];
+    const double densityf = rho0 * lambda0 + rho1 * lambda1;
+    const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+    double m = densityf * (UFace * n.x + VFace * n.y + WFace * n.z);
+    const dv3 nh = n / mag(n);
+    const dv3 Xpac = fc - nh * dot(fc - cc0, nh);
+    const dv3 Xnac = fc - nh * dot(fc - cc1, nh);
+    const dv3 delp = Xpac - cc0, deln = Xnac - cc1;
+    const dv3 gP0 = ld3(a.gPx, a.gPy, a.gPz, c0), gP1 = ld3(a.gPx, a.gPy, a.gPz, c1);
+    const double cell0_P = a.P[c0] + dot(gP0, delp);
+    const double cell1_P = a.P[c1] + dot(gP1, deln);
+    const dv3 Xpn = Xnac - Xpac;
+    const dv3 Xpn2 = cc1 - cc0;
+    const double A0 = a.AU[c0], A1 = a.AU[c1];
+    const double ApV0 = (A0 != 0.0) ? 1.0 / A0 : 0.0;
+    const double ApV1 = (A1 != 0.0) ? 1.0 / A1 : 0.0;
+    double ApV = rho0 * ApV0 * lambda0 + rho1 * ApV1 * lambda1;
+    const double volume_avg = g.vol[c0] * lambda0 + g.vol[c1] * lambda1;
+    ApV = ApV * g.farea[f] * volume_avg / dot(Xpn2, nh);
+    const double dpx = (gP1.x * lambda1 + gP0.x * lambda0) * Xpn.x;
+    const double dpy = (gP1.y * lambda1 + gP0.y * lambda0) * Xpn.y;
+    const double dpz = (gP1.z * lambda1 + gP0.z * lambda0) * Xpn.z;
+    m -= ApV * ((cell1_P - cell0_P) - dpx - dpy - dpz);
+    ApV_out = ApV;
+    return m;
+}
+
+// calculate_mass_flux (FlowSolver.inl:1880-2104) + setup_pressure_matrix
+// (:2106-2201) for one cell.  S_phi.U receives, in the reference's order, the
+// inlet fluxes of the main face loop, the outlet fluxes of the balance loop,
+// then the interior divergence of setup_pressure_matrix.
+__global__ void __launch_bounds__(256) k_mass_pressure(Geo g, MassArgs a)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const int N = g.N;
+    double S = 0.0, AV = 0.0;
+    int fo[6];
+    for (int k = 0; k < 6; k++) fo[k] = g.forder[6 * c + k];
+    for (int k = 0; k < 6; k++) {
+        const int f = fo[k];
+        if (g.fc1[f] >= N && g.btype[g.fc1[f] - N] == INLET) S = S - a.mflux[f];
+    }
+    for (int k = 0; k < 6; k++) {
+        const int f = fo[k];
+        if (g.fc1[f] >= N && g.btype[g.fc1[f] - N] == OUTLET) S -= a.mflux[f];
+    }
+    double own[6];
+    int own_nb[6];
+    int nint = 0;
+    for (int k = 0; k < 6; k++) {
+        const int f = fo[k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        if (c1 >= N) continue;
+        double ApV;
+        const double m = rhie_chow(g, a, f, c0, c1, ApV);
+        if (c == c0) {
+            a.mflux[f] = m;
+            a.ff0[f] = -ApV; a.ff1[f] = -ApV;
+        }
+        if (a.assemble) {
+            if (c == c0) { S -= m; AV -= -ApV; own_nb[nint] = c1; }
+            else { S += m; AV -= -ApV; own_nb[nint] = c0; }
+            own[nint] = -ApV;
+            nint++;
+        }
+    }
+    a.SU[c] = S;
+    if (!a.assemble) return;
+    a.AV[c] = AV;
+    if (a.into_pressure) {
+        a.Pb[c] = S;
+        int pos[6], dpos;
+        row_positions(c, nint, own_nb, pos, dpos);
+        const int rp = g.row_ptr[c];
+        a.PAvals[rp + dpos] = AV;
+        for (int k = 0; k < nint; k++) a.PAvals[rp + pos[k]] = own[k];
+    }
+}
+
+static MassArgs mass_args(Ctx *c, int assemble, int into_pressure)
+{
+    MassArgs a;
+    a.U = c->phi[U]; a.V = c->phi[V]; a.W = c->phi[W]; a.P = c->phi[P]; a.AU = c->AU;
+    a.gUx = c->gx[U]; a.gUy = c->gy[U]; a.gUz = c->gz[U];
+    a.gVx = c->gx[V]; a.gVy = c->gy[V]; a.gVz = c->gz[V];
+    a.gWx = c->gx[W]; a.gWy = c->gy[W]; a.gWz = c->gz[W];
+    a.gPx = c->gx[P]; a.gPy = c->gy[P]; a.gPz = c->gz[P];
+    a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.SU = c->SU; a.AV = c->AV; a.Pb = c->Pb;
+    a.PAvals = c->PA_vals;
+    a.assemble = assemble; a.into_pressure = into_pressure;
+    return a;
+}
+
+int set_up_field(Ctx *c)
+{
+    // precomp_AU reads the mass fluxes left by the constructor (all zero)
+    MC_TRY(precomp_AU(c));
+    MC_TRY(boundary_mass_flux(c));
+    hipLaunchKernelGGL(k_mass_pressure, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), mass_args(c, 0, 0));
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+int calculate_mass_flux_assemble(Ctx *c)
+{
+    const bool into = c->first_pressure || c->opt.mode == MC_MODE_SIMPLE;
+    MC_TRY(boundary_mass_flux(c));
+    KTimer kt(c, MC_KC_P_ASM, 184.0 * c->N + 88.0 * c->F + 8.0 * c->nnz);
+    hipLaunchKernelGGL(k_mass_pressure, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c),
+                       mass_args(c, 1, into ? 1 : 0));
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// ============================================================= A14 ==
+// mass-flux correction (FlowSolver.inl:2402-2412) fused with the
+// non-orthogonal update_mass_flux (:1812-1878): both add to the flux in that
+// order; the latter's face correction is kept for the RHS gather.
+__global__ void __launch_bounds__(256) k_flux_correct(Geo g, const double *PP, const double *gPPx, const double *gPPy,
+                                                      const double *gPPz, const double *ff0, double *mflux, double *fcorr)
+{
+    const int f = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (f >= g.F) return;
+    const int c0 = g.fc0[f], c1 = g.fc1[f];
+    if (c1 >= g.N) return;
+    const double fact = ff0[f];
+    double m = mflux[f];
+    m += (fact * (PP[c1] - PP[c0]));
+    const dv3 fc = ld3(g.fcx, g.fcy, g.fcz, f), cc0 = ld3(g.cx, g.cy, g.cz, c0), cc1 = ld3(g.cx, g.cy, g.cz, c1);
+    const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+    const dv3 nh = n / mag(n);
+    const dv3 Xpac = fc - nh * dot(fc - cc0, nh);
+    const dv3 Xnac = fc - nh * dot(fc - cc1, nh);
+    dv3 Xn = Xnac - cc1, Xp = Xpac - cc0;
+    if (fabs(Xn.x) < 0.0000000000000003) Xn.x = 0.0;
+    if (fabs(Xn.y) < 0.0000000000000003) Xn.y = 0.0;
+    if (fabs(Xn.z) < 0.0000000000000003) Xn.z = 0.0;
+    if (fabs(Xp.x) < 0.0000000000000003) Xp.x = 0.0;
+    if (fabs(Xp.y) < 0.0000000000000003) Xp.y = 0.0;
+    if (fabs(Xp.z) < 0.0000000000000003) Xp.z = 0.0;
+    const double dpx = gPPx[c1] * Xn.x - gPPx[c0] * Xp.x;
+    const double dpy = gPPy[c1] * Xn.y - gPPy[c0] * Xp.y;
+    const double dpz = gPPz[c1] * Xn.z - gPPz[c0] * Xp.z;
+    const double fcv = fact * (dpx + dpy + dpz) * 0.8;
+    m += fcv;
+    mflux[f] = m;
+    fcorr[f] = fcv;
+}
+
+// cell correction (FlowSolver.inl:2416-2433), S_phi.U reset + gather of the
+// non-orthogonal correction (:2437-2444, 1875-1876), PP reset (:2447-2450).
+// which: 0 none, 1 cell 0 only (compat first loop), 2 every cell.
+__global__ void __launch_bounds__(256) k_cell_correct(Geo g, const double *fcorr, double *SU, double *P, double *PP,
+                                                      double *Uf, double *Vf, double *Wf, const double *gPPx,
+                                                      const double *gPPy, const double *gPPz, const double *AU,
+                                                      int which)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c < 6) PP[g.N + c] = 0.0;
+    if (c >= g.N) return;
+    double S = 0.0;
+    for (int k = 0; k < 6; k++) {
+        const int f = g.forder[6 * c + k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        if (c1 >= g.N) continue;
+        if (c == c0) S -= fcorr[f];
+        else S += fcorr[f];
+    }
+    SU[c] = S;
+    if (which == 2 || (which == 1 && c == 0)) {
+        const double Ar = (AU[c] != 0.0) ? 1.0 / AU[c] : 0.0;
+        const double fact = g.vol[c] * Ar;
+        P[c] += 0.2 * PP[c];
+        Uf[c] -= gPPx[c] * fact;
+        Vf[c] -= gPPy[c] * fact;
+        Wf[c] -= gPPz[c] * fact;
+    }
+    PP[c] = 0.0;
+}
+
+// Update_P_at_boundaries (FlowSolver.inl:2275-2290): slot <- last writer.
+__global__ void k_slots_copy(int N, int c0, int c1, int c2, int c3, int c4, int c5, double *phi)
+{
+    const int s = threadIdx.x;
+    if (s >= 6) return;
+    const int cells[6] = {c0, c1, c2, c3, c4, c5};
+    if (cells[s] >= 0) phi[N + s] = phi[cells[s]];
+}
+
+int pressure_boundary_slots(Ctx *c, int field)
+{
+    const int *lc = c->last_cell;
+    hipLaunchKernelGGL(k_slots_copy, dim3(1), dim3(64), 0, c->stream, c->N, lc[0], lc[1], lc[2], lc[3], lc[4], lc[5],
+                       c->phi[field]);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// update_P (FlowSolver.inl:2292-2315): outlet copy, else extrapolate.
+__global__ void k_update_P(Geo g, double *P, const double *gx, const double *gy, const double *gz)
+{
+    const int s = threadIdx.x;
+    if (s >= 6) return;
+    const int f = g.last_face[s];
+    if (f < 0) return;
+    const int c0 = g.fc0[f];
+    if (g.btype[s] == OUTLET) {
+        P[g.N + s] = P[c0];
+    } else {
+        const dv3 ds = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
+        P[g.N + s] = P[c0] + dot(ld3(gx, gy, gz, c0), ds);
+    }
+}
+
+int update_P(Ctx *c)
+{
+    hipLaunchKernelGGL(k_update_P, dim3(1), dim3(64), 0, c->stream, geo(c), c->phi[P], c->gx[P], c->gy[P], c->gz[P]);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+__global__ void k_copy(int n, const double *a, double *b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// calculate_pressure (FlowSolver.inl:2317-2463)
+int calculate_pressure(Ctx *c)
+{
+    const int N = c->N;
+    MC_TRY(calculate_mass_flux_assemble(c));
+    const bool simple = c->opt.mode == MC_MODE_SIMPLE;
+    // AMG values follow the matrix that will be solved
+    if (c->first_pressure || simple) {
+        if (!c->amg_ready) MC_TRY(amg_setup(c, c->PA_vals));
+        else MC_TRY(amg_update_values(c, c->PA_vals));
+    }
+    int loop = 0;
+    double pmax = 0.0, pref = 0.0;
+    double first_bnorm = 0.0;
+    for (;;) {
+        loop++;
+        // solve (compat: Pressure_b / Pressure_A are frozen after the first
+        // assembly, so every later solve reproduces the stored solution)
+        const bool fresh = simple || c->first_pressure || c->u == nullptr;
+        if (simple) {
+            MC_HIP(c, hipMemcpyAsync(c->Pb, c->SU, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (fresh || !c->st.p_solves) {
+            mc_solve_status st{};
+            // loops after the first solve only the O(1e-16) non-orthogonal
+            // remainder of update_mass_flux: their tolerance is an absolute
+            // floor relative to the first loop's RHS
+            const double atol = (loop == 1) ? 0.0 : c->opt.p_rtol * first_bnorm;
+            MC_TRY(pressure_solve(c, c->Pb, c->phi[PP], &st, atol));
+            if (loop == 1) first_bnorm = st.bnorm;
+            if (!simple) MC_HIP(c, hipMemcpyAsync(c->u, c->phi[PP], (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            MC_HIP(c, hipMemcpyAsync(c->phi[PP], c->u, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+        }
+        MC_TRY(reduce_maxabs_pp(c, c->phi[PP], &pmax));
+        if (loop == 1) pref = pmax;
+        MC_TRY(pressure_boundary_slots(c, PP));
+        MC_TRY(grad_single(c, PP, true));
+        {
+            KTimer kt(c, MC_KC_CORRECT, 0.0);
+            hipLaunchKernelGGL(k_flux_correct, dim3(div_up(c->F, 256)), dim3(256), 0, c->stream, geo(c), c->phi[PP],
+                               c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
+            const int which = simple ? 2 : (c->first_pressure ? 1 : 0);
+            hipLaunchKernelGGL(k_cell_correct, dim3(div_up(N, 256)), dim3(256), 0, c->stream, geo(c), c->fcorr, c->SU,
+                               c->phi[P], c->phi[PP], c->phi[U], c->phi[V], c->phi[W], c->gx[PP], c->gy[PP],
+                               c->gz[PP], c->AU, which);
+            MC_LAUNCH_CHECK(c);
+        }
+        c->first_pressure = false;
+        if (loop >= 4 || pmax <= 0.25 * pref) break;
+    }
+    c->st.p_loops += loop;
+    c->st.last_loops = loop;
+    c->st.last_p_max = pref;
+    MC_TRY(pressure_boundary_slots(c, P));
+    MC_TRY(grad_single(c, P, true));
+    MC_TRY(update_P(c));
+    return MC_OK;
+}
+
+// ============================================================= A16 ==
+struct ScalArgs {
+    double *phi;
+    const double *gx, *gy, *gz;
+    const double *U, *V, *W, *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz;
+    double *TE, *ED, *TP;
+    const double *mflux;
+    double *ff0, *ff1, *AV, *SU, *vals;
+    int type;
+    double ev, inlet_ev, rdelta, urf, inlet_phi, cmu75;
+};
+
+__device__ __forceinline__ double scalar_visac(int type, double ev, double Visac)
+{
+    if (type == TEMP) return (ev + Visac / 0.9) / 0.6905;
+    if (type == TERBTE) return ev + Visac;
+    if (type == TERBED) return ev + Visac / 1.219;
+    return (ev + Visac / 0.9) / 0.9;
+}
+
+// FluxScalar (FlowSolver.inl:2516-2744), the unsteady term (:2908-2916),
+// solveTurbulenceModels (:2746-2883) and setup_sparse_matrix (:1186-1297).
+__global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const int N = g.N;
+    const double ev = a.ev;
+    const double GammaBlend = 0.0;
+    const bool wallcell = g.is_wall[c] != 0;
+    const bool ed = a.type == TERBED;
+    double S = 0.0, A = 0.0;
+    double own_ff[6];
+    int own_nb[6];
+    int nint = 0;
+    int fo[6];
+    for (int k = 0; k < 6; k++) fo[k] = g.forder[6 * c + k];
+    for (int k = 0; k < 6; k++) {
+        const int f = fo[k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const double mf = a.mflux[f];
+        if (c1 < N) {
+            const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
+            double Visac = ev * lambda0 + ev * lambda1;
+            Visac -= ev;
+            Visac = scalar_visac(a.type, ev, Visac);
+            const dv3 dPhiXac = ld3(a.gx, a.gy, a.gz, c0) * lambda0 + ld3(a.gx, a.gy, a.gz, c1) * lambda1;
+            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const double VisFace = Visac * g.frlc[f];
+            const double p0 = a.phi[c0], p1 = a.phi[c1];
+            const double PhiFace = (mf >= 0.0) ? p0 : p1;
+            const double fce = mf * PhiFace;
+            const double fde1 = Visac * dot(dPhiXac, ld3(g.fnx, g.fny, g.fnz, f));
+            const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
+            const double fdi = VisFace * dot(dPhiXac, Xpn);
+            double ff0 = -VisFace - dmax(mf, 0.0);
+            double ff1 = -VisFace + dmin(mf, 0.0);
+            const double blend = GammaBlend * (fce - fci);
+            if (ed) {  // FlowSolver.inl:2855-2875: rows of wall cells lose their off-diagonals
+                if (g.is_wall[c0]) ff1 = 0.0;
+                if (g.is_wall[c1]) ff0 = 0.0;
+            }
+            if (c == c0) {
+                S = S - blend + fde1 - fdi;
+                own_ff[nint] = ff1; own_nb[nint] = c1;
+                a.ff0[f] = ff0; a.ff1[f] = ff1;
+            } else {
+                S = S + blend - fde1 + fdi;
+                own_ff[nint] = ff0; own_nb[nint] = c0;
+            }
+            nint++;
+        } else {
+            const int side = c1 - N;
+            const int bt = g.btype[side];
+            const bool last = (f == g.last_face[side]);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 gP = ld3(a.gx, a.gy, a.gz, c0);
+            if (bt == INLET) {
+                const double PhiFace = a.inlet_phi;
+                double Visac = a.inlet_ev;
+                Visac -= ev;
+                Visac = scalar_visac(a.type, ev, Visac);
+                const double VisFace = Visac * g.frlc[f];
+                const double fde = Visac * dot(gP, ld3(g.fnx, g.fny, g.fnz, f));
+                const double fdi = VisFace * dot(gP, Xpn);
+                const double fb = -VisFace + dmin(mf, 0.0);
+                A = A - fb;
+                S = S - fb * PhiFace + fde - fdi;
+                if (last) a.phi[N + side] = PhiFace;
+            } else if (bt == OUTLET) {
+                double Visac = ev;
+                Visac -= ev;
+                Visac = scalar_visac(a.type, ev, Visac);
+                const double PhiFace = a.phi[c0] + dot(gP, Xpn);
+                const double VisFace = Visac * g.frlc[f];
+                const double fde = Visac * dot(gP, ld3(g.fnx, g.fny, g.fnz, f));
+                const double fdi = VisFace * dot(gP, Xpn);
+                S = S + fde - fdi;
+                if (last) a.phi[N + side] = PhiFace;
+            } else if (bt == WALL) {
+                if (last && !ed) a.phi[N + side] = a.phi[c0];
+            }
+        }
+    }
+    const double vol = g.vol[c], rho = g.rho[c];
+    double phic = a.phi[c];
+    const double fv = rho * vol * a.rdelta;
+    S += fv * phic;
+    A += fv;
+    if (a.type == TERBTE) {
+        const dv3 dUdXp = ld3(a.gUx, a.gUy, a.gUz, c), dVdXp = ld3(a.gVx, a.gVy, a.gVz, c), dWdXp = ld3(a.gWx, a.gWy, a.gWz, c);
+        const double s1 = (dUdXp.x + dUdXp.x) * dUdXp.x + (dUdXp.y + dVdXp.x) * dUdXp.y + (dUdXp.z + dWdXp.x) * dUdXp.z;
+        const double s2 = (dVdXp.x + dUdXp.y) * dVdXp.x + (dVdXp.y + dVdXp.y) * dVdXp.y + (dVdXp.z + dWdXp.y) * dVdXp.z;
+        const double s3 = (dWdXp.x + dUdXp.z) * dWdXp.x + (dWdXp.y + dVdXp.z) * dWdXp.y + (dWdXp.z + dWdXp.z) * dWdXp.z;
+        const double VisT = ev - ev;
+        double TP = VisT * (s1 + s2 + s3);
+        const double Dis = rho * a.ED[c];
+        const double TEc = a.TE[c];
+        S = S + TP * vol;
+        A = A + Dis / (TEc + 0.000000000000000001) * vol;
+        if (wallcell) {
+            for (int k = 0; k < 6; k++) {
+                const int f = fo[k];
+                const int c1 = g.fc1[f];
+                if (c1 < N || g.btype[c1 - N] != WALL) continue;
+                S = S - TP * vol;
+                const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c);
+                const dv3 Up = v3(a.U[c] - 0., a.V[c] - 0., a.W[c] - 0.);
+                const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+                const dv3 nh = n / mag(n);
+                const double dp = dot(Up, nh);
+                const dv3 Ut = Up - nh * dp;
+                const double Uvel = sqrt(dot(Ut, Ut));
+                const double distance_to_face = mag(Xpn);
+                const double rkapdn = 1.0 / (0.419 * distance_to_face);
+                const double Tau_w = ev * Uvel / distance_to_face;
+                const double Utau = sqrt(Tau_w / rho);
+                TP = Tau_w * Utau * rkapdn;
+                if (f == g.last_face[c1 - N]) a.TE[N + c1 - N] = TEc;
+                S = S + TP * vol;
+                const double DisP = a.cmu75 * sqrt(TEc) * rkapdn;
+                A = A + rho * DisP * vol;
+            }
+        }
+        a.TP[c] = TP;
+    } else if (ed) {
+        const double TEc = a.TE[c];
+        const double fact = a.ED[c] / (TEc + 0.000000000000000001) * vol;
+        S = S + 1.44 * fact * a.TP[c];
+        A = A + 1.92 * fact * rho;
+        if (wallcell) {
+            for (int k = 0; k < 6; k++) {
+                const int f = fo[k];
+                const int c1 = g.fc1[f];
+                if (c1 < N || g.btype[c1 - N] != WALL) continue;
+                const double distance = mag(ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c));
+                const double Dis = a.cmu75 * pow(TEc, 1.5) / (distance * 0.419);
+                phic = Dis;
+                S = Dis;
+                A = 1;
+                if (f == g.last_face[c1 - N]) a.ED[N + c1 - N] = Dis;
+            }
+            a.ED[c] = phic;
+        }
+    }
+    for (int k = 0; k < nint; k++) A -= own_ff[k];
+    A *= (1. / a.urf);
+    S = S + (1.0 - a.urf) * A * phic;
+    a.AV[c] = A;
+    a.SU[c] = S;
+    int pos[6], dpos;
+    row_positions(c, nint, own_nb, pos, dpos);
+    const int rp = g.row_ptr[c];
+    a.vals[rp + dpos] = A;
+    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+}
+
+static int scalar_field(int type)
+{
+    static const int f[7] = {TE, ED, TEM, FUL, PRO, VARF, VARP};
+    return f[type];
+}
+
+int scalar_assemble(Ctx *c, int type)
+{
+    const int field = scalar_field(type);
+    ScalArgs a;
+    a.phi = c->phi[field];
+    a.gx = c->gx[field]; a.gy = c->gy[field]; a.gz = c->gz[field];
+    a.U = c->phi[U]; a.V = c->phi[V]; a.W = c->phi[W];
+    a.gUx = c->gx[U]; a.gUy = c->gy[U]; a.gUz = c->gz[U];
+    a.gVx = c->gx[V]; a.gVy = c->gy[V]; a.gVz = c->gz[V];
+    a.gWx = c->gx[W]; a.gWy = c->gy[W]; a.gWz = c->gz[W];
+    a.TE = c->phi[TE]; a.ED = c->phi[ED]; a.TP = c->phi[TP];
+    a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.SU = c->SU; a.vals = c->A_vals;
+    a.type = type;
+    a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta;
+    a.urf = (type == TERBTE || type == TERBED) ? 0.5 : 0.95;
+    a.cmu75 = c->k.cmu75;
+    switch (type) {
+    case TEMP: a.inlet_phi = c->k.gas_tem; break;
+    case TERBTE: a.inlet_phi = c->k.inlet_te; break;
+    case TERBED: a.inlet_phi = c->k.inlet_ed; break;
+    case FUEL: case VARFU: a.inlet_phi = c->k.gas_ful; break;
+    default: a.inlet_phi = 0.0; break;
+    }
+    KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
+    hipLaunchKernelGGL(k_scalar, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// Non-negativity repair of TE/ED (FlowSolver.inl:2956-2985).  The reference
+// sweeps cells in ascending order and reads already-repaired lower
+// neighbours; the fixed point below reproduces that order exactly.
+__global__ void __launch_bounds__(256) k_clip(Geo g, const double *orig, const double *cur, double *next,
+                                              int *changed)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const double v = orig[c];
+    double out = v;
+    if (v < 0.0) {
+        int count = 0;
+        double phisum = 0.0;
+        for (int j = 0; j < 6; j++) {
+            const int nb = g.nbr[6 * c + j];
+            if (nb < 0) continue;
+            count++;
+            phisum += (nb < c) ? cur[nb] : orig[nb];
+        }
+        phisum = (phisum / count);
+        out = dmax(phisum, 0.000000000001);
+    }
+    if (out != cur[c] && !(out != out && cur[c] != cur[c])) atomicAdd(changed, 1);
+    next[c] = out;
+}
+
+__global__ void __launch_bounds__(256) k_count_neg(int n, const double *v, int *cnt)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int neg = (i < n && v[i] < 0.0) ? 1 : 0;
+    const unsigned long long b = __ballot(neg);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, __popcll(b));
+}
+
+static int clip_nonneg(Ctx *c, double *phi)
+{
+    const int N = c->N;
+    int *cnt = (int *)(c->dscal + 700);
+    int *hcnt = (int *)(c->hscal + 700);
+    MC_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_count_neg, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, phi, cnt);
+    MC_HIP(c, hipMemcpyAsync(hcnt, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    if (*hcnt == 0) return MC_OK;
+    double *orig = c->wk[14], *a = c->wk[15], *b = phi;
+    MC_HIP(c, hipMemcpyAsync(orig, phi, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+    // cur starts as orig (= phi); iterate next = clip(cur) until fixed point
+    for (int pass = 0; pass < 10000; pass++) {
+        MC_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_clip, dim3(div_up(N, 256)), dim3(256), 0, c->stream, geo(c), orig, b, a, cnt);
+        MC_LAUNCH_CHECK(c);
+        MC_HIP(c, hipMemcpyAsync(b, a, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+        MC_HIP(c, hipMemcpyAsync(hcnt, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        MC_HIP(c, hipStreamSynchronize(c->stream));
+        if (*hcnt == 0) break;
+    }
+    return MC_OK;
+}
+
+int scalar_solve(Ctx *c, int type)
+{
+    const int field = scalar_field(type);
+    MC_TRY(scalar_assemble(c, type));
+    double *b[1] = {c->SU};
+    double *x[1] = {c->phi[field]};
+    mc_solve_status st{};
+    MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    c->st.scalar_iters += st.iterations;
+    if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
+    if (type == TERBTE || type == TERBED) MC_TRY(clip_nonneg(c, c->phi[field]));
+    return MC_OK;
+}
+
+// ============================================================= A17 ==
+int simple_iteration(Ctx *c, int inner)
+{
+    MC_TRY(grad_multi(c));
+    if (c->timestep_count == 0 && inner == 0) MC_TRY(set_up_field(c));
+    MC_TRY(momentum_assemble(c));
+    MC_TRY(momentum_solve(c));
+    MC_TRY(calculate_pressure(c));
+    if (c->opt.solve_scalars) {
+        for (int t = TERBTE; t <= VARPR; t++) MC_TRY(scalar_solve(c, t));
+    }
+    c->st.simple_iterations++;
+    c->global_iter++;
+    return MC_OK;
+}
+
+}  // namespace mc

@@ -1,0 +1,301 @@
+// mc_internal.hpp -- shared internals of the HIP flow-solver library.
+//
+// Device layout (one context = one GPU, one stream):
+//   * cells are numbered as the reference mesh generator numbers them
+//     (src/examples/mesh_examples.cpp:305-384: x fastest, one block);
+//   * every per-cell and per-face quantity is a separate SoA array so that a
+//     wavefront of 64 consecutive cells/faces issues fully coalesced loads;
+//   * vector quantities (centres, normals, gradients) are three SoA arrays;
+//   * cell arrays have N + 6 entries: the 6 boundary-side slots of the
+//     reference's phi_vector (FlowSolver.hpp:211-213) sit after the cells;
+//   * indices are int32 (N, F < 2^31 for every mesh that fits 288 GB).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mcflow.h"
+
+namespace mc {
+
+constexpr int NFIELDS = 13;
+enum Field { U = 0, V, W, P, PP, TE, ED, TP, TEM, FUL, PRO, VARF, VARP };
+enum ScalarType { TERBTE = 0, TERBED, TEMP, FUEL, PROG, VARFU, VARPR };
+enum BType { NOT_BOUNDARY = 0, WALL = 1, INLET = 2, OUTLET = 3 };
+
+// Physical constants fixed by the reference (Mesh.hpp:114-120,
+// FlowSolver.hpp:296-297,342).
+struct Consts {
+    double gas_vel_x, gas_vel_y, gas_vel_z, gas_pre, gas_tem, gas_te, gas_ed, gas_ful, gas_pro;
+    double ev;            // effective_viscosity
+    double inlet_ev;      // inlet_effective_viscosity (FlowSolver.hpp:421-422)
+    double rdelta;        // 1 / delta
+    double inlet_te;      // 3/2 (0.1^2 |U|^2)
+    double inlet_ed;      // 0.09^0.75 inlet_te^1.5
+    double cmu75;         // 0.09^0.75
+};
+
+// Per-level storage of the aggregation-AMG hierarchy.
+struct AmgLevel {
+    int n = 0, nnz = 0;
+    int *row_ptr = nullptr, *col = nullptr;   // CSR pattern of this level
+    double *vals = nullptr;                   // FP64 values
+    float *vals32 = nullptr;                  // FP32 copy (mixed form)
+    double *dinv = nullptr;                   // 1/diag
+    float *dinv32 = nullptr;
+    // aggregation to the next level
+    int nc = 0;
+    int *agg = nullptr;                       // [n] aggregate id
+    int *agg_ptr = nullptr, *agg_members = nullptr; // coarse row -> fine members
+    // Galerkin gather map for the next level's values: coarse nnz k sums
+    // this level's values gmap[gptr[k] .. gptr[k+1])
+    int *gptr = nullptr, *gmap = nullptr;
+    // vectors
+    double *x = nullptr, *b = nullptr, *t = nullptr;
+    float *x32 = nullptr, *b32 = nullptr, *t32 = nullptr;
+};
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    mc_options opt{};
+    Consts k{};
+    std::string err;
+
+    // ---- mesh ----
+    int64_t m = 0;              // box modifier (0 when the mesh came from the host)
+    int N = 0, F = 0, NP = 0, Fb = 0;
+    int *fc0 = nullptr, *fc1 = nullptr;     // [F] face cells; boundary: fc1 = N + side
+    int *cell_faces = nullptr;               // [6N] side order (FRONT..UP)
+    int *forder = nullptr;                   // [6N] the cell's faces in ascending face id
+    int *nbr = nullptr;                      // [6N] neighbour per side, -1 at the boundary
+    int *bfaces = nullptr;                   // [Fb] boundary faces, ascending
+    int btype[6] = {0, 0, 0, 0, 0, 0};
+    int last_face[6] = {-1, -1, -1, -1, -1, -1};   // highest face id per box side
+    int last_cell[6] = {-1, -1, -1, -1, -1, -1};
+    int wall_cells = 0;
+    uint8_t *is_wall_cell = nullptr;         // [N]
+    // kept for mesh download / geometry
+    double *px = nullptr, *py = nullptr, *pz = nullptr;   // [NP] points
+    uint64_t *cells = nullptr;               // [8N] vertex ids (freed after setup unless kept)
+    uint8_t *cells_per_point = nullptr;
+
+    // ---- geometry ----
+    double *cx = nullptr, *cy = nullptr, *cz = nullptr;        // [N] cell centres
+    double *fcx = nullptr, *fcy = nullptr, *fcz = nullptr;     // [F] face centres
+    double *fnx = nullptr, *fny = nullptr, *fnz = nullptr;     // [F] area normals
+    double *farea = nullptr, *flam = nullptr, *frlc = nullptr; // [F]
+    double *vol = nullptr, *rho = nullptr;                     // [N]
+
+    // ---- fields ----
+    double *phi[NFIELDS] = {};
+    double *gx[NFIELDS] = {}, *gy[NFIELDS] = {}, *gz[NFIELDS] = {};
+    double *AU = nullptr, *AV = nullptr, *AW = nullptr;
+    double *SU = nullptr, *SV = nullptr, *SW = nullptr;
+    double *ff0 = nullptr, *ff1 = nullptr, *mflux = nullptr, *fcorr = nullptr;
+
+    // ---- CSR (PETSc SeqAIJ pattern) ----
+    int nnz = 0;
+    int *row_ptr = nullptr, *col = nullptr, *diag_idx = nullptr;
+    double *A_vals = nullptr;      // momentum / scalar matrix
+    double *PA_vals = nullptr;     // pressure-correction matrix
+    float *PA_vals32 = nullptr;
+    double *Pb = nullptr;          // pressure RHS (compat: frozen copy)
+    double *u = nullptr;           // solution scratch
+    bool first_pressure = true;
+    int64_t timestep_count = 0;
+    int64_t global_iter = 0;
+
+    // ---- solver workspace ----
+    double *wk[16] = {};           // N-length work vectors
+    float *wk32[8] = {};
+    double *red = nullptr;         // reduction partials
+    unsigned *red_cnt = nullptr;   // last-block counters
+    double *dscal = nullptr;       // device scalars for the solvers
+    double *hscal = nullptr;       // pinned host mirror
+    int red_blocks = 0;
+
+    // ---- AMG ----
+    std::vector<AmgLevel> amg;
+    bool amg_ready = false;
+    int amg_coarse_n = 0;
+    double *amg_dense = nullptr;   // [n*n] coarsest (regularised) inverse
+    double *amg_work = nullptr;
+
+    // ---- learned initial guess ----
+    void *mpnn = nullptr;
+
+    // ---- device allocations owned by the context ----
+    std::vector<void *> allocs;
+
+    // ---- stats / timing ----
+    mc_stats st{};
+    bool profile = false;
+    hipEvent_t ev[64] = {};
+    int n_ev = 0;
+    double kt_ms[MC_KC_COUNT] = {};      // accumulated kernel-class time
+    int64_t kt_calls[MC_KC_COUNT] = {};
+    double kt_bytes[MC_KC_COUNT] = {};   // accumulated algorithmic bytes
+};
+
+// ---------------------------------------------------------------- errors --
+#define MC_HIP(ctx, call)                                                            \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return MC_ERR_HIP;                                                       \
+        }                                                                            \
+    } while (0)
+
+#define MC_TRY(expr)                    \
+    do {                                        \
+        int r_ = (expr);                        \
+        if (r_ != MC_OK) return r_;             \
+    } while (0)
+
+#define MC_LAUNCH_CHECK(ctx) MC_HIP(ctx, hipGetLastError())
+
+// ---------------------------------------------------------- device utils --
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup i -> XCD i % 8), and each XCD has its own L2.  Remap so that
+// every XCD processes one contiguous range of cells: neighbours at +-1 and
+// +-nx then sit in the same L2 as the cell that gathers them.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks)
+{
+    constexpr int NXCD = 8;
+    const int per = nblocks / NXCD, rem = nblocks % NXCD;
+    const int x = bid % NXCD, local = bid / NXCD;
+    return x * per + (x < rem ? x : rem) + local;
+}
+
+// std::min / std::max semantics (FlowSolver.inl uses them on doubles).
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Block-wide sum of NV values (blockDim.x == 256), result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*4 */)
+{
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        v[k] = x;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) lds[k * 4 + wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = ((lds[k * 4 + 0] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+    }
+}
+
+// Deterministic two-level reduction: every block stores NV partials, the
+// last block to finish (vector atomic on a global counter) sums them in a
+// fixed order and returns true in thread 0 with the totals in v.
+template <int NV>
+__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double *partials, unsigned *counter,
+                                                  int bid, int nblocks)
+{
+    __shared__ double lds[NV * 4];
+    __shared__ bool am_last;
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) partials[(size_t)k * nblocks + bid] = v[k];
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        am_last = (prev == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
+            s += __builtin_nontemporal_load(&partials[(size_t)k * nblocks + i]);
+        acc[k] = s;
+    }
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = acc[k];
+        *counter = 0u;  // re-arm for the next launch (stream order)
+    }
+    return threadIdx.x == 0;
+}
+
+// ------------------------------------------------------------ host utils --
+int alloc(Ctx *c, void **p, size_t bytes);
+template <typename T>
+inline int dalloc(Ctx *c, T **p, size_t count)
+{
+    return alloc(c, (void **)p, count * sizeof(T) + 16);
+}
+int dfree_all(Ctx *c);
+
+// profiling of kernel classes with HIP events on the context stream
+struct KTimer {
+    Ctx *c;
+    int kc;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(Ctx *c_, int kc_, double bytes_);
+    ~KTimer();
+};
+void profile_flush(Ctx *c);
+
+// stages (mc_mesh.hip)
+int mesh_box_generate(Ctx *c, int64_t m);
+int mesh_from_host(Ctx *c, const mc_mesh_desc *d);
+int mesh_finish(Ctx *c);   // geometry, init, CSR pattern, face orders
+
+// stages (mc_flow.hip)
+int grad_multi(Ctx *c);                                  // 11 fields
+int grad_single(Ctx *c, int field, bool pressure);
+int set_up_field(Ctx *c);
+int momentum_assemble(Ctx *c);
+int momentum_solve(Ctx *c);
+int calculate_mass_flux_assemble(Ctx *c);               // mass flux + pressure matrix
+int calculate_pressure(Ctx *c);
+int scalar_assemble(Ctx *c, int type);
+int scalar_solve(Ctx *c, int type);
+int simple_iteration(Ctx *c, int inner);
+int pressure_boundary_slots(Ctx *c, int field);
+int update_P(Ctx *c);
+int precomp_AU(Ctx *c);
+int boundary_mass_flux(Ctx *c);
+
+// solvers (mc_linalg.hip)
+int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x,
+                   double rtol, int maxit, mc_solve_status *st);
+int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol = 0.0);
+int amg_setup(Ctx *c, const double *vals);
+int amg_update_values(Ctx *c, const double *vals);
+int spmv(Ctx *c, const double *vals, const double *x, double *y);
+int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
+            double atol, mc_solve_status *st);
+int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
+                   double atol, mc_solve_status *st);
+int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host);
+int mean_project(Ctx *c, const double *b, double *out, int n);
+
+// learned initial guess (mc_mpnn.hip)
+int mpnn_guess(Ctx *c, const double *b, double *x0, int *ok);
+void mpnn_destroy(Ctx *c);
+
+}  // namespace mc

@@ -1,0 +1,1209 @@
+// mc_linalg.hip -- Krylov solvers and the aggregation-AMG hierarchy.
+//
+// Replaces the AmgX solves of the reference GPU path (gpu/FlowSolver.inl:
+// 716-782, 943-969) and PETSc's KSPBCGS / KSPFGMRES+PCGAMG of the CPU path
+// (FlowSolver.hpp:513-527, FlowSolver.inl:1341-1375, 2203-2273).
+//
+//  * SpMV: CSR, 8 lanes per row (one wavefront = 8 rows), the row sum is
+//    reduced with 3 cross-lane shuffles; matrix values and columns stream
+//    once per call, coalesced.  NR right-hand sides share one matrix read.
+//  * Reductions: fixed-order two-level block sums finished by the last block
+//    (deterministic, no host round trip).  Solver scalars live in device
+//    memory; kernels of an already-converged system return at once, so the
+//    host only polls a flag every few iterations.
+//  * BiCGStab: van der Vorst's unpreconditioned method, x0 = 0 (momentum and
+//    scalar equations; the reference's KSPBCGS+PCNONE).
+//  * Pressure: PCG preconditioned by an unsmoothed-aggregation AMG V-cycle
+//    (damped Jacobi, symmetric nu/nu sweeps, scaled coarse correction, dense
+//    regularised inverse on the coarsest level), on the singular Neumann
+//    system with the RHS projected to mean zero; mixed form: FP32 PCG-AMG
+//    inside FP64 iterative refinement.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "mc_internal.hpp"
+
+namespace mc {
+
+// ------------------------------------------------------------ layout --
+constexpr int RED_MAX_BLOCKS = 2048;
+constexpr int ROWS_PER_WAVE = 8;               // 8 lanes per row
+constexpr int BLOCK = 256;
+constexpr int ROWS_PER_BLOCK = BLOCK / 8;      // 32
+
+struct Grid {
+    int blocks;
+    int chunk;   // rows (or elements) per block, multiple of the granule
+};
+
+static Grid grid_for(int n, int granule)
+{
+    const int64_t g = div_up(n, granule);
+    int blocks = (int)std::min<int64_t>(g, RED_MAX_BLOCKS);
+    if (blocks < 1) blocks = 1;
+    const int64_t per = div_up(g, blocks);
+    Grid gr;
+    gr.chunk = (int)(per * granule);
+    gr.blocks = div_up(n, gr.chunk);
+    if (gr.blocks < 1) gr.blocks = 1;
+    return gr;
+}
+
+// Device scalars of one Krylov solve (NR <= 4 systems).
+struct KState {
+    double rho_old[4], alpha[4], omega[4], rho[4], beta[4];
+    double tol[4], bnorm[4], rnorm[4], sn[4], mean;
+    double atol;
+    int done[4], half[4], iters[4], conv[4];
+    int all_done, nr, maxit, pad;
+};
+
+template <int NR>
+__device__ __forceinline__ bool all_done_dev(const KState *ks)
+{
+    return ks->all_done != 0;
+}
+
+// ------------------------------------------------------------ SpMV --
+// y_k[r] = sum_j A[r,j] x_k[j] for the rows of one 8-lane group.
+template <int NR, typename T, typename VT>
+__device__ __forceinline__ void csr_row8(int r, int n, const int *__restrict__ rp, const int *__restrict__ col,
+                                         const T *__restrict__ vals, const VT *const *x, double (&acc)[NR])
+{
+    const int lane8 = threadIdx.x & 7;
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    if (r < n) {
+        const int beg = rp[r], end = rp[r + 1];
+        for (int e = beg + lane8; e < end; e += 8) {
+            const int j = col[e];
+            const double a = (double)vals[e];
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] += a * (double)x[k][j];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+        double v = acc[k];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        acc[k] = v;
+    }
+}
+
+template <typename T, typename VT>
+__global__ void __launch_bounds__(BLOCK) k_spmv(int n, const int *rp, const int *col, const T *vals, const VT *x,
+                                                VT *y, int chunk)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
+    const VT *xs[1] = {x};
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc[1];
+        csr_row8<1, T, VT>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
+        if ((threadIdx.x & 7) == 0 && r < r1) y[r] = (VT)acc[0];
+    }
+}
+
+int spmv(Ctx *c, const double *vals, const double *x, double *y)
+{
+    const Grid g = grid_for(c->N, ROWS_PER_BLOCK);
+    hipLaunchKernelGGL((k_spmv<double, double>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, c->row_ptr, c->col,
+                       vals, x, y, g.chunk);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// ------------------------------------------------------- BiCGStab --
+struct BiVecs {
+    double *x[4], *r[4], *rh[4], *p[4], *v[4], *s[4], *t[4];
+    const double *b[4];
+};
+
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt,
+                                                   int chunk, double rtol, int maxit)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const double bi = w.b[k][i];
+            w.x[k][i] = 0.0; w.r[k][i] = bi; w.rh[k][i] = bi; w.p[k][i] = 0.0; w.v[k][i] = 0.0;
+            acc[k] += bi * bi;
+        }
+    }
+    if (block_reduce_last<NR>(acc, partials, cnt, b, gridDim.x)) {
+        int all = 1;
+        for (int k = 0; k < NR; k++) {
+            const double bn = sqrt(acc[k]);
+            ks->bnorm[k] = bn; ks->tol[k] = rtol * bn; ks->rnorm[k] = bn;
+            ks->rho_old[k] = 1.0; ks->alpha[k] = 1.0; ks->omega[k] = 1.0;
+            ks->rho[k] = acc[k];  // (rh, r) with rh = r = b
+            ks->beta[k] = (ks->rho[k] / 1.0) * (1.0 / 1.0);
+            ks->iters[k] = 0; ks->half[k] = 0;
+            ks->done[k] = (bn == 0.0) ? 1 : 0;
+            ks->conv[k] = (bn == 0.0) ? 1 : 0;
+            all &= ks->done[k];
+        }
+        ks->nr = NR; ks->maxit = maxit;
+        ks->all_done = all;
+    }
+}
+
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_p(int n, BiVecs w, const KState *ks, int chunk)
+{
+    if (ks->all_done) return;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k]) continue;
+            const double beta = ks->beta[k], om = ks->omega[k];
+            w.p[k][i] = w.r[k][i] + beta * (w.p[k][i] - om * w.v[k][i]);
+        }
+    }
+}
+
+// v = A p ; alpha = rho / (rh, v)
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_spmv_v(int n, const int *rp, const int *col, const double *vals, BiVecs w,
+                                                     KState *ks, double *partials, unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
+    double dsum[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) dsum[k] = 0.0;
+    const double *xs[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) xs[k] = w.p[k];
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc[NR];
+        csr_row8<NR, double, double>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
+        if ((threadIdx.x & 7) == 0 && r < r1) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                if (ks->done[k]) continue;
+                w.v[k][r] = acc[k];
+                dsum[k] += w.rh[k][r] * acc[k];
+            }
+        }
+    }
+    if (block_reduce_last<NR>(dsum, partials, cnt, b, gridDim.x)) {
+        for (int k = 0; k < NR; k++)
+            if (!ks->done[k]) ks->alpha[k] = ks->rho[k] / dsum[k];
+    }
+}
+
+// s = r - alpha v ; half-step convergence test on ||s||
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_s(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k]) continue;
+            const double s = w.r[k][i] - ks->alpha[k] * w.v[k][i];
+            w.s[k][i] = s;
+            acc[k] += s * s;
+        }
+    }
+    if (block_reduce_last<NR>(acc, partials, cnt, b, gridDim.x)) {
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k]) continue;
+            const double sn = sqrt(acc[k]);
+            ks->sn[k] = sn;
+            ks->half[k] = (sn <= ks->tol[k]) ? 1 : 0;
+        }
+    }
+}
+
+// t = A s ; omega = (t,s)/(t,t)
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_spmv_t(int n, const int *rp, const int *col, const double *vals, BiVecs w,
+                                                     KState *ks, double *partials, unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
+    double dsum[2 * NR];
+#pragma unroll
+    for (int k = 0; k < 2 * NR; k++) dsum[k] = 0.0;
+    const double *xs[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) xs[k] = w.s[k];
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc[NR];
+        csr_row8<NR, double, double>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
+        if ((threadIdx.x & 7) == 0 && r < r1) {
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                if (ks->done[k] || ks->half[k]) continue;
+                w.t[k][r] = acc[k];
+                dsum[2 * k] += acc[k] * w.s[k][r];
+                dsum[2 * k + 1] += acc[k] * acc[k];
+            }
+        }
+    }
+    if (block_reduce_last<2 * NR>(dsum, partials, cnt, b, gridDim.x)) {
+        for (int k = 0; k < NR; k++)
+            if (!ks->done[k] && !ks->half[k]) ks->omega[k] = dsum[2 * k] / dsum[2 * k + 1];
+    }
+}
+
+// x += alpha p + omega s ; r = s - omega t ; next rho ; convergence
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_xr(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    double acc[2 * NR];
+#pragma unroll
+    for (int k = 0; k < 2 * NR; k++) acc[k] = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k]) continue;
+            const double al = ks->alpha[k];
+            if (ks->half[k]) {
+                w.x[k][i] += al * w.p[k][i];
+            } else {
+                const double om = ks->omega[k];
+                const double s = w.s[k][i];
+                w.x[k][i] += al * w.p[k][i] + om * s;
+                const double r = s - om * w.t[k][i];
+                w.r[k][i] = r;
+                acc[2 * k] += r * r;
+                acc[2 * k + 1] += w.rh[k][i] * r;
+            }
+        }
+    }
+    if (block_reduce_last<2 * NR>(acc, partials, cnt, b, gridDim.x)) {
+        int all = 1;
+        for (int k = 0; k < NR; k++) {
+            if (!ks->done[k]) {
+                ks->iters[k] += 1;
+                if (ks->half[k]) {
+                    ks->rnorm[k] = ks->sn[k];
+                    ks->done[k] = 1; ks->conv[k] = 1;
+                } else {
+                    const double rn = sqrt(acc[2 * k]);
+                    ks->rnorm[k] = rn;
+                    ks->rho_old[k] = ks->rho[k];
+                    ks->rho[k] = acc[2 * k + 1];
+                    if (rn <= ks->tol[k]) { ks->done[k] = 1; ks->conv[k] = 1; }
+                    else if (ks->rho[k] == 0.0 || ks->iters[k] >= ks->maxit) ks->done[k] = 1;
+                    else ks->beta[k] = (ks->rho[k] / ks->rho_old[k]) * (ks->alpha[k] / ks->omega[k]);
+                }
+            }
+            all &= ks->done[k];
+        }
+        ks->all_done = all;
+    }
+}
+
+static int poll_done(Ctx *c, KState *ks, KState *host)
+{
+    MC_HIP(c, hipMemcpyAsync(host, ks, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
+template <int NR>
+static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *const *x, double rtol, int maxit,
+                       mc_solve_status *st)
+{
+    const int n = c->N;
+    BiVecs w{};
+    for (int k = 0; k < NR; k++) {
+        w.b[k] = b[k]; w.x[k] = x[k];
+        w.r[k] = c->wk[k * 6 + 0 < 16 ? 0 : 0];
+    }
+    // work vectors: 6 per RHS from c->wk (16 available) plus the aux pool
+    static_assert(NR <= 4, "NR");
+    double *pool[24];
+    int np = 0;
+    for (int i = 0; i < 16; i++) pool[np++] = c->wk[i];
+    for (int i = 0; i < 8 && np < 24; i++) pool[np++] = (double *)c->wk32[i];  // wk32 sized as doubles
+    int q = 0;
+    for (int k = 0; k < NR; k++) {
+        w.r[k] = pool[q++]; w.rh[k] = pool[q++]; w.p[k] = pool[q++];
+        w.v[k] = pool[q++]; w.s[k] = pool[q++]; w.t[k] = pool[q++];
+    }
+    KState *ks = (KState *)c->dscal;
+    KState *hs = (KState *)c->hscal;
+    const Grid gv = grid_for(n, BLOCK);
+    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
+    hipLaunchKernelGGL(k_bi_init<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                       gv.chunk, rtol, maxit);
+    MC_LAUNCH_CHECK(c);
+    const int check = std::max(1, c->opt.check_every);
+    int it = 0;
+    for (;;) {
+        for (int j = 0; j < check && it < maxit; j++, it++) {
+            hipLaunchKernelGGL(k_bi_p<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, gv.chunk);
+            {
+                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 20.0 * n * NR);
+                hipLaunchKernelGGL(k_bi_spmv_v<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
+                                   vals, w, ks, c->red, c->red_cnt, gm.chunk);
+            }
+            hipLaunchKernelGGL(k_bi_s<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               gv.chunk);
+            {
+                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 20.0 * n * NR);
+                hipLaunchKernelGGL(k_bi_spmv_t<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
+                                   vals, w, ks, c->red, c->red_cnt, gm.chunk);
+            }
+            hipLaunchKernelGGL(k_bi_xr<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               gv.chunk);
+        }
+        MC_LAUNCH_CHECK(c);
+        MC_TRY(poll_done(c, ks, hs));
+        if (hs->all_done || it >= maxit) break;
+    }
+    if (st) {
+        st->iterations = 0; st->converged = 1; st->relres = 0.0; st->outer = 0; st->guess_used = 0;
+        for (int k = 0; k < NR; k++) {
+            st->iterations += hs->iters[k];
+            st->converged &= hs->conv[k];
+            const double rr = hs->bnorm[k] > 0 ? hs->rnorm[k] / hs->bnorm[k] : 0.0;
+            st->relres = std::max(st->relres, rr);
+        }
+    }
+    return MC_OK;
+}
+
+int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x, double rtol, int maxit,
+                   mc_solve_status *st)
+{
+    switch (nr) {
+    case 1: return bicgstab_nr<1>(c, vals, b, x, rtol, maxit, st);
+    case 2: return bicgstab_nr<2>(c, vals, b, x, rtol, maxit, st);
+    case 3: return bicgstab_nr<3>(c, vals, b, x, rtol, maxit, st);
+    case 4: return bicgstab_nr<4>(c, vals, b, x, rtol, maxit, st);
+    default: c->err = "bicgstab: 1..4 right-hand sides"; return MC_ERR_ARG;
+    }
+}
+
+// ------------------------------------------------- generic reductions --
+__global__ void __launch_bounds__(BLOCK) k_sum(int n, const double *v, double *out, double *partials, unsigned *cnt,
+                                               int chunk, double scale)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    double acc[1] = {0.0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += v[i];
+    if (block_reduce_last<1>(acc, partials, cnt, b, gridDim.x)) *out = acc[0] * scale;
+}
+
+__global__ void k_sub_scalar(int n, const double *in, const double *s, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] - *s;
+}
+
+int mean_project(Ctx *c, const double *b, double *out, int n)
+{
+    const Grid g = grid_for(n, BLOCK);
+    double *mean = c->dscal + 512;
+    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 8, g.chunk,
+                       1.0 / (double)n);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n, b, mean, out);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// max(PP[0], max_{i>=1} |PP[i]|), the reference's Pressure_correction_max
+// (FlowSolver.inl:2371-2378)
+__global__ void __launch_bounds__(BLOCK) k_maxabs(int n, const double *v, double *out, double *partials, unsigned *cnt,
+                                                  int chunk)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+    double m = 0.0;
+    for (int i = max(i0, 1) + threadIdx.x; i < i1; i += BLOCK) m = fmax(m, fabs(v[i]));
+    __shared__ double lds[BLOCK / 64];
+    __shared__ bool am_last;
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+        partials[b] = m;
+        __threadfence();
+        am_last = atomicAdd(cnt, 1u) == (unsigned)(gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double mm = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK) mm = fmax(mm, __builtin_nontemporal_load(&partials[i]));
+    for (int off = 32; off > 0; off >>= 1) mm = fmax(mm, __shfl_down(mm, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mm = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+        const double p0 = v[0];
+        *out = (p0 != p0) ? p0 : ((mm > p0) ? mm : p0);
+        *cnt = 0u;
+    }
+}
+
+int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host)
+{
+    const Grid g = grid_for(c->N, BLOCK);
+    double *d = c->dscal + 520;
+    hipLaunchKernelGGL(k_maxabs, dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, v, d, c->red, c->red_cnt + 9, g.chunk);
+    MC_LAUNCH_CHECK(c);
+    MC_HIP(c, hipMemcpyAsync(c->hscal + 520, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    *out_host = c->hscal[520];
+    return MC_OK;
+}
+
+// =================================================================== AMG ==
+// Host-side aggregation: three passes of greedy pairwise matching (strongest
+// unmatched neighbour, ties to the lowest column) per level -> aggregates of
+// about 8 cells.  Done once per context; later matrices on the same pattern
+// only recompute the Galerkin values on the GPU.
+struct HostCSR {
+    int n = 0;
+    std::vector<int> rp, col;
+    std::vector<double> val;
+};
+
+static void pairwise_pass(const HostCSR &A, std::vector<int> &agg, int &nc)
+{
+    agg.assign(A.n, -1);
+    nc = 0;
+    for (int i = 0; i < A.n; i++) {
+        if (agg[i] >= 0) continue;
+        int best = -1;
+        double bw = 0.0;
+        for (int k = A.rp[i]; k < A.rp[i + 1]; k++) {
+            const int j = A.col[k];
+            if (j == i || agg[j] >= 0) continue;
+            const double w = -A.val[k];
+            if (w > bw * (1.0 + 1e-12)) { best = j; bw = w; }
+        }
+        agg[i] = nc;
+        if (best >= 0) agg[best] = nc;
+        nc++;
+    }
+}
+
+// Galerkin product with piecewise-constant prolongation; also returns, for
+// every coarse nonzero, the list of fine nonzeros summed into it.
+static void galerkin(const HostCSR &A, const std::vector<int> &agg, int nc, HostCSR &Ac, std::vector<int> *gptr,
+                     std::vector<int> *gmap)
+{
+    // members of each aggregate
+    std::vector<int> mp(nc + 1, 0), mem(A.n);
+    for (int i = 0; i < A.n; i++) mp[agg[i] + 1]++;
+    for (int I = 0; I < nc; I++) mp[I + 1] += mp[I];
+    {
+        std::vector<int> pos(mp.begin(), mp.end() - 1);
+        for (int i = 0; i < A.n; i++) mem[pos[agg[i]]++] = i;
+    }
+    Ac.n = nc;
+    Ac.rp.assign(nc + 1, 0);
+    Ac.col.clear();
+    Ac.val.clear();
+    std::vector<std::pair<int, int>> ent;  // (coarse col, fine nnz)
+    std::vector<int> gp, gm;
+    gp.push_back(0);
+    for (int I = 0; I < nc; I++) {
+        ent.clear();
+        for (int t = mp[I]; t < mp[I + 1]; t++) {
+            const int i = mem[t];
+            for (int k = A.rp[i]; k < A.rp[i + 1]; k++) ent.emplace_back(agg[A.col[k]], k);
+        }
+        std::sort(ent.begin(), ent.end());
+        for (size_t e = 0; e < ent.size();) {
+            const int J = ent[e].first;
+            double s = 0.0;
+            size_t f = e;
+            for (; f < ent.size() && ent[f].first == J; f++) {
+                s += A.val[ent[f].second];
+                gm.push_back(ent[f].second);
+            }
+            Ac.col.push_back(J);
+            Ac.val.push_back(s);
+            gp.push_back((int)gm.size());
+            e = f;
+        }
+        Ac.rp[I + 1] = (int)Ac.col.size();
+    }
+    if (gptr) *gptr = std::move(gp);
+    if (gmap) *gmap = std::move(gm);
+}
+
+__global__ void k_gather_sum(int nnzc, const int *gptr, const int *gmap, const double *vf, double *vc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnzc) return;
+    double s = 0.0;
+    for (int t = gptr[k]; t < gptr[k + 1]; t++) s += vf[gmap[t]];
+    vc[k] = s;
+}
+
+__global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v, double *dinv, float *dinv32,
+                           float *v32, int nnz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        double d = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) d = v[k];
+        const double di = (d != 0.0) ? 1.0 / d : 0.0;
+        dinv[i] = di;
+        dinv32[i] = (float)di;
+    }
+    for (int k = i; k < nnz; k += gridDim.x * blockDim.x) v32[k] = (float)v[k];
+}
+
+// Dense regularised coarsest operator A + alpha 11^T (alpha = mean diag / n):
+// for a mean-zero right-hand side its inverse returns the mean-zero solution
+// of the singular Neumann system.  Gauss-Jordan in one workgroup.
+__global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, const int *col, const double *v,
+                                                        double *M /* n x 2n work */, double *inv)
+{
+    __shared__ double alpha_s;
+    __shared__ double piv_s;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) s += v[k];
+        alpha_s = (s / n) / n;
+    }
+    __syncthreads();
+    const int w = 2 * n;
+    for (int e = tid; e < n * w; e += blockDim.x) {
+        const int i = e / w, j = e % w;
+        M[e] = (j < n) ? alpha_s : ((j - n == i) ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x)
+        for (int k = rp[i]; k < rp[i + 1]; k++) M[(size_t)i * w + col[k]] += v[k];
+    __syncthreads();
+    for (int p = 0; p < n; p++) {
+        if (tid == 0) piv_s = 1.0 / M[(size_t)p * w + p];
+        __syncthreads();
+        for (int j = tid; j < w; j += blockDim.x) M[(size_t)p * w + j] *= piv_s;
+        __syncthreads();
+        for (int e = tid; e < n * w; e += blockDim.x) {
+            const int i = e / w, j = e % w;
+            if (i == p) continue;
+            const double f = M[(size_t)i * w + p];
+            if (j == p) continue;
+            M[e] -= f * M[(size_t)p * w + j];
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += blockDim.x) if (i != p) M[(size_t)i * w + p] = 0.0;
+        __syncthreads();
+    }
+    for (int e = tid; e < n * n; e += blockDim.x) {
+        const int i = e / n, j = e % n;
+        inv[e] = M[(size_t)i * w + n + j];
+    }
+}
+
+int amg_setup(Ctx *c, const double *vals)
+{
+    // download the fine matrix
+    HostCSR A;
+    A.n = c->N;
+    A.rp.resize(c->N + 1);
+    A.col.resize(c->nnz);
+    A.val.resize(c->nnz);
+    MC_HIP(c, hipMemcpyAsync(A.rp.data(), c->row_ptr, (size_t)(c->N + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipMemcpyAsync(A.col.data(), c->col, (size_t)c->nnz * 4, hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipMemcpyAsync(A.val.data(), vals, (size_t)c->nnz * 8, hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    c->amg.clear();
+    const int coarse_max = std::max(8, c->opt.amg_coarse_max);
+    HostCSR cur = std::move(A);
+    bool first = true;
+    for (;;) {
+        AmgLevel L;
+        L.n = cur.n;
+        L.nnz = (int)cur.col.size();
+        if (first) {
+            L.row_ptr = c->row_ptr; L.col = c->col;
+        } else {
+            MC_TRY(dalloc(c, &L.row_ptr, L.n + 1));
+            MC_TRY(dalloc(c, &L.col, L.nnz));
+            MC_HIP(c, hipMemcpy(L.row_ptr, cur.rp.data(), (size_t)(L.n + 1) * 4, hipMemcpyHostToDevice));
+            MC_HIP(c, hipMemcpy(L.col, cur.col.data(), (size_t)L.nnz * 4, hipMemcpyHostToDevice));
+        }
+        MC_TRY(dalloc(c, &L.vals, L.nnz));
+        MC_TRY(dalloc(c, &L.vals32, L.nnz));
+        MC_TRY(dalloc(c, &L.dinv, L.n));
+        MC_TRY(dalloc(c, &L.dinv32, L.n));
+        MC_TRY(dalloc(c, &L.x, L.n)); MC_TRY(dalloc(c, &L.b, L.n)); MC_TRY(dalloc(c, &L.t, L.n));
+        MC_TRY(dalloc(c, &L.x32, L.n)); MC_TRY(dalloc(c, &L.b32, L.n)); MC_TRY(dalloc(c, &L.t32, L.n));
+        if (cur.n <= coarse_max) {
+            c->amg.push_back(L);
+            break;
+        }
+        // three pairwise passes -> level aggregation
+        std::vector<int> agg(cur.n), a1;
+        std::iota(agg.begin(), agg.end(), 0);
+        HostCSR g = cur;  // graph being matched
+        int nc = cur.n;
+        for (int pass = 0; pass < 3; pass++) {
+            int ncn = 0;
+            pairwise_pass(g, a1, ncn);
+            for (int i = 0; i < cur.n; i++) agg[i] = a1[agg[i]];
+            HostCSR gn;
+            galerkin(g, a1, ncn, gn, nullptr, nullptr);
+            g = std::move(gn);
+            nc = ncn;
+            if (nc <= coarse_max) break;
+        }
+        HostCSR Ac;
+        std::vector<int> gp, gm;
+        galerkin(cur, agg, nc, Ac, &gp, &gm);
+        L.nc = nc;
+        MC_TRY(dalloc(c, &L.agg, L.n));
+        MC_HIP(c, hipMemcpy(L.agg, agg.data(), (size_t)L.n * 4, hipMemcpyHostToDevice));
+        {
+            std::vector<int> mp(nc + 1, 0), mem(L.n);
+            for (int i = 0; i < L.n; i++) mp[agg[i] + 1]++;
+            for (int I = 0; I < nc; I++) mp[I + 1] += mp[I];
+            std::vector<int> pos(mp.begin(), mp.end() - 1);
+            for (int i = 0; i < L.n; i++) mem[pos[agg[i]]++] = i;
+            MC_TRY(dalloc(c, &L.agg_ptr, nc + 1));
+            MC_TRY(dalloc(c, &L.agg_members, L.n));
+            MC_HIP(c, hipMemcpy(L.agg_ptr, mp.data(), (size_t)(nc + 1) * 4, hipMemcpyHostToDevice));
+            MC_HIP(c, hipMemcpy(L.agg_members, mem.data(), (size_t)L.n * 4, hipMemcpyHostToDevice));
+        }
+        MC_TRY(dalloc(c, &L.gptr, gp.size()));
+        MC_TRY(dalloc(c, &L.gmap, gm.size()));
+        MC_HIP(c, hipMemcpy(L.gptr, gp.data(), gp.size() * 4, hipMemcpyHostToDevice));
+        MC_HIP(c, hipMemcpy(L.gmap, gm.data(), gm.size() * 4, hipMemcpyHostToDevice));
+        c->amg.push_back(L);
+        cur = std::move(Ac);
+        first = false;
+    }
+    const int nco = c->amg.back().n;
+    c->amg_coarse_n = nco;
+    MC_TRY(dalloc(c, &c->amg_dense, (size_t)nco * nco));
+    MC_TRY(dalloc(c, &c->amg_work, (size_t)nco * 2 * nco));
+    c->amg_ready = true;
+    return amg_update_values(c, vals);
+}
+
+int amg_update_values(Ctx *c, const double *vals)
+{
+    if (!c->amg_ready) return amg_setup(c, vals);
+    const double *vf = vals;
+    for (size_t l = 0; l < c->amg.size(); l++) {
+        AmgLevel &L = c->amg[l];
+        MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col, L.vals,
+                           L.dinv, L.dinv32, L.vals32, L.nnz);
+        MC_LAUNCH_CHECK(c);
+        if (l + 1 < c->amg.size()) {
+            AmgLevel &Lc = c->amg[l + 1];
+            hipLaunchKernelGGL(k_gather_sum, dim3(div_up(Lc.nnz, 256)), dim3(256), 0, c->stream, Lc.nnz, L.gptr, L.gmap,
+                               L.vals, Lc.vals);
+            MC_LAUNCH_CHECK(c);
+            vf = Lc.vals;
+        }
+    }
+    AmgLevel &Lc = c->amg.back();
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(1024), 0, c->stream, Lc.n, Lc.row_ptr, Lc.col, Lc.vals,
+                       c->amg_work, c->amg_dense);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+// ------------------------------------------------------ V-cycle kernels --
+// first two pre-smoothing sweeps from x = 0: x1 = w D^-1 b, x2 = x1 + w D^-1 (b - A x1)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_jac_first2(int n, const int *rp, const int *col, const T *vals,
+                                                      const T *dinv, const T *b, T *xout, T omega, int chunk, int two)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    const int lane8 = threadIdx.x & 7;
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc = 0.0;
+        if (r < r1 && two) {
+            for (int e = rp[r] + lane8; e < rp[r + 1]; e += 8) {
+                const int j = col[e];
+                acc += (double)vals[e] * (double)(omega * dinv[j] * b[j]);
+            }
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        if (lane8 == 0 && r < r1) {
+            const T x1 = omega * dinv[r] * b[r];
+            xout[r] = two ? (T)(x1 + omega * dinv[r] * (b[r] - (T)acc)) : x1;
+        }
+    }
+}
+
+// x_out = y + w D^-1 (b - A y), y_j = x_j + cs * xc[agg_j] (agg == nullptr: y = x)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_jac(int n, const int *rp, const int *col, const T *vals, const T *dinv,
+                                               const T *b, const T *x, const int *agg, const T *xc, T cs, T *xout,
+                                               T omega, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    const int lane8 = threadIdx.x & 7;
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc = 0.0;
+        if (r < r1) {
+            for (int e = rp[r] + lane8; e < rp[r + 1]; e += 8) {
+                const int j = col[e];
+                T y = x[j];
+                if (agg) y += cs * xc[agg[j]];
+                acc += (double)vals[e] * (double)y;
+            }
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        if (lane8 == 0 && r < r1) {
+            T y = x[r];
+            if (agg) y += cs * xc[agg[r]];
+            xout[r] = y + omega * dinv[r] * (b[r] - (T)acc);
+        }
+    }
+}
+
+// bc[I] = sum_{i in aggregate I} (b_i - (A x)_i): residual fused with restriction
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_resid_restrict(int nc, const int *aptr, const int *amem, const int *rp,
+                                                          const int *col, const T *vals, const T *b, const T *x, T *bc)
+{
+    const int I = blockIdx.x * (BLOCK / 8) + (threadIdx.x >> 3);
+    const int lane8 = threadIdx.x & 7;
+    double s = 0.0;
+    if (I < nc) {
+        for (int t = aptr[I]; t < aptr[I + 1]; t++) {
+            const int i = amem[t];
+            double acc = 0.0;
+            for (int e = rp[i] + lane8; e < rp[i + 1]; e += 8) acc += (double)vals[e] * (double)x[col[e]];
+            if (lane8 == 0) s += (double)b[i];
+            s -= acc;
+        }
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (lane8 == 0 && I < nc) bc[I] = (T)s;
+}
+
+// coarsest level: x = Ainv b in FP64
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, const T *b, T *x)
+{
+    __shared__ double bs[1024];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) bs[i] = (double)b[i];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int j = 0; j < n; j++) s += inv[(size_t)i * n + j] * bs[j];
+    x[i] = (T)s;
+}
+
+template <typename T>
+struct LevelPtrs {
+    static T *vals(AmgLevel &L);
+    static T *dinv(AmgLevel &L);
+    static T *x(AmgLevel &L);
+    static T *b(AmgLevel &L);
+    static T *t(AmgLevel &L);
+};
+template <> double *LevelPtrs<double>::vals(AmgLevel &L) { return L.vals; }
+template <> double *LevelPtrs<double>::dinv(AmgLevel &L) { return L.dinv; }
+template <> double *LevelPtrs<double>::x(AmgLevel &L) { return L.x; }
+template <> double *LevelPtrs<double>::b(AmgLevel &L) { return L.b; }
+template <> double *LevelPtrs<double>::t(AmgLevel &L) { return L.t; }
+template <> float *LevelPtrs<float>::vals(AmgLevel &L) { return L.vals32; }
+template <> float *LevelPtrs<float>::dinv(AmgLevel &L) { return L.dinv32; }
+template <> float *LevelPtrs<float>::x(AmgLevel &L) { return L.x32; }
+template <> float *LevelPtrs<float>::b(AmgLevel &L) { return L.b32; }
+template <> float *LevelPtrs<float>::t(AmgLevel &L) { return L.t32; }
+
+// V-cycle on level l: x_l = M^-1 b_l (b given in 'b', result written to 'x').
+template <typename T>
+static int vcycle(Ctx *c, size_t l, const T *b, T *x)
+{
+    AmgLevel &L = c->amg[l];
+    const T omega = (T)c->opt.amg_omega;
+    const int nu = std::max(1, c->opt.amg_nu);
+    if (l + 1 == c->amg.size()) {
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x);
+        return MC_OK;
+    }
+    const Grid g = grid_for(L.n, ROWS_PER_BLOCK);
+    T *vals = LevelPtrs<T>::vals(L), *dinv = LevelPtrs<T>::dinv(L), *tmp = LevelPtrs<T>::t(L);
+    // pre-smoothing: x <- S^nu 0
+    T *cur = x, *oth = tmp;
+    hipLaunchKernelGGL(k_jac_first2<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv,
+                       b, cur, omega, g.chunk, nu >= 2 ? 1 : 0);
+    for (int s = 2; s < nu; s++) {
+        hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
+                           cur, (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk);
+        std::swap(cur, oth);
+    }
+    // residual + restriction
+    AmgLevel &Lc = c->amg[l + 1];
+    T *bc = LevelPtrs<T>::b(Lc), *xc = LevelPtrs<T>::x(Lc);
+    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, L.nc, L.agg_ptr,
+                       L.agg_members, L.row_ptr, L.col, vals, b, cur, bc);
+    MC_TRY(vcycle<T>(c, l + 1, bc, xc));
+    // prolongation fused with the first post-smoothing sweep
+    const T cs = (T)c->opt.amg_cscale;
+    hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
+                       cur, L.agg, xc, cs, oth, omega, g.chunk);
+    std::swap(cur, oth);
+    for (int s = 1; s < nu; s++) {
+        hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
+                           cur, (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk);
+        std::swap(cur, oth);
+    }
+    if (cur != x) MC_HIP(c, hipMemcpyAsync(x, cur, (size_t)L.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    return MC_OK;
+}
+
+// ------------------------------------------------------------ PCG --
+template <typename T>
+struct PcgVecs {
+    T *x, *r, *z, *p, *q;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const double *mean, PcgVecs<T> w, KState *ks,
+                                                    double *partials, unsigned *cnt, int chunk, double rtol,
+                                                    double atol, int maxit)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[1] = {0.0};
+    const double mu = mean ? *mean : 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        const T r = (T)((double)b[i] - mu);
+        w.r[i] = r; w.x[i] = (T)0; w.p[i] = (T)0;
+        acc[0] += (double)r * (double)r;
+    }
+    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
+        const double bn = sqrt(acc[0]);
+        ks->bnorm[0] = bn; ks->rnorm[0] = bn;
+        ks->tol[0] = fmax(rtol * bn, atol);
+        ks->rho_old[0] = 0.0; ks->iters[0] = 0; ks->maxit = maxit;
+        ks->done[0] = (bn <= ks->tol[0]) ? 1 : 0;
+        ks->conv[0] = ks->done[0];
+        ks->all_done = ks->done[0];
+    }
+}
+
+// rho = (r, z); p = z + beta p
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_rz(int n, PcgVecs<T> w, KState *ks, double *partials, unsigned *cnt,
+                                                  int chunk)
+{
+    if (ks->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[1] = {0.0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += (double)w.r[i] * (double)w.z[i];
+    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
+        ks->rho[0] = acc[0];
+        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
+        ks->rho_old[0] = acc[0];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_p(int n, PcgVecs<T> w, const KState *ks, int chunk)
+{
+    if (ks->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    const T beta = (T)ks->beta[0];
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) w.p[i] = w.z[i] + beta * w.p[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_q(int n, const int *rp, const int *col, const T *vals, PcgVecs<T> w,
+                                                 KState *ks, double *partials, unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    double dsum[1] = {0.0};
+    const T *xs[1] = {w.p};
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int r = base + (threadIdx.x >> 3);
+        double acc[1];
+        csr_row8<1, T, T>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
+        if ((threadIdx.x & 7) == 0 && r < r1) {
+            w.q[r] = (T)acc[0];
+            dsum[0] += (double)w.p[r] * acc[0];
+        }
+    }
+    if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) ks->alpha[0] = ks->rho[0] / dsum[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, PcgVecs<T> w, KState *ks, double *partials, unsigned *cnt,
+                                                  int chunk)
+{
+    if (ks->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    const T al = (T)ks->alpha[0];
+    double acc[1] = {0.0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        w.x[i] += al * w.p[i];
+        const T r = w.r[i] - al * w.q[i];
+        w.r[i] = r;
+        acc[0] += (double)r * (double)r;
+    }
+    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
+        const double rn = sqrt(acc[0]);
+        ks->rnorm[0] = rn;
+        ks->iters[0] += 1;
+        if (rn <= ks->tol[0]) { ks->done[0] = 1; ks->conv[0] = 1; }
+        else if (ks->iters[0] >= ks->maxit || !(rn == rn)) ks->done[0] = 1;
+        ks->all_done = ks->done[0];
+    }
+}
+
+template <typename T>
+static int pcg_core(Ctx *c, const T *vals, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit,
+                    KState *ks, KState *hs, int *iters_out)
+{
+    const int n = c->N;
+    const Grid gv = grid_for(n, BLOCK);
+    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
+    const double *mean = nullptr;
+    if (project) {
+        // mean of b (as double) for the projection onto the range of A
+        double *dm = c->dscal + 600;
+        if (sizeof(T) == 8) {
+            hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)b, dm, c->red,
+                               c->red_cnt + 10, gv.chunk, 1.0 / (double)n);
+            mean = dm;
+        }
+    }
+    hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, ks, c->red,
+                       c->red_cnt, gv.chunk, rtol, atol, maxit);
+    MC_LAUNCH_CHECK(c);
+    const int check = std::max(1, c->opt.check_every);
+    int it = 0;
+    for (;;) {
+        for (int j = 0; j < check && it < maxit; j++, it++) {
+            {
+                KTimer kt(c, MC_KC_VCYCLE, 0.0);
+                MC_TRY(vcycle<T>(c, 0, w.r, w.z));
+            }
+            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               gv.chunk);
+            hipLaunchKernelGGL(k_pcg_p<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, gv.chunk);
+            {
+                KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32,
+                          (4.0 + sizeof(T)) * c->nnz + (4.0 + 2 * sizeof(T)) * n);
+                hipLaunchKernelGGL(k_pcg_q<T>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col, vals,
+                                   w, ks, c->red, c->red_cnt, gm.chunk);
+            }
+            hipLaunchKernelGGL(k_pcg_xr<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               gv.chunk);
+        }
+        MC_LAUNCH_CHECK(c);
+        MC_TRY(poll_done(c, ks, hs));
+        if (hs->all_done || it >= maxit) break;
+    }
+    *iters_out = hs->iters[0];
+    return MC_OK;
+}
+
+__global__ void k_sub_mean(int n, double *x, const double *mean)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] -= *mean;
+}
+
+static int project_mean_zero(Ctx *c, double *x)
+{
+    const int n = c->N;
+    const Grid gv = grid_for(n, BLOCK);
+    double *dm = c->dscal + 610;
+    hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)x, dm, c->red,
+                       c->red_cnt + 11, gv.chunk, 1.0 / (double)n);
+    hipLaunchKernelGGL(k_sub_mean, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, dm);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
+            mc_solve_status *st)
+{
+    KState *ks = (KState *)c->dscal;
+    KState *hs = (KState *)c->hscal;
+    PcgVecs<double> w{x, c->wk[0], c->wk[1], c->wk[2], c->wk[3]};
+    int iters = 0;
+    MC_TRY(pcg_core<double>(c, vals, b, true, w, rtol, atol, maxit, ks, hs, &iters));
+    MC_TRY(project_mean_zero(c, x));
+    if (st) {
+        st->iterations = iters;
+        st->converged = hs->conv[0];
+        st->relres = hs->bnorm[0] > 0 ? hs->rnorm[0] / hs->bnorm[0] : 0.0;
+        st->outer = 1;
+        st->bnorm = hs->bnorm[0];
+    }
+    return MC_OK;
+}
+
+// ---------------------------------------------- mixed: FP32 inside FP64 IR --
+__global__ void __launch_bounds__(BLOCK) k_ir_resid(int n, const int *rp, const int *col, const double *vals,
+                                                    const double *b, const double *mean, const double *x, double *r,
+                                                    double *partials, unsigned *cnt, int chunk, double *out)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    double dsum[1] = {0.0};
+    const double *xs[1] = {x};
+    const double mu = *mean;
+    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
+        const int row = base + (threadIdx.x >> 3);
+        double acc[1];
+        csr_row8<1, double, double>(row < r1 ? row : n, n, rp, col, vals, xs, acc);
+        if ((threadIdx.x & 7) == 0 && row < r1) {
+            const double rr = (b[row] - mu) - acc[0];
+            r[row] = rr;
+            dsum[0] += rr * rr;
+        }
+    }
+    if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) *out = sqrt(dsum[0]);
+}
+
+__global__ void k_to32_scaled(int n, const double *r, const double *nrm, float *r32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double s = (*nrm > 0.0) ? 1.0 / *nrm : 0.0;
+    r32[i] = (float)(r[i] * s);
+}
+
+__global__ void k_ir_update(int n, double *x, const float *e32, const double *nrm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)e32[i] * *nrm;
+}
+
+__global__ void k_zero(int n, double *x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
+                   mc_solve_status *st)
+{
+    const int n = c->N;
+    const Grid gv = grid_for(n, BLOCK);
+    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
+    KState *ks = (KState *)c->dscal;
+    KState *hs = (KState *)c->hscal;
+    double *mean = c->dscal + 620, *rn = c->dscal + 621;
+    double *r = c->wk[4];
+    float *r32 = c->wk32[0], *e32 = c->wk32[1];
+    PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5]};
+    hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 12,
+                       gv.chunk, 1.0 / (double)n);
+    hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
+    // FP32 copy of the fine matrix values (the AMG levels already hold theirs)
+    const float *vals32 = c->amg[0].vals32;
+    double bnorm = -1.0, rnorm = 0.0;
+    int total = 0, outer = 0;
+    bool conv = false;
+    const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
+    for (outer = 0; outer < 50; outer++) {
+        {
+            KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 28.0 * n);
+            hipLaunchKernelGGL(k_ir_resid, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col, vals, b,
+                               mean, x, r, c->red, c->red_cnt + 13, gm.chunk, rn);
+        }
+        MC_LAUNCH_CHECK(c);
+        MC_HIP(c, hipMemcpyAsync(c->hscal + 621, rn, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        MC_HIP(c, hipStreamSynchronize(c->stream));
+        rnorm = c->hscal[621];
+        if (bnorm < 0) bnorm = rnorm;
+        if (rnorm <= std::max(rtol * bnorm, atol) || !(rnorm == rnorm)) { conv = rnorm == rnorm; break; }
+        if (total >= maxit) break;
+        hipLaunchKernelGGL(k_to32_scaled, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, r, rn, r32);
+        int it = 0;
+        MC_TRY(pcg_core<float>(c, vals32, r32, false, w, inner, 0.0, maxit - total, ks, hs, &it));
+        total += it;
+        hipLaunchKernelGGL(k_ir_update, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, e32, rn);
+        MC_LAUNCH_CHECK(c);
+    }
+    MC_TRY(project_mean_zero(c, x));
+    if (st) {
+        st->iterations = total;
+        st->converged = conv ? 1 : 0;
+        st->relres = bnorm > 0 ? rnorm / bnorm : 0.0;
+        st->outer = outer;
+        st->bnorm = bnorm;
+    }
+    return MC_OK;
+}
+
+int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    mc_solve_status s{};
+    int rc;
+    if (c->opt.psolve == MC_PSOLVE_MIXED)
+        rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s);
+    else
+        rc = pcg_amg(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s);
+    const auto t1 = std::chrono::steady_clock::now();
+    c->st.p_solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (rc) return rc;
+    c->st.p_iters += s.iterations;
+    c->st.p_solves += 1;
+    c->st.p_outer += s.outer;
+    c->st.last_p_relres = s.relres;
+    if (st) *st = s;
+    if (!s.converged) {
+        c->err = "pressure solve did not converge";
+        return MC_ERR_NOCONV;
+    }
+    return MC_OK;
+}
+
+}  // namespace mc
