@@ -93,6 +93,11 @@ typedef struct mc_stats {
     double  p_solve_ms;       /* host wall time inside pressure solves           */
     double  last_p_max;       /* max |PP| of the last correction loop            */
     int     last_loops;
+    double  p_relres_max;     /* max over solves of ||b - A x|| / (tol / rtol): the
+                                 residual relative to the RHS scale its tolerance
+                                 was set against (loop 1: its own RHS)           */
+    int64_t clip_cells;       /* TE/ED cells repaired to non-negative values     */
+    int64_t clip_passes;      /* relaxation passes of that repair                */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>

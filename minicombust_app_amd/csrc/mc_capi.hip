@@ -92,6 +92,7 @@ static int alloc_workspace(Ctx *c)
     MC_HIP(c, hipMemset(c->dscal, 0, 1024 * sizeof(double)));
     MC_HIP(c, hipHostMalloc((void **)&c->hscal, 1024 * sizeof(double), hipHostMallocDefault));
     memset(c->hscal, 0, 1024 * sizeof(double));
+    MC_TRY(flow_workspace(c));
     return MC_OK;
 }
 
@@ -131,7 +132,7 @@ int mc_options_default(mc_options *o)
     o->amg_nu = 2;
     o->amg_omega = 0.8;
     o->amg_cscale = 1.8;
-    o->amg_coarse_max = 256;
+    o->amg_coarse_max = 128;
     o->mixed_inner_rtol = 1.0e-3;
     o->check_every = 2;
     return MC_OK;
@@ -145,7 +146,7 @@ static int validate(const mc_options *o, std::string &err)
         err = "tolerances / iteration limits";
         return MC_ERR_ARG;
     }
-    if (o->amg_coarse_max < 8 || o->amg_coarse_max > 1024) { err = "amg_coarse_max must be in [8, 1024]"; return MC_ERR_ARG; }
+    if (o->amg_coarse_max < 8 || o->amg_coarse_max > 128) { err = "amg_coarse_max must be in [8, 128]"; return MC_ERR_ARG; }
     if (o->amg_nu < 1 || o->amg_nu > 8) { err = "amg_nu must be in [1, 8]"; return MC_ERR_ARG; }
     return MC_OK;
 }

@@ -10,6 +10,8 @@
 // by the owner of the highest-numbered face of that side: the reference's
 // last writer.  Compiled without FP contraction so each expression rounds as
 // written in FlowSolver.inl.
+#include <hipcub/hipcub.hpp>
+
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -941,6 +943,168 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
 }
 
+// The passive scalars FUL, PRO, VARF and VARP share one Visac rule
+// (FlowSolver.inl:2557-2560), the same under-relaxation (0.95) and no
+// turbulence terms, so their Scalar_solve systems have bit-identical matrices
+// and differ only in the right-hand side: one kernel assembles the shared
+// matrix once and the four right-hand sides, each with exactly the additions
+// of the single-field kernel above.
+constexpr int SB = 4;
+struct ScalBatchArgs {
+    double *phi[SB];
+    const double *gx[SB], *gy[SB], *gz[SB];
+    double *S[SB];
+    double inlet_phi[SB];
+    const double *mflux;
+    double *ff0, *ff1, *AV, *vals;
+    int type;
+    double ev, inlet_ev, rdelta, urf;
+};
+
+__global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
+    const int N = g.N;
+    const double ev = a.ev;
+    const double GammaBlend = 0.0;
+    double S[SB], A = 0.0;
+#pragma unroll
+    for (int s = 0; s < SB; s++) S[s] = 0.0;
+    double own_ff[6];
+    int own_nb[6];
+    int nint = 0;
+    for (int k = 0; k < 6; k++) {
+        const int f = g.forder[6 * c + k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const double mf = a.mflux[f];
+        if (c1 < N) {
+            const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
+            double Visac = ev * lambda0 + ev * lambda1;
+            Visac -= ev;
+            Visac = scalar_visac(a.type, ev, Visac);
+            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
+            const double VisFace = Visac * g.frlc[f];
+            const double ff0 = -VisFace - dmax(mf, 0.0);
+            const double ff1 = -VisFace + dmin(mf, 0.0);
+#pragma unroll
+            for (int s = 0; s < SB; s++) {
+                const dv3 dPhiXac = ld3(a.gx[s], a.gy[s], a.gz[s], c0) * lambda0 + ld3(a.gx[s], a.gy[s], a.gz[s], c1) * lambda1;
+                const double p0 = a.phi[s][c0], p1 = a.phi[s][c1];
+                const double PhiFace = (mf >= 0.0) ? p0 : p1;
+                const double fce = mf * PhiFace;
+                const double fde1 = Visac * dot(dPhiXac, nrm);
+                const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
+                const double fdi = VisFace * dot(dPhiXac, Xpn);
+                const double blend = GammaBlend * (fce - fci);
+                if (c == c0) S[s] = S[s] - blend + fde1 - fdi;
+                else S[s] = S[s] + blend - fde1 + fdi;
+            }
+            if (c == c0) {
+                own_ff[nint] = ff1; own_nb[nint] = c1;
+                a.ff0[f] = ff0; a.ff1[f] = ff1;
+            } else {
+                own_ff[nint] = ff0; own_nb[nint] = c0;
+            }
+            nint++;
+        } else {
+            const int side = c1 - N;
+            const int bt = g.btype[side];
+            const bool last = (f == g.last_face[side]);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
+            if (bt == INLET) {
+                double Visac = a.inlet_ev;
+                Visac -= ev;
+                Visac = scalar_visac(a.type, ev, Visac);
+                const double VisFace = Visac * g.frlc[f];
+                const double fb = -VisFace + dmin(mf, 0.0);
+                A = A - fb;
+#pragma unroll
+                for (int s = 0; s < SB; s++) {
+                    const dv3 gP = ld3(a.gx[s], a.gy[s], a.gz[s], c0);
+                    const double fde = Visac * dot(gP, nrm);
+                    const double fdi = VisFace * dot(gP, Xpn);
+                    S[s] = S[s] - fb * a.inlet_phi[s] + fde - fdi;
+                    if (last) a.phi[s][N + side] = a.inlet_phi[s];
+                }
+            } else if (bt == OUTLET) {
+                double Visac = ev;
+                Visac -= ev;
+                Visac = scalar_visac(a.type, ev, Visac);
+                const double VisFace = Visac * g.frlc[f];
+#pragma unroll
+                for (int s = 0; s < SB; s++) {
+                    const dv3 gP = ld3(a.gx[s], a.gy[s], a.gz[s], c0);
+                    const double PhiFace = a.phi[s][c0] + dot(gP, Xpn);
+                    const double fde = Visac * dot(gP, nrm);
+                    const double fdi = VisFace * dot(gP, Xpn);
+                    S[s] = S[s] + fde - fdi;
+                    if (last) a.phi[s][N + side] = PhiFace;
+                }
+            } else if (bt == WALL) {
+                if (last) {
+#pragma unroll
+                    for (int s = 0; s < SB; s++) a.phi[s][N + side] = a.phi[s][c0];
+                }
+            }
+        }
+    }
+    const double fv = g.rho[c] * g.vol[c] * a.rdelta;
+    double phic[SB];
+#pragma unroll
+    for (int s = 0; s < SB; s++) {
+        phic[s] = a.phi[s][c];
+        S[s] += fv * phic[s];
+    }
+    A += fv;
+    for (int k = 0; k < nint; k++) A -= own_ff[k];
+    A *= (1. / a.urf);
+#pragma unroll
+    for (int s = 0; s < SB; s++) {
+        S[s] = S[s] + (1.0 - a.urf) * A * phic[s];
+        a.S[s][c] = S[s];
+    }
+    a.AV[c] = A;
+    int pos[6], dpos;
+    row_positions(c, nint, own_nb, pos, dpos);
+    const int rp = g.row_ptr[c];
+    a.vals[rp + dpos] = A;
+    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+}
+
+// FUL, PRO, VARF, VARP (FlowSolver.inl:3243-3260) as one 4-RHS solve.
+int scalar_batch_solve(Ctx *c)
+{
+    static const int types[SB] = {FUEL, PROG, VARFU, VARPR};
+    static const int fields[SB] = {FUL, PRO, VARF, VARP};
+    ScalBatchArgs a;
+    for (int s = 0; s < SB; s++) {
+        a.phi[s] = c->phi[fields[s]];
+        a.gx[s] = c->gx[fields[s]]; a.gy[s] = c->gy[fields[s]]; a.gz[s] = c->gz[fields[s]];
+        a.S[s] = c->srhs[s];
+        a.inlet_phi[s] = (types[s] == FUEL || types[s] == VARFU) ? c->k.gas_ful : 0.0;
+    }
+    a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.vals = c->A_vals;
+    a.type = FUEL;
+    a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
+    {
+        KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * SB * c->N + 56.0 * c->F + 8.0 * c->nnz);
+        hipLaunchKernelGGL(k_scalar_batch, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+        MC_LAUNCH_CHECK(c);
+    }
+    double *b[SB], *x[SB];
+    for (int s = 0; s < SB; s++) { b[s] = c->srhs[s]; x[s] = c->phi[fields[s]]; }
+    mc_solve_status st{};
+    MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    c->st.scalar_iters += st.iterations;
+    if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
+    // S_phi.U holds the last equation's right-hand side, as after VARP
+    MC_HIP(c, hipMemcpyAsync(c->SU, c->srhs[SB - 1], (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+    return MC_OK;
+}
+
 static int scalar_field(int type)
 {
     static const int f[7] = {TE, ED, TEM, FUL, PRO, VARF, VARP};
@@ -978,59 +1142,82 @@ int scalar_assemble(Ctx *c, int type)
 
 // Non-negativity repair of TE/ED (FlowSolver.inl:2956-2985).  The reference
 // sweeps cells in ascending order and reads already-repaired lower
-// neighbours; the fixed point below reproduces that order exactly.
-__global__ void __launch_bounds__(256) k_clip(Geo g, const double *orig, const double *cur, double *next,
-                                              int *changed)
-{
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= g.N) return;
-    const double v = orig[c];
-    double out = v;
-    if (v < 0.0) {
-        int count = 0;
-        double phisum = 0.0;
-        for (int j = 0; j < 6; j++) {
-            const int nb = g.nbr[6 * c + j];
-            if (nb < 0) continue;
-            count++;
-            phisum += (nb < c) ? cur[nb] : orig[nb];
-        }
-        phisum = (phisum / count);
-        out = dmax(phisum, 0.000000000001);
-    }
-    if (out != cur[c] && !(out != out && cur[c] != cur[c])) atomicAdd(changed, 1);
-    next[c] = out;
-}
-
-__global__ void __launch_bounds__(256) k_count_neg(int n, const double *v, int *cnt)
+// neighbours.  A negative cell's new value depends only on lower-numbered
+// negative cells (a DAG), so the sequential result is the unique fixed point
+//   v_c = max(mean_j (j < c ? v_j : orig_j), 1e-12)   over the negative cells.
+// The negative cells are compacted once (ascending); passes relax them in
+// place until a pass changes nothing -- that state is the fixed point, hence
+// the reference's values exactly, whatever order the threads ran in.
+__global__ void __launch_bounds__(256) k_flag_neg(int n, const double *v, uint8_t *flag)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const int neg = (i < n && v[i] < 0.0) ? 1 : 0;
-    const unsigned long long b = __ballot(neg);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, __popcll(b));
+    if (i < n) flag[i] = v[i] < 0.0 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_clip_list(Geo g, const int *list, int count, const double *orig, double *phi,
+                                                   int *changed)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const int c = list[i];
+    int cnt = 0;
+    double phisum = 0.0;
+    for (int j = 0; j < 6; j++) {
+        const int nb = g.nbr[6 * c + j];
+        if (nb < 0) continue;
+        cnt++;
+        phisum += (nb < c) ? phi[nb] : orig[nb];
+    }
+    phisum = (phisum / cnt);
+    const double out = dmax(phisum, 0.000000000001);
+    const double cur = phi[c];
+    if (out != cur && !(out != out && cur != cur)) {
+        phi[c] = out;
+        atomicAdd(changed, 1);
+    }
+}
+
+int flow_workspace(Ctx *c)
+{
+    for (int s = 0; s < 4; s++) MC_TRY(dalloc(c, &c->srhs[s], c->N));
+    MC_TRY(dalloc(c, &c->neg_list, c->N));
+    size_t tb = 0;
+    MC_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<int>(0), (const uint8_t *)nullptr,
+                                            (int *)nullptr, (int *)nullptr, c->N, c->stream));
+    c->cub_tmp_bytes = tb;
+    MC_TRY(alloc(c, &c->cub_tmp, tb + 256));
+    return MC_OK;
 }
 
 static int clip_nonneg(Ctx *c, double *phi)
 {
     const int N = c->N;
-    int *cnt = (int *)(c->dscal + 700);
-    int *hcnt = (int *)(c->hscal + 700);
-    MC_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_count_neg, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, phi, cnt);
-    MC_HIP(c, hipMemcpyAsync(hcnt, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int *dcount = (int *)(c->dscal + 700);
+    int *hcount = (int *)(c->hscal + 700);
+    int *changed = (int *)(c->dscal + 704);   // 4 pass counters
+    uint8_t *flag = (uint8_t *)c->wk[13];
+    hipLaunchKernelGGL(k_flag_neg, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, phi, flag);
+    MC_LAUNCH_CHECK(c);
+    size_t tb = c->cub_tmp_bytes;
+    MC_HIP(c, hipcub::DeviceSelect::Flagged(c->cub_tmp, tb, hipcub::CountingInputIterator<int>(0), flag, c->neg_list,
+                                            dcount, N, c->stream));
+    MC_HIP(c, hipMemcpyAsync(hcount, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
-    if (*hcnt == 0) return MC_OK;
-    double *orig = c->wk[14], *a = c->wk[15], *b = phi;
+    const int K = *hcount;
+    c->st.clip_cells += K;
+    if (K == 0) return MC_OK;
+    double *orig = c->wk[14];
     MC_HIP(c, hipMemcpyAsync(orig, phi, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
-    // cur starts as orig (= phi); iterate next = clip(cur) until fixed point
-    for (int pass = 0; pass < 10000; pass++) {
-        MC_HIP(c, hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
-        hipLaunchKernelGGL(k_clip, dim3(div_up(N, 256)), dim3(256), 0, c->stream, geo(c), orig, b, a, cnt);
+    for (int batch = 0; batch < 100000; batch++) {
+        MC_HIP(c, hipMemsetAsync(changed, 0, 4 * sizeof(int), c->stream));
+        for (int p = 0; p < 4; p++)
+            hipLaunchKernelGGL(k_clip_list, dim3(div_up(K, 256)), dim3(256), 0, c->stream, geo(c), c->neg_list, K,
+                               orig, phi, changed + p);
         MC_LAUNCH_CHECK(c);
-        MC_HIP(c, hipMemcpyAsync(b, a, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
-        MC_HIP(c, hipMemcpyAsync(hcnt, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        c->st.clip_passes += 4;
+        MC_HIP(c, hipMemcpyAsync(hcount, changed + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         MC_HIP(c, hipStreamSynchronize(c->stream));
-        if (*hcnt == 0) break;
+        if (*hcount == 0) break;
     }
     return MC_OK;
 }
@@ -1053,12 +1240,19 @@ int scalar_solve(Ctx *c, int type)
 int simple_iteration(Ctx *c, int inner)
 {
     MC_TRY(grad_multi(c));
-    if (c->timestep_count == 0 && inner == 0) MC_TRY(set_up_field(c));
+    // FlowSolver.inl:3204-3208: the first pass of the first timestep
+    if (c->timestep_count == 0 && inner == 0 && !c->field_set_up) {
+        MC_TRY(set_up_field(c));
+        c->field_set_up = true;
+    }
     MC_TRY(momentum_assemble(c));
     MC_TRY(momentum_solve(c));
     MC_TRY(calculate_pressure(c));
     if (c->opt.solve_scalars) {
-        for (int t = TERBTE; t <= VARPR; t++) MC_TRY(scalar_solve(c, t));
+        MC_TRY(scalar_solve(c, TERBTE));
+        MC_TRY(scalar_solve(c, TERBED));
+        MC_TRY(scalar_solve(c, TEMP));
+        MC_TRY(scalar_batch_solve(c));
     }
     c->st.simple_iterations++;
     c->global_iter++;

@@ -107,10 +107,15 @@ struct Ctx {
     double *Pb = nullptr;          // pressure RHS (compat: frozen copy)
     double *u = nullptr;           // solution scratch
     bool first_pressure = true;
+    bool field_set_up = false;   // set_up_field has run (first pass of timestep 0)
     int64_t timestep_count = 0;
     int64_t global_iter = 0;
 
     // ---- solver workspace ----
+    double *srhs[4] = {};          // right-hand sides of the batched scalar solve
+    int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
+    void *cub_tmp = nullptr;       // hipcub scratch for the compaction
+    size_t cub_tmp_bytes = 0;
     double *wk[16] = {};           // N-length work vectors
     float *wk32[8] = {};
     double *red = nullptr;         // reduction partials
@@ -274,6 +279,8 @@ int calculate_mass_flux_assemble(Ctx *c);               // mass flux + pressure 
 int calculate_pressure(Ctx *c);
 int scalar_assemble(Ctx *c, int type);
 int scalar_solve(Ctx *c, int type);
+int scalar_batch_solve(Ctx *c);
+int flow_workspace(Ctx *c);
 int simple_iteration(Ctx *c, int inner);
 int pressure_boundary_slots(Ctx *c, int field);
 int update_P(Ctx *c);

@@ -121,7 +121,8 @@ int spmv(Ctx *c, const double *vals, const double *x, double *y)
 
 // ------------------------------------------------------- BiCGStab --
 struct BiVecs {
-    double *x[4], *r[4], *rh[4], *p[4], *v[4], *s[4], *t[4];
+    double *x[4], *r[4], *p[4], *v[4], *s[4], *t[4];
+    const double *rh[4];
     const double *b[4];
 };
 
@@ -134,11 +135,13 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, 
     double acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    // r^ is b itself (never written during the solve); p and v start at zero
+    // implicitly (k_bi_p reads them only after the first iteration)
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
 #pragma unroll
         for (int k = 0; k < NR; k++) {
             const double bi = w.b[k][i];
-            w.x[k][i] = 0.0; w.r[k][i] = bi; w.rh[k][i] = bi; w.p[k][i] = 0.0; w.v[k][i] = 0.0;
+            w.x[k][i] = 0.0; w.r[k][i] = bi;
             acc[k] += bi * bi;
         }
     }
@@ -171,7 +174,9 @@ __global__ void __launch_bounds__(BLOCK) k_bi_p(int n, BiVecs w, const KState *k
         for (int k = 0; k < NR; k++) {
             if (ks->done[k]) continue;
             const double beta = ks->beta[k], om = ks->omega[k];
-            w.p[k][i] = w.r[k][i] + beta * (w.p[k][i] - om * w.v[k][i]);
+            const bool first = ks->iters[k] == 0;
+            const double pold = first ? 0.0 : w.p[k][i], vold = first ? 0.0 : w.v[k][i];
+            w.p[k][i] = w.r[k][i] + beta * (pold - om * vold);
         }
     }
 }
@@ -336,21 +341,18 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
                        mc_solve_status *st)
 {
     const int n = c->N;
-    BiVecs w{};
-    for (int k = 0; k < NR; k++) {
-        w.b[k] = b[k]; w.x[k] = x[k];
-        w.r[k] = c->wk[k * 6 + 0 < 16 ? 0 : 0];
-    }
-    // work vectors: 6 per RHS from c->wk (16 available) plus the aux pool
     static_assert(NR <= 4, "NR");
+    BiVecs w{};
+    // five work vectors per RHS (r, p, v, s, t) from the 16 FP64 work vectors
+    // and the 8 scratch vectors (sized for doubles); r^ aliases b
     double *pool[24];
     int np = 0;
     for (int i = 0; i < 16; i++) pool[np++] = c->wk[i];
-    for (int i = 0; i < 8 && np < 24; i++) pool[np++] = (double *)c->wk32[i];  // wk32 sized as doubles
+    for (int i = 0; i < 8; i++) pool[np++] = (double *)c->wk32[i];
     int q = 0;
     for (int k = 0; k < NR; k++) {
-        w.r[k] = pool[q++]; w.rh[k] = pool[q++]; w.p[k] = pool[q++];
-        w.v[k] = pool[q++]; w.s[k] = pool[q++]; w.t[k] = pool[q++];
+        w.b[k] = b[k]; w.x[k] = x[k]; w.rh[k] = b[k];
+        w.r[k] = pool[q++]; w.p[k] = pool[q++]; w.v[k] = pool[q++]; w.s[k] = pool[q++]; w.t[k] = pool[q++];
     }
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
@@ -365,14 +367,14 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
         for (int j = 0; j < check && it < maxit; j++, it++) {
             hipLaunchKernelGGL(k_bi_p<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, gv.chunk);
             {
-                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 20.0 * n * NR);
+                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 4.0 * n + 24.0 * n * NR);
                 hipLaunchKernelGGL(k_bi_spmv_v<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
                                    vals, w, ks, c->red, c->red_cnt, gm.chunk);
             }
             hipLaunchKernelGGL(k_bi_s<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
                                gv.chunk);
             {
-                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 20.0 * n * NR);
+                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 4.0 * n + 24.0 * n * NR);
                 hipLaunchKernelGGL(k_bi_spmv_t<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
                                    vals, w, ks, c->red, c->red_cnt, gm.chunk);
             }
@@ -585,49 +587,47 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
 
 // Dense regularised coarsest operator A + alpha 11^T (alpha = mean diag / n):
 // for a mean-zero right-hand side its inverse returns the mean-zero solution
-// of the singular Neumann system.  Gauss-Jordan in one workgroup.
+// of the singular Neumann system.  In-place Gauss-Jordan inversion (SPD, no
+// pivoting) held entirely in LDS: n <= 128 -> at most 128 KiB of gfx950's
+// 160 KiB, one workgroup, no global-memory traffic inside the n steps.
+constexpr int DENSE_MAX = 128;
 __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, const int *col, const double *v,
-                                                        double *M /* n x 2n work */, double *inv)
+                                                        double *inv)
 {
-    __shared__ double alpha_s;
-    __shared__ double piv_s;
+    __shared__ double M[DENSE_MAX * DENSE_MAX];
+    __shared__ double colk[DENSE_MAX];
+    __shared__ double red[1024 / 64];
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; i++)
-            for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) s += v[k];
-        alpha_s = (s / n) / n;
-    }
+    // alpha = (sum of diagonal / n) / n
+    double s = 0.0;
+    for (int i = tid; i < n; i += blockDim.x)
+        for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) s += v[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    const int w = 2 * n;
-    for (int e = tid; e < n * w; e += blockDim.x) {
-        const int i = e / w, j = e % w;
-        M[e] = (j < n) ? alpha_s : ((j - n == i) ? 1.0 : 0.0);
-    }
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
+    const double alpha = (tot / n) / n;
+    for (int e = tid; e < n * n; e += blockDim.x) M[e] = alpha;
     __syncthreads();
     for (int i = tid; i < n; i += blockDim.x)
-        for (int k = rp[i]; k < rp[i + 1]; k++) M[(size_t)i * w + col[k]] += v[k];
+        for (int k = rp[i]; k < rp[i + 1]; k++) M[i * n + col[k]] += v[k];
     __syncthreads();
-    for (int p = 0; p < n; p++) {
-        if (tid == 0) piv_s = 1.0 / M[(size_t)p * w + p];
+    for (int k = 0; k < n; k++) {
+        const double akk = M[k * n + k];
+        if (tid < n) colk[tid] = M[tid * n + k];
         __syncthreads();
-        for (int j = tid; j < w; j += blockDim.x) M[(size_t)p * w + j] *= piv_s;
+        if (tid < n) M[k * n + tid] = ((tid == k) ? 1.0 : M[k * n + tid]) / akk;
         __syncthreads();
-        for (int e = tid; e < n * w; e += blockDim.x) {
-            const int i = e / w, j = e % w;
-            if (i == p) continue;
-            const double f = M[(size_t)i * w + p];
-            if (j == p) continue;
-            M[e] -= f * M[(size_t)p * w + j];
+        for (int e = tid; e < n * n; e += blockDim.x) {
+            const int i = e / n, j = e - (e / n) * n;
+            if (i == k) continue;
+            const double mij = (j == k) ? 0.0 : M[e];
+            M[e] = mij - colk[i] * M[k * n + j];
         }
         __syncthreads();
-        for (int i = tid; i < n; i += blockDim.x) if (i != p) M[(size_t)i * w + p] = 0.0;
-        __syncthreads();
     }
-    for (int e = tid; e < n * n; e += blockDim.x) {
-        const int i = e / n, j = e % n;
-        inv[e] = M[(size_t)i * w + n + j];
-    }
+    for (int e = tid; e < n * n; e += blockDim.x) inv[e] = M[e];
 }
 
 int amg_setup(Ctx *c, const double *vals)
@@ -711,7 +711,6 @@ int amg_setup(Ctx *c, const double *vals)
     const int nco = c->amg.back().n;
     c->amg_coarse_n = nco;
     MC_TRY(dalloc(c, &c->amg_dense, (size_t)nco * nco));
-    MC_TRY(dalloc(c, &c->amg_work, (size_t)nco * 2 * nco));
     c->amg_ready = true;
     return amg_update_values(c, vals);
 }
@@ -735,8 +734,9 @@ int amg_update_values(Ctx *c, const double *vals)
         }
     }
     AmgLevel &Lc = c->amg.back();
+    if (Lc.n > DENSE_MAX) { c->err = "coarsest AMG level larger than the dense limit"; return MC_ERR_STATE; }
     hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(1024), 0, c->stream, Lc.n, Lc.row_ptr, Lc.col, Lc.vals,
-                       c->amg_work, c->amg_dense);
+                       c->amg_dense);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -1198,6 +1198,11 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     c->st.p_solves += 1;
     c->st.p_outer += s.outer;
     c->st.last_p_relres = s.relres;
+    {
+        const double ref = std::max(s.bnorm, atol / c->opt.p_rtol);
+        const double eff = ref > 0 ? s.relres * s.bnorm / ref : 0.0;
+        c->st.p_relres_max = std::max(c->st.p_relres_max, eff);
+    }
     if (st) *st = s;
     if (!s.converged) {
         c->err = "pressure solve did not converge";

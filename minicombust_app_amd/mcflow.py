@@ -52,7 +52,8 @@ class Stats(C.Structure):
         ("simple_iterations", C.c_int64), ("mom_iters", C.c_int64), ("scalar_iters", C.c_int64),
         ("p_iters", C.c_int64), ("p_solves", C.c_int64), ("p_loops", C.c_int64), ("p_outer", C.c_int64),
         ("last_p_relres", C.c_double), ("p_solve_ms", C.c_double), ("last_p_max", C.c_double),
-        ("last_loops", C.c_int),
+        ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
+        ("clip_passes", C.c_int64),
     ]
 
 

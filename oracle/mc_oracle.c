@@ -265,6 +265,304 @@ int mco_pcg_singular(int64_t n, const int64_t *rp, const int64_t *col, const dou
     return it;
 }
 
+/* ----------------------------------------------------------------------
+ * Aggregation-AMG preconditioned CG (CPU).  The reference preconditions its
+ * pressure solve with PETSc GAMG (FlowSolver.hpp:521-527); this plain-C
+ * unsmoothed-aggregation V-cycle gives the CPU path a multigrid solve of the
+ * same class: three passes of greedy pairwise matching per level (strongest
+ * unmatched neighbour), damped Jacobi (2 sweeps before and after), scaled
+ * coarse correction, dense Cholesky of the regularised coarsest operator.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int64_t n, nnz;
+    int64_t *rp, *col;
+    double *val, *dinv;
+    int64_t nc;
+    int64_t *agg;           /* aggregate of each row */
+    double *x, *b, *t, *r;
+} amg_level;
+
+typedef struct {
+    int nlev;
+    amg_level lev[32];
+    double *chol;           /* coarsest: Cholesky factor (n x n, lower) */
+} amg_h;
+
+static amg_h *g_amg = NULL;
+static const int64_t *g_amg_rp = NULL;
+
+static void csr_free(amg_level *L)
+{
+    free(L->rp); free(L->col); free(L->val); free(L->dinv); free(L->agg);
+    free(L->x); free(L->b); free(L->t); free(L->r);
+}
+
+static void pairwise(int64_t n, const int64_t *rp, const int64_t *col, const double *val, int64_t *agg, int64_t *nc)
+{
+    for (int64_t i = 0; i < n; i++) agg[i] = -1;
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        int64_t best = -1;
+        double bw = 0.0;
+        for (int64_t e = rp[i]; e < rp[i + 1]; e++) {
+            const int64_t j = col[e];
+            if (j == i || agg[j] >= 0) continue;
+            const double w = -val[e];
+            if (w > bw * (1.0 + 1e-12)) { best = j; bw = w; }
+        }
+        agg[i] = k;
+        if (best >= 0) agg[best] = k;
+        k++;
+    }
+    *nc = k;
+}
+
+/* coarse CSR of P^T A P for piecewise-constant P (columns sorted) */
+static void galerkin_c(int64_t n, const int64_t *rp, const int64_t *col, const double *val, const int64_t *agg,
+                       int64_t nc, int64_t **rpc, int64_t **colc, double **valc, int64_t *nnzc)
+{
+    int64_t *cnt = calloc(nc + 1, sizeof(int64_t)), *mem = malloc(n * sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) cnt[agg[i] + 1]++;
+    for (int64_t I = 0; I < nc; I++) cnt[I + 1] += cnt[I];
+    int64_t *pos = malloc(nc * sizeof(int64_t));
+    for (int64_t I = 0; I < nc; I++) pos[I] = cnt[I];
+    for (int64_t i = 0; i < n; i++) mem[pos[agg[i]]++] = i;
+    int64_t cap = rp[n] + 16, nz = 0;
+    int64_t *rc = malloc((nc + 1) * sizeof(int64_t)), *cc = malloc(cap * sizeof(int64_t));
+    double *vc = malloc(cap * sizeof(double));
+    int64_t *mark = malloc(nc * sizeof(int64_t));
+    for (int64_t I = 0; I < nc; I++) mark[I] = -1;
+    rc[0] = 0;
+    for (int64_t I = 0; I < nc; I++) {
+        const int64_t start = nz;
+        for (int64_t t = cnt[I]; t < cnt[I + 1]; t++) {
+            const int64_t i = mem[t];
+            for (int64_t e = rp[i]; e < rp[i + 1]; e++) {
+                const int64_t J = agg[col[e]];
+                if (mark[J] < start) { mark[J] = nz; cc[nz] = J; vc[nz] = 0.0; nz++; }
+                vc[mark[J]] += val[e];
+            }
+        }
+        /* sort the row by column */
+        for (int64_t a = start + 1; a < nz; a++) {
+            const int64_t cj = cc[a]; const double cv = vc[a];
+            int64_t b = a - 1;
+            while (b >= start && cc[b] > cj) { cc[b + 1] = cc[b]; vc[b + 1] = vc[b]; b--; }
+            cc[b + 1] = cj; vc[b + 1] = cv;
+        }
+        for (int64_t a = start; a < nz; a++) mark[cc[a]] = a;
+        rc[I + 1] = nz;
+    }
+    free(cnt); free(mem); free(pos); free(mark);
+    *rpc = rc; *colc = cc; *valc = vc; *nnzc = nz;
+}
+
+static void amg_level_vectors(amg_level *L)
+{
+    L->dinv = malloc(L->n * sizeof(double));
+    for (int64_t i = 0; i < L->n; i++) {
+        double d = 0.0;
+        for (int64_t e = L->rp[i]; e < L->rp[i + 1]; e++) if (L->col[e] == i) d = L->val[e];
+        L->dinv[i] = (d != 0.0) ? 1.0 / d : 0.0;
+    }
+    L->x = calloc(L->n, sizeof(double)); L->b = calloc(L->n, sizeof(double));
+    L->t = calloc(L->n, sizeof(double)); L->r = calloc(L->n, sizeof(double));
+}
+
+static amg_h *amg_build(int64_t n, const int64_t *rp, const int64_t *col, const double *val)
+{
+    amg_h *h = calloc(1, sizeof(amg_h));
+    amg_level *L = &h->lev[0];
+    L->n = n; L->nnz = rp[n];
+    L->rp = malloc((n + 1) * sizeof(int64_t)); memcpy(L->rp, rp, (n + 1) * sizeof(int64_t));
+    L->col = malloc(L->nnz * sizeof(int64_t)); memcpy(L->col, col, L->nnz * sizeof(int64_t));
+    L->val = malloc(L->nnz * sizeof(double)); memcpy(L->val, val, L->nnz * sizeof(double));
+    amg_level_vectors(L);
+    int l = 0;
+    while (h->lev[l].n > 256 && l < 30) {
+        amg_level *F = &h->lev[l];
+        int64_t *agg = malloc(F->n * sizeof(int64_t));
+        for (int64_t i = 0; i < F->n; i++) agg[i] = i;
+        int64_t gn = F->n, *grp = F->rp, *gcol = F->col, gnnz = F->nnz, nc = F->n;
+        double *gval = F->val;
+        int own = 0;
+        for (int pass = 0; pass < 3; pass++) {
+            int64_t *a1 = malloc(gn * sizeof(int64_t)), ncn = 0;
+            pairwise(gn, grp, gcol, gval, a1, &ncn);
+            for (int64_t i = 0; i < F->n; i++) agg[i] = a1[agg[i]];
+            int64_t *r2, *c2; double *v2;
+            galerkin_c(gn, grp, gcol, gval, a1, ncn, &r2, &c2, &v2, &gnnz);
+            if (own) { free(grp); free(gcol); free(gval); }
+            grp = r2; gcol = c2; gval = v2; gn = ncn; own = 1; nc = ncn;
+            free(a1);
+            if (nc <= 256) break;
+        }
+        F->agg = agg; F->nc = nc;
+        amg_level *C = &h->lev[l + 1];
+        C->n = nc; C->nnz = gnnz; C->rp = grp; C->col = gcol; C->val = gval;
+        amg_level_vectors(C);
+        l++;
+    }
+    h->nlev = l + 1;
+    /* coarsest: Cholesky of A + alpha 1 1^T, alpha = mean diagonal / n */
+    amg_level *C = &h->lev[l];
+    const int64_t nn = C->n;
+    double *M = calloc(nn * nn, sizeof(double));
+    double dsum = 0.0;
+    for (int64_t i = 0; i < nn; i++)
+        for (int64_t e = C->rp[i]; e < C->rp[i + 1]; e++) if (C->col[e] == i) dsum += C->val[e];
+    const double alpha = (dsum / nn) / nn;
+    for (int64_t i = 0; i < nn * nn; i++) M[i] = alpha;
+    for (int64_t i = 0; i < nn; i++)
+        for (int64_t e = C->rp[i]; e < C->rp[i + 1]; e++) M[i * nn + C->col[e]] += C->val[e];
+    for (int64_t j = 0; j < nn; j++) {
+        double s = M[j * nn + j];
+        for (int64_t k = 0; k < j; k++) s -= M[j * nn + k] * M[j * nn + k];
+        const double d = sqrt(s);
+        M[j * nn + j] = d;
+        for (int64_t i = j + 1; i < nn; i++) {
+            double t = M[i * nn + j];
+            for (int64_t k = 0; k < j; k++) t -= M[i * nn + k] * M[j * nn + k];
+            M[i * nn + j] = t / d;
+        }
+    }
+    h->chol = M;
+    return h;
+}
+
+static void amg_free(amg_h *h)
+{
+    if (!h) return;
+    for (int l = 0; l < h->nlev; l++) csr_free(&h->lev[l]);
+    free(h->chol);
+    free(h);
+}
+
+/* values of every level from the fine values (pattern unchanged) */
+static void amg_refresh(amg_h *h, const double *val)
+{
+    memcpy(h->lev[0].val, val, h->lev[0].nnz * sizeof(double));
+    for (int l = 0; l + 1 < h->nlev; l++) {
+        amg_level *F = &h->lev[l], *C = &h->lev[l + 1];
+        int64_t *r2, *c2, nz; double *v2;
+        galerkin_c(F->n, F->rp, F->col, F->val, F->agg, F->nc, &r2, &c2, &v2, &nz);
+        memcpy(C->val, v2, nz * sizeof(double));
+        free(r2); free(c2); free(v2);
+    }
+    for (int l = 0; l < h->nlev; l++) {
+        amg_level *L = &h->lev[l];
+        for (int64_t i = 0; i < L->n; i++) {
+            double d = 0.0;
+            for (int64_t e = L->rp[i]; e < L->rp[i + 1]; e++) if (L->col[e] == i) d = L->val[e];
+            L->dinv[i] = (d != 0.0) ? 1.0 / d : 0.0;
+        }
+    }
+    /* refactor the coarsest */
+    amg_level *C = &h->lev[h->nlev - 1];
+    const int64_t nn = C->n;
+    double *M = h->chol;
+    double dsum = 0.0;
+    for (int64_t i = 0; i < nn; i++)
+        for (int64_t e = C->rp[i]; e < C->rp[i + 1]; e++) if (C->col[e] == i) dsum += C->val[e];
+    const double alpha = (dsum / nn) / nn;
+    for (int64_t i = 0; i < nn * nn; i++) M[i] = alpha;
+    for (int64_t i = 0; i < nn; i++)
+        for (int64_t e = C->rp[i]; e < C->rp[i + 1]; e++) M[i * nn + C->col[e]] += C->val[e];
+    for (int64_t j = 0; j < nn; j++) {
+        double s = M[j * nn + j];
+        for (int64_t k = 0; k < j; k++) s -= M[j * nn + k] * M[j * nn + k];
+        const double d = sqrt(s);
+        M[j * nn + j] = d;
+        for (int64_t i = j + 1; i < nn; i++) {
+            double t = M[i * nn + j];
+            for (int64_t k = 0; k < j; k++) t -= M[i * nn + k] * M[j * nn + k];
+            M[i * nn + j] = t / d;
+        }
+    }
+}
+
+static void amg_vcycle(amg_h *h, int l, const double *b, double *x)
+{
+    amg_level *L = &h->lev[l];
+    const int64_t n = L->n;
+    const double om = 0.8, cs = 1.8;
+    if (l == h->nlev - 1) {
+        const double *M = h->chol;
+        for (int64_t i = 0; i < n; i++) {
+            double s = b[i];
+            for (int64_t k = 0; k < i; k++) s -= M[i * n + k] * x[k];
+            x[i] = s / M[i * n + i];
+        }
+        for (int64_t i = n - 1; i >= 0; i--) {
+            double s = x[i];
+            for (int64_t k = i + 1; k < n; k++) s -= M[k * n + i] * x[k];
+            x[i] = s / M[i * n + i];
+        }
+        return;
+    }
+    for (int64_t i = 0; i < n; i++) x[i] = om * L->dinv[i] * b[i];
+    for (int sweep = 1; sweep < 2; sweep++) {
+        mco_spmv(n, L->rp, L->col, L->val, x, L->t);
+        for (int64_t i = 0; i < n; i++) x[i] += om * L->dinv[i] * (b[i] - L->t[i]);
+    }
+    mco_spmv(n, L->rp, L->col, L->val, x, L->t);
+    amg_level *C = &h->lev[l + 1];
+    for (int64_t I = 0; I < C->n; I++) C->b[I] = 0.0;
+    for (int64_t i = 0; i < n; i++) C->b[L->agg[i]] += b[i] - L->t[i];
+    amg_vcycle(h, l + 1, C->b, C->x);
+    for (int64_t i = 0; i < n; i++) x[i] += cs * C->x[L->agg[i]];
+    for (int sweep = 0; sweep < 2; sweep++) {
+        mco_spmv(n, L->rp, L->col, L->val, x, L->t);
+        for (int64_t i = 0; i < n; i++) x[i] += om * L->dinv[i] * (b[i] - L->t[i]);
+    }
+}
+
+int mco_pcg_amg(int64_t n, const int64_t *rp, const int64_t *col, const double *A, const double *bin, double *x,
+                double rtol, double atol, int maxit, double *relres, double *bnorm_out)
+{
+    if (!g_amg || g_amg_rp != rp || g_amg->lev[0].n != n) {
+        amg_free(g_amg);
+        g_amg = amg_build(n, rp, col, A);
+        g_amg_rp = rp;
+    } else {
+        amg_refresh(g_amg, A);
+    }
+    double *b = malloc(n * sizeof(double)), *r = malloc(n * sizeof(double));
+    double *z = malloc(n * sizeof(double)), *p = malloc(n * sizeof(double)), *q = malloc(n * sizeof(double));
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; i++) mean += bin[i];
+    mean /= (double)n;
+    for (int64_t i = 0; i < n; i++) b[i] = bin[i] - mean;
+    const double bnorm = sqrt(dot(n, b, b));
+    const double tol = (rtol * bnorm > atol) ? rtol * bnorm : atol;
+    if (bnorm_out) *bnorm_out = bnorm;
+    for (int64_t i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; p[i] = 0.0; }
+    double rn = bnorm, rz_old = 0.0;
+    int it = 0;
+    if (bnorm > tol) {
+        for (it = 1; it <= maxit; it++) {
+            amg_vcycle(g_amg, 0, r, z);
+            const double rz = dot(n, r, z);
+            const double beta = (it == 1) ? 0.0 : rz / rz_old;
+            rz_old = rz;
+            for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+            mco_spmv(n, rp, col, A, p, q);
+            const double alpha = rz / dot(n, p, q);
+            for (int64_t i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; }
+            rn = sqrt(dot(n, r, r));
+            if (rn <= tol) break;
+        }
+    }
+    double xm = 0.0;
+    for (int64_t i = 0; i < n; i++) xm += x[i];
+    xm /= (double)n;
+    for (int64_t i = 0; i < n; i++) x[i] -= xm;
+    if (relres) *relres = (bnorm > 0.0) ? rn / bnorm : 0.0;
+    free(b); free(r); free(z); free(p); free(q);
+    return it;
+}
+
 /* ======================================================================
  * Constructor (include/flow/FlowSolver.hpp:147-539)
  * ====================================================================== */
@@ -277,6 +575,7 @@ void mco_default_opts(mco_opts *o)
     o->p_rtol = 1.0e-8;
     o->p_maxit = 100000;
     o->solve_scalars = 1;
+    o->p_solver = 0;
 }
 
 static void setup_sparse_matrix(mco_state *s, double URFactor, double *A_c, double *phi_c, double *S_c);
@@ -932,7 +1231,8 @@ void mco_solve_pressure_matrix(mco_state *s)
     if (s->first_pressure || s->opts.mode == MCO_MODE_SIMPLE) memcpy(s->Pb, s->S_U, N * sizeof(double));
     else memcpy(s->b, s->S_U, N * sizeof(double));
     double rr = 0.0;
-    const int it = mco_pcg_singular(N, s->row_ptr, s->col, s->PA_vals, s->Pb, s->u, s->opts.p_rtol, s->p_atol,
+    const int it = (s->opts.p_solver == 1 ? mco_pcg_amg : mco_pcg_singular)(
+                                    N, s->row_ptr, s->col, s->PA_vals, s->Pb, s->u, s->opts.p_rtol, s->p_atol,
                                     s->opts.p_maxit, &rr, &s->p_bnorm_last);
     s->stats.p_iters += it;
     s->stats.p_solves += 1;

@@ -70,6 +70,7 @@ typedef struct mco_opts {
     double p_rtol;           /* pressure-correction relative tolerance        */
     int    p_maxit;
     int    solve_scalars;    /* 1: run the 7 scalar transport solves          */
+    int    p_solver;         /* 0: Jacobi-PCG (checker), 1: aggregation-AMG PCG */
 } mco_opts;
 
 typedef struct mco_stats {
@@ -155,6 +156,9 @@ int  mco_bicgstab(int64_t n, const int64_t *row_ptr, const int64_t *col,
 int  mco_pcg_singular(int64_t n, const int64_t *row_ptr, const int64_t *col,
                       const double *vals, const double *b, double *x,
                       double rtol, double atol, int maxit, double *relres, double *bnorm);
+int  mco_pcg_amg(int64_t n, const int64_t *row_ptr, const int64_t *col,
+                 const double *vals, const double *b, double *x,
+                 double rtol, double atol, int maxit, double *relres, double *bnorm);
 void mco_spmv(int64_t n, const int64_t *row_ptr, const int64_t *col,
               const double *vals, const double *x, double *y);
 

@@ -40,7 +40,7 @@ class Mesh(C.Structure):
 class Opts(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("delta", C.c_double), ("mom_rtol", C.c_double), ("mom_maxit", C.c_int),
-        ("p_rtol", C.c_double), ("p_maxit", C.c_int), ("solve_scalars", C.c_int),
+        ("p_rtol", C.c_double), ("p_maxit", C.c_int), ("solve_scalars", C.c_int), ("p_solver", C.c_int),
     ]
 
 
