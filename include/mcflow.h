@@ -98,6 +98,7 @@ typedef struct mc_stats {
                                  was set against (loop 1: its own RHS)           */
     int64_t clip_cells;       /* TE/ED cells repaired to non-negative values     */
     int64_t clip_passes;      /* relaxation passes of that repair                */
+    int64_t host_polls;       /* host waits on solver convergence flags          */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>
@@ -203,6 +204,10 @@ int  mc_calculate_pressure(mc_ctx *ctx);
    (gpu_kernels.cuh:60-66,36) and the solve.                                  */
 int  mc_scalar_assemble(mc_ctx *ctx, int type);
 int  mc_scalar_solve(mc_ctx *ctx, int type);
+/* The four passive-scalar Scalar_solve calls (FUEL, PROG, VARFU, VARPR,
+   FlowSolver.inl:3243-3260): their matrices are identical, so one assembly
+   and one 4-RHS BiCGStab give the four results of the sequential calls.     */
+int  mc_scalar_batch_solve(mc_ctx *ctx);
 /* C_kernel_Update_P_at_boundaries (gpu_kernels.cuh:46) and C_kernel_Update_P
    (gpu_kernels.cuh:52) on a field.                                          */
 int  mc_update_boundary_slots(mc_ctx *ctx, int field);

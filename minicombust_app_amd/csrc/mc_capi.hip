@@ -104,6 +104,7 @@ static int finish_ctor(Ctx *c)
 {
     MC_TRY(mesh_finish(c));
     MC_TRY(alloc_workspace(c));
+    MC_TRY(dia_setup_fine(c));
     c->first_pressure = true;
     c->timestep_count = 0;
     MC_HIP(c, hipStreamSynchronize(c->stream));
@@ -298,6 +299,8 @@ int mc_scalar_solve(mc_ctx *x, int type)
     if (type < 0 || type > 6) return MC_ERR_ARG;
     return scalar_solve(c, type);
 }
+
+int mc_scalar_batch_solve(mc_ctx *x) { CTX_OR_FAIL return scalar_batch_solve(c); }
 
 int mc_update_boundary_slots(mc_ctx *x, int field)
 {

@@ -39,10 +39,18 @@ struct Consts {
     double cmu75;         // 0.09^0.75
 };
 
+constexpr int DIA_MAX = 32;     // most diagonals a diagonal-storage matrix may have
+constexpr int DENSE_MAX = 128;  // largest coarsest AMG level (inverted in LDS)
+
 // Per-level storage of the aggregation-AMG hierarchy.
 struct AmgLevel {
     int n = 0, nnz = 0;
     int *row_ptr = nullptr, *col = nullptr;   // CSR pattern of this level
+    int nd = 0;                               // diagonals of the diagonal copy (0: none)
+    int off[DIA_MAX] = {};
+    int *dia_map = nullptr;                   // CSR index -> d*n + i
+    double *dia = nullptr;
+    float *dia32 = nullptr;
     double *vals = nullptr;                   // FP64 values
     float *vals32 = nullptr;                  // FP32 copy (mixed form)
     double *dinv = nullptr;                   // 1/diag
@@ -106,6 +114,10 @@ struct Ctx {
     float *PA_vals32 = nullptr;
     double *Pb = nullptr;          // pressure RHS (compat: frozen copy)
     double *u = nullptr;           // solution scratch
+    int fine_nd = 0;               // diagonal copy of the fine pattern (0: CSR kernels)
+    int fine_off[DIA_MAX] = {};
+    int *fine_map = nullptr;
+    double *A_dia = nullptr;       // diagonal copy of the momentum / scalar matrix
     bool first_pressure = true;
     bool field_set_up = false;   // set_up_field has run (first pass of timestep 0)
     int64_t timestep_count = 0;
@@ -294,6 +306,7 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
 int amg_setup(Ctx *c, const double *vals);
 int amg_update_values(Ctx *c, const double *vals);
 int spmv(Ctx *c, const double *vals, const double *x, double *y);
+int dia_setup_fine(Ctx *c);
 int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
             double atol, mc_solve_status *st);
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,

@@ -4,25 +4,35 @@
 // 716-782, 943-969) and PETSc's KSPBCGS / KSPFGMRES+PCGAMG of the CPU path
 // (FlowSolver.hpp:513-527, FlowSolver.inl:1341-1375, 2203-2273).
 //
-//  * SpMV: CSR, 8 lanes per row (one wavefront = 8 rows), the row sum is
-//    reduced with 3 cross-lane shuffles; matrix values and columns stream
-//    once per call, coalesced.  NR right-hand sides share one matrix read.
-//  * Reductions: fixed-order two-level block sums finished by the last block
-//    (deterministic, no host round trip).  Solver scalars live in device
-//    memory; kernels of an already-converged system return at once, so the
-//    host only polls a flag every few iterations.
+// Storage.  Matrices are assembled in the PETSc CSR pattern.  When a pattern
+// has exactly 7 distinct diagonals (the box mesh's 7-point coupling) or 27
+// (a coarse level of the regular 2x2x2 aggregation of that mesh), the solver
+// also keeps a diagonal-indexed copy, v[d*n + i] = A[i, i + off_d] (zero
+// where the pattern has no entry): one thread per row, no column indices to
+// stream, and all ND loads of a row independent and coalesced.  Any other
+// pattern runs the CSR kernels: 8 lanes per row, the row sum reduced with 3
+// cross-lane shuffles.  A ND-template drives every kernel through for_rows().
+//
+// Reductions: fixed-order two-level block sums finished by the last block
+// (deterministic, no host round trip).  Solver scalars live in device memory;
+// kernels of a converged system return at once, and the host polls a flag
+// after a number of iterations predicted from the observed convergence rate.
+//
 //  * BiCGStab: van der Vorst's unpreconditioned method, x0 = 0 (momentum and
-//    scalar equations; the reference's KSPBCGS+PCNONE).
+//    scalar equations; the reference's KSPBCGS+PCNONE), up to 4 RHS sharing
+//    each matrix read.
 //  * Pressure: PCG preconditioned by an unsmoothed-aggregation AMG V-cycle
-//    (damped Jacobi, symmetric nu/nu sweeps, scaled coarse correction, dense
-//    regularised inverse on the coarsest level), on the singular Neumann
-//    system with the RHS projected to mean zero; mixed form: FP32 PCG-AMG
-//    inside FP64 iterative refinement.
+//    (damped Jacobi nu/nu, scaled coarse correction, dense regularised inverse
+//    on the coarsest level) on the singular Neumann system with the RHS
+//    projected to mean zero.  Mixed form: FP32 PCG-AMG (SpMV, smoothers,
+//    hierarchy) inside FP64 iterative refinement; the refinement residual,
+//    the update and the coarsest solve stay FP64.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <set>
 
 #include "mc_internal.hpp"
 
@@ -30,23 +40,21 @@ namespace mc {
 
 // ------------------------------------------------------------ layout --
 constexpr int RED_MAX_BLOCKS = 2048;
-constexpr int ROWS_PER_WAVE = 8;               // 8 lanes per row
 constexpr int BLOCK = 256;
-constexpr int ROWS_PER_BLOCK = BLOCK / 8;      // 32
 
 struct Grid {
     int blocks;
-    int chunk;   // rows (or elements) per block, multiple of the granule
+    int chunk;   // rows per block (multiple of BLOCK)
 };
 
-static Grid grid_for(int n, int granule)
+static Grid grid_for(int n)
 {
-    const int64_t g = div_up(n, granule);
+    const int64_t g = div_up(n, BLOCK);
     int blocks = (int)std::min<int64_t>(g, RED_MAX_BLOCKS);
     if (blocks < 1) blocks = 1;
     const int64_t per = div_up(g, blocks);
     Grid gr;
-    gr.chunk = (int)(per * granule);
+    gr.chunk = (int)(per * BLOCK);
     gr.blocks = div_up(n, gr.chunk);
     if (gr.blocks < 1) gr.blocks = 1;
     return gr;
@@ -61,60 +69,165 @@ struct KState {
     int all_done, nr, maxit, pad;
 };
 
-template <int NR>
-__device__ __forceinline__ bool all_done_dev(const KState *ks)
+// ------------------------------------------------------- matrix view --
+template <typename T>
+struct Mat {
+    int n;
+    const int *rp, *col;
+    const T *csr;
+    const T *dia;          // diagonal storage, nullptr -> CSR kernels
+    int nd;
+    int off[DIA_MAX];
+};
+
+__device__ __forceinline__ int clampi(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
+
+// Visit the rows [r0, r1) handled by this thread and call f(row, A x) once
+// per row.  ND > 0: one thread per row over ND diagonals (compile-time, fully
+// unrolled); ND == 0: CSR, 8 lanes per row, lane 0 calls f.  X(k, j) yields
+// operand k at column j, so fused producers (prolongation, p-update) feed the
+// product without a separate pass.
+template <int ND, int NR, typename T, typename XF, typename F>
+__device__ __forceinline__ void for_rows(const Mat<T> &A, int r0, int r1, XF X, F f)
 {
-    return ks->all_done != 0;
+    if constexpr (ND > 0) {
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += BLOCK) {
+            double acc[NR];
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = 0.0;
+            double a[ND];
+            int j[ND];
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                a[d] = (double)A.dia[(size_t)d * A.n + i];
+                j[d] = clampi(i + A.off[d], A.n);
+            }
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+#pragma unroll
+                for (int k = 0; k < NR; k++) acc[k] += a[d] * X(k, j[d]);
+            }
+            f(i, acc);
+        }
+    } else {
+        const int lane8 = threadIdx.x & 7;
+        for (int base = r0; base < r1; base += BLOCK / 8) {
+            const int r = base + (threadIdx.x >> 3);
+            double acc[NR];
+#pragma unroll
+            for (int k = 0; k < NR; k++) acc[k] = 0.0;
+            if (r < r1) {
+                const int beg = A.rp[r], end = A.rp[r + 1];
+                for (int e = beg + lane8; e < end; e += 8) {
+                    const int jj = A.col[e];
+                    const double a = (double)A.csr[e];
+#pragma unroll
+                    for (int k = 0; k < NR; k++) acc[k] += a * X(k, jj);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                double v = acc[k];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                acc[k] = v;
+            }
+            if (lane8 == 0 && r < r1) f(r, acc);
+        }
+    }
+}
+
+// Launch a kernel template over the supported diagonal counts; KERNEL names
+// the count ND_ (7, 27, or 0 for the CSR path).
+#define MC_DISPATCH_ND(nd, KERNEL, ...)                                                   \
+    do {                                                                                  \
+        if ((nd) == 7) { constexpr int ND_ = 7; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }   \
+        else if ((nd) == 27) { constexpr int ND_ = 27; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); } \
+        else { constexpr int ND_ = 0; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }          \
+    } while (0)
+
+// ------------------------------------------ CSR -> diagonal storage --
+// Offsets of a pattern; diagonal storage when there are exactly 7 or 27.
+static bool dia_analyze(int n, const int *rp, const int *col, std::vector<int> &offs, std::vector<int> &map)
+{
+    std::set<int> s;
+    for (int i = 0; i < n; i++)
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            s.insert(col[k] - i);
+            if (s.size() > 27) return false;
+        }
+    if (s.size() != 7 && s.size() != 27) return false;
+    offs.assign(s.begin(), s.end());
+    map.resize(rp[n]);
+    for (int i = 0; i < n; i++)
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            const int d = (int)(std::lower_bound(offs.begin(), offs.end(), col[k] - i) - offs.begin());
+            map[k] = d * n + i;
+        }
+    return true;
+}
+
+__global__ void k_csr_to_dia(int nnz, const int *map, const double *csr, double *dia, float *dia32)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const double v = csr[k];
+    dia[map[k]] = v;
+    if (dia32) dia32[map[k]] = (float)v;
+}
+
+int dia_setup_fine(Ctx *c)
+{
+    std::vector<int> rp(c->N + 1), col(c->nnz), offs, map;
+    MC_HIP(c, hipMemcpy(rp.data(), c->row_ptr, (size_t)(c->N + 1) * 4, hipMemcpyDeviceToHost));
+    MC_HIP(c, hipMemcpy(col.data(), c->col, (size_t)c->nnz * 4, hipMemcpyDeviceToHost));
+    c->fine_nd = 0;
+    if (!dia_analyze(c->N, rp.data(), col.data(), offs, map)) return MC_OK;
+    c->fine_nd = (int)offs.size();
+    for (int d = 0; d < c->fine_nd; d++) c->fine_off[d] = offs[d];
+    MC_TRY(dalloc(c, &c->fine_map, c->nnz));
+    MC_HIP(c, hipMemcpy(c->fine_map, map.data(), (size_t)c->nnz * 4, hipMemcpyHostToDevice));
+    MC_TRY(dalloc(c, &c->A_dia, (size_t)c->fine_nd * c->N));
+    MC_HIP(c, hipMemset(c->A_dia, 0, (size_t)c->fine_nd * c->N * 8));
+    return MC_OK;
+}
+
+static Mat<double> fine_mat(Ctx *c, const double *csr, const double *dia)
+{
+    Mat<double> A;
+    A.n = c->N; A.rp = c->row_ptr; A.col = c->col; A.csr = csr;
+    A.dia = c->fine_nd ? dia : nullptr;
+    A.nd = c->fine_nd;
+    for (int d = 0; d < DIA_MAX; d++) A.off[d] = d < c->fine_nd ? c->fine_off[d] : 0;
+    return A;
+}
+
+// Momentum / scalar matrix: refresh the diagonal copy of A_vals.
+static int fine_dia_from(Ctx *c, const double *csr, double *dia)
+{
+    if (!c->fine_nd) return MC_OK;
+    hipLaunchKernelGGL(k_csr_to_dia, dim3(div_up(c->nnz, 256)), dim3(256), 0, c->stream, c->nnz, c->fine_map, csr, dia,
+                       (float *)nullptr);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
 }
 
 // ------------------------------------------------------------ SpMV --
-// y_k[r] = sum_j A[r,j] x_k[j] for the rows of one 8-lane group.
-template <int NR, typename T, typename VT>
-__device__ __forceinline__ void csr_row8(int r, int n, const int *__restrict__ rp, const int *__restrict__ col,
-                                         const T *__restrict__ vals, const VT *const *x, double (&acc)[NR])
-{
-    const int lane8 = threadIdx.x & 7;
-#pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    if (r < n) {
-        const int beg = rp[r], end = rp[r + 1];
-        for (int e = beg + lane8; e < end; e += 8) {
-            const int j = col[e];
-            const double a = (double)vals[e];
-#pragma unroll
-            for (int k = 0; k < NR; k++) acc[k] += a * (double)x[k][j];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < NR; k++) {
-        double v = acc[k];
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        acc[k] = v;
-    }
-}
-
-template <typename T, typename VT>
-__global__ void __launch_bounds__(BLOCK) k_spmv(int n, const int *rp, const int *col, const T *vals, const VT *x,
-                                                VT *y, int chunk)
+template <int ND>
+__global__ void __launch_bounds__(BLOCK) k_spmv(Mat<double> A, const double *x, double *y, int chunk)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
-    const VT *xs[1] = {x};
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc[1];
-        csr_row8<1, T, VT>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
-        if ((threadIdx.x & 7) == 0 && r < r1) y[r] = (VT)acc[0];
-    }
+    const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return x[j]; }, [&](int i, const double (&ax)[1]) { y[i] = ax[0]; });
 }
 
 int spmv(Ctx *c, const double *vals, const double *x, double *y)
 {
-    const Grid g = grid_for(c->N, ROWS_PER_BLOCK);
-    hipLaunchKernelGGL((k_spmv<double, double>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, c->row_ptr, c->col,
-                       vals, x, y, g.chunk);
+    const Grid g = grid_for(c->N);
+    if (c->fine_nd) MC_TRY(fine_dia_from(c, vals, c->A_dia));
+    const Mat<double> A = fine_mat(c, vals, c->A_dia);
+    MC_DISPATCH_ND(A.nd, (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, x, y, g.chunk);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -151,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, 
             const double bn = sqrt(acc[k]);
             ks->bnorm[k] = bn; ks->tol[k] = rtol * bn; ks->rnorm[k] = bn;
             ks->rho_old[k] = 1.0; ks->alpha[k] = 1.0; ks->omega[k] = 1.0;
-            ks->rho[k] = acc[k];  // (rh, r) with rh = r = b
+            ks->rho[k] = acc[k];  // (r^, r) with r^ = r = b
             ks->beta[k] = (ks->rho[k] / 1.0) * (1.0 / 1.0);
             ks->iters[k] = 0; ks->half[k] = 0;
             ks->done[k] = (bn == 0.0) ? 1 : 0;
@@ -181,33 +294,25 @@ __global__ void __launch_bounds__(BLOCK) k_bi_p(int n, BiVecs w, const KState *k
     }
 }
 
-// v = A p ; alpha = rho / (rh, v)
-template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_spmv_v(int n, const int *rp, const int *col, const double *vals, BiVecs w,
-                                                     KState *ks, double *partials, unsigned *cnt, int chunk)
+// v = A p ; alpha = rho / (r^, v)
+template <int ND, int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_spmv_v(Mat<double> A, BiVecs w, KState *ks, double *partials,
+                                                     unsigned *cnt, int chunk)
 {
     if (ks->all_done) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
+    const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
     double dsum[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) dsum[k] = 0.0;
-    const double *xs[NR];
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return w.p[k][j]; }, [&](int r, const double (&ax)[NR]) {
 #pragma unroll
-    for (int k = 0; k < NR; k++) xs[k] = w.p[k];
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc[NR];
-        csr_row8<NR, double, double>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
-        if ((threadIdx.x & 7) == 0 && r < r1) {
-#pragma unroll
-            for (int k = 0; k < NR; k++) {
-                if (ks->done[k]) continue;
-                w.v[k][r] = acc[k];
-                dsum[k] += w.rh[k][r] * acc[k];
-            }
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k]) continue;
+            w.v[k][r] = ax[k];
+            dsum[k] += w.rh[k][r] * ax[k];
         }
-    }
+    });
     if (block_reduce_last<NR>(dsum, partials, cnt, b, gridDim.x)) {
         for (int k = 0; k < NR; k++)
             if (!ks->done[k]) ks->alpha[k] = ks->rho[k] / dsum[k];
@@ -244,33 +349,25 @@ __global__ void __launch_bounds__(BLOCK) k_bi_s(int n, BiVecs w, KState *ks, dou
 }
 
 // t = A s ; omega = (t,s)/(t,t)
-template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_spmv_t(int n, const int *rp, const int *col, const double *vals, BiVecs w,
-                                                     KState *ks, double *partials, unsigned *cnt, int chunk)
+template <int ND, int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_spmv_t(Mat<double> A, BiVecs w, KState *ks, double *partials,
+                                                     unsigned *cnt, int chunk)
 {
     if (ks->all_done) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = b * chunk, r1 = min(n, r0 + chunk);
+    const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
     double dsum[2 * NR];
 #pragma unroll
     for (int k = 0; k < 2 * NR; k++) dsum[k] = 0.0;
-    const double *xs[NR];
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return w.s[k][j]; }, [&](int r, const double (&ax)[NR]) {
 #pragma unroll
-    for (int k = 0; k < NR; k++) xs[k] = w.s[k];
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc[NR];
-        csr_row8<NR, double, double>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
-        if ((threadIdx.x & 7) == 0 && r < r1) {
-#pragma unroll
-            for (int k = 0; k < NR; k++) {
-                if (ks->done[k] || ks->half[k]) continue;
-                w.t[k][r] = acc[k];
-                dsum[2 * k] += acc[k] * w.s[k][r];
-                dsum[2 * k + 1] += acc[k] * acc[k];
-            }
+        for (int k = 0; k < NR; k++) {
+            if (ks->done[k] || ks->half[k]) continue;
+            w.t[k][r] = ax[k];
+            dsum[2 * k] += ax[k] * w.s[k][r];
+            dsum[2 * k + 1] += ax[k] * ax[k];
         }
-    }
+    });
     if (block_reduce_last<2 * NR>(dsum, partials, cnt, b, gridDim.x)) {
         for (int k = 0; k < NR; k++)
             if (!ks->done[k] && !ks->half[k]) ks->omega[k] = dsum[2 * k] / dsum[2 * k + 1];
@@ -336,12 +433,23 @@ static int poll_done(Ctx *c, KState *ks, KState *host)
     return MC_OK;
 }
 
+// Iterations to enqueue before the next host poll: the remaining count
+// predicted from the last observed residual reduction rate (at least 1).
+static int predict_batch(double rn_prev, double rn, int its, double tol, int cap)
+{
+    if (its <= 0 || !(rn > 0) || !(rn_prev > 0) || rn >= rn_prev) return 1;
+    const double rate = std::pow(rn / rn_prev, 1.0 / its);
+    const double need = std::log(tol / rn) / std::log(rate);
+    int k = (int)std::floor(need);
+    return std::max(1, std::min(k, cap));
+}
+
 template <int NR>
 static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *const *x, double rtol, int maxit,
                        mc_solve_status *st)
 {
-    const int n = c->N;
     static_assert(NR <= 4, "NR");
+    const int n = c->N;
     BiVecs w{};
     // five work vectors per RHS (r, p, v, s, t) from the 16 FP64 work vectors
     // and the 8 scratch vectors (sized for doubles); r^ aliases b
@@ -354,44 +462,53 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
         w.b[k] = b[k]; w.x[k] = x[k]; w.rh[k] = b[k];
         w.r[k] = pool[q++]; w.p[k] = pool[q++]; w.v[k] = pool[q++]; w.s[k] = pool[q++]; w.t[k] = pool[q++];
     }
+    MC_TRY(fine_dia_from(c, vals, c->A_dia));
+    const Mat<double> A = fine_mat(c, vals, c->A_dia);
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
-    const Grid gv = grid_for(n, BLOCK);
-    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
-    hipLaunchKernelGGL(k_bi_init<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                       gv.chunk, rtol, maxit);
+    const Grid g = grid_for(n);
+    hipLaunchKernelGGL(k_bi_init<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt, g.chunk,
+                       rtol, maxit);
     MC_LAUNCH_CHECK(c);
-    const int check = std::max(1, c->opt.check_every);
-    int it = 0;
+    const double mbytes = (c->fine_nd ? 8.0 * c->nnz + 16.0 * n : 12.0 * c->nnz + 20.0 * n) + 8.0 * n * (NR - 1) * 2;
+    int it = 0, batch = 1;
+    double rn_prev = -1.0;
     for (;;) {
-        for (int j = 0; j < check && it < maxit; j++, it++) {
-            hipLaunchKernelGGL(k_bi_p<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, gv.chunk);
+        for (int j = 0; j < batch && it < maxit; j++, it++) {
+            hipLaunchKernelGGL(k_bi_p<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, g.chunk);
             {
-                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 4.0 * n + 24.0 * n * NR);
-                hipLaunchKernelGGL(k_bi_spmv_v<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
-                                   vals, w, ks, c->red, c->red_cnt, gm.chunk);
+                KTimer kt(c, MC_KC_SPMV64, mbytes);
+                MC_DISPATCH_ND(A.nd, (k_bi_spmv_v<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w, ks,
+                               c->red, c->red_cnt, g.chunk);
             }
-            hipLaunchKernelGGL(k_bi_s<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                               gv.chunk);
+            hipLaunchKernelGGL(k_bi_s<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               g.chunk);
             {
-                KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 4.0 * n + 24.0 * n * NR);
-                hipLaunchKernelGGL(k_bi_spmv_t<NR>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col,
-                                   vals, w, ks, c->red, c->red_cnt, gm.chunk);
+                KTimer kt(c, MC_KC_SPMV64, mbytes);
+                MC_DISPATCH_ND(A.nd, (k_bi_spmv_t<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w, ks,
+                               c->red, c->red_cnt, g.chunk);
             }
-            hipLaunchKernelGGL(k_bi_xr<NR>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                               gv.chunk);
+            hipLaunchKernelGGL(k_bi_xr<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                               g.chunk);
         }
         MC_LAUNCH_CHECK(c);
         MC_TRY(poll_done(c, ks, hs));
+        c->st.host_polls++;
         if (hs->all_done || it >= maxit) break;
+        double rn = 0.0;
+        for (int k = 0; k < NR; k++)
+            if (!hs->done[k]) rn = std::max(rn, hs->rnorm[k] / std::max(hs->tol[k], 1e-300));
+        batch = predict_batch(rn_prev, rn, batch, 1.0, 8);
+        rn_prev = rn;
     }
     if (st) {
-        st->iterations = 0; st->converged = 1; st->relres = 0.0; st->outer = 0; st->guess_used = 0;
+        st->iterations = 0; st->converged = 1; st->relres = 0.0; st->outer = 0; st->guess_used = 0; st->bnorm = 0.0;
         for (int k = 0; k < NR; k++) {
             st->iterations += hs->iters[k];
             st->converged &= hs->conv[k];
             const double rr = hs->bnorm[k] > 0 ? hs->rnorm[k] / hs->bnorm[k] : 0.0;
             st->relres = std::max(st->relres, rr);
+            st->bnorm = std::max(st->bnorm, hs->bnorm[k]);
         }
     }
     return MC_OK;
@@ -428,7 +545,7 @@ __global__ void k_sub_scalar(int n, const double *in, const double *s, double *o
 
 int mean_project(Ctx *c, const double *b, double *out, int n)
 {
-    const Grid g = grid_for(n, BLOCK);
+    const Grid g = grid_for(n);
     double *mean = c->dscal + 512;
     hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 8, g.chunk,
                        1.0 / (double)n);
@@ -475,7 +592,7 @@ __global__ void __launch_bounds__(BLOCK) k_maxabs(int n, const double *v, double
 
 int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host)
 {
-    const Grid g = grid_for(c->N, BLOCK);
+    const Grid g = grid_for(c->N);
     double *d = c->dscal + 520;
     hipLaunchKernelGGL(k_maxabs, dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, v, d, c->red, c->red_cnt + 9, g.chunk);
     MC_LAUNCH_CHECK(c);
@@ -488,7 +605,7 @@ int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host)
 // =================================================================== AMG ==
 // Host-side aggregation: three passes of greedy pairwise matching (strongest
 // unmatched neighbour, ties to the lowest column) per level -> aggregates of
-// about 8 cells.  Done once per context; later matrices on the same pattern
+// at most 8 cells.  Done once per context; later matrices on the same pattern
 // only recompute the Galerkin values on the GPU.
 struct HostCSR {
     int n = 0;
@@ -521,7 +638,6 @@ static void pairwise_pass(const HostCSR &A, std::vector<int> &agg, int &nc)
 static void galerkin(const HostCSR &A, const std::vector<int> &agg, int nc, HostCSR &Ac, std::vector<int> *gptr,
                      std::vector<int> *gmap)
 {
-    // members of each aggregate
     std::vector<int> mp(nc + 1, 0), mem(A.n);
     for (int i = 0; i < A.n; i++) mp[agg[i] + 1]++;
     for (int I = 0; I < nc; I++) mp[I + 1] += mp[I];
@@ -590,7 +706,6 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
 // of the singular Neumann system.  In-place Gauss-Jordan inversion (SPD, no
 // pivoting) held entirely in LDS: n <= 128 -> at most 128 KiB of gfx950's
 // 160 KiB, one workgroup, no global-memory traffic inside the n steps.
-constexpr int DENSE_MAX = 128;
 __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, const int *col, const double *v,
                                                         double *inv)
 {
@@ -598,7 +713,6 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
     __shared__ double colk[DENSE_MAX];
     __shared__ double red[1024 / 64];
     const int tid = threadIdx.x;
-    // alpha = (sum of diagonal / n) / n
     double s = 0.0;
     for (int i = tid; i < n; i += blockDim.x)
         for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) s += v[k];
@@ -632,7 +746,6 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
 
 int amg_setup(Ctx *c, const double *vals)
 {
-    // download the fine matrix
     HostCSR A;
     A.n = c->N;
     A.rp.resize(c->N + 1);
@@ -643,7 +756,7 @@ int amg_setup(Ctx *c, const double *vals)
     MC_HIP(c, hipMemcpyAsync(A.val.data(), vals, (size_t)c->nnz * 8, hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
     c->amg.clear();
-    const int coarse_max = std::max(8, c->opt.amg_coarse_max);
+    const int coarse_max = std::max(8, std::min(c->opt.amg_coarse_max, DENSE_MAX));
     HostCSR cur = std::move(A);
     bool first = true;
     for (;;) {
@@ -664,14 +777,28 @@ int amg_setup(Ctx *c, const double *vals)
         MC_TRY(dalloc(c, &L.dinv32, L.n));
         MC_TRY(dalloc(c, &L.x, L.n)); MC_TRY(dalloc(c, &L.b, L.n)); MC_TRY(dalloc(c, &L.t, L.n));
         MC_TRY(dalloc(c, &L.x32, L.n)); MC_TRY(dalloc(c, &L.b32, L.n)); MC_TRY(dalloc(c, &L.t32, L.n));
+        // diagonal storage for this level's pattern
+        {
+            std::vector<int> offs, map;
+            L.nd = 0;
+            if (cur.n > coarse_max && dia_analyze(cur.n, cur.rp.data(), cur.col.data(), offs, map)) {
+                L.nd = (int)offs.size();
+                for (int d = 0; d < L.nd; d++) L.off[d] = offs[d];
+                MC_TRY(dalloc(c, &L.dia_map, L.nnz));
+                MC_HIP(c, hipMemcpy(L.dia_map, map.data(), (size_t)L.nnz * 4, hipMemcpyHostToDevice));
+                MC_TRY(dalloc(c, &L.dia, (size_t)L.nd * L.n));
+                MC_TRY(dalloc(c, &L.dia32, (size_t)L.nd * L.n));
+                MC_HIP(c, hipMemset(L.dia, 0, (size_t)L.nd * L.n * 8));
+                MC_HIP(c, hipMemset(L.dia32, 0, (size_t)L.nd * L.n * 4));
+            }
+        }
         if (cur.n <= coarse_max) {
             c->amg.push_back(L);
             break;
         }
-        // three pairwise passes -> level aggregation
         std::vector<int> agg(cur.n), a1;
         std::iota(agg.begin(), agg.end(), 0);
-        HostCSR g = cur;  // graph being matched
+        HostCSR g = cur;
         int nc = cur.n;
         for (int pass = 0; pass < 3; pass++) {
             int ncn = 0;
@@ -683,6 +810,7 @@ int amg_setup(Ctx *c, const double *vals)
             nc = ncn;
             if (nc <= coarse_max) break;
         }
+        if (nc > (int)(0.75 * cur.n)) { c->err = "aggregation stalled"; return MC_ERR_STATE; }
         HostCSR Ac;
         std::vector<int> gp, gm;
         galerkin(cur, agg, nc, Ac, &gp, &gm);
@@ -693,6 +821,9 @@ int amg_setup(Ctx *c, const double *vals)
             std::vector<int> mp(nc + 1, 0), mem(L.n);
             for (int i = 0; i < L.n; i++) mp[agg[i] + 1]++;
             for (int I = 0; I < nc; I++) mp[I + 1] += mp[I];
+            int maxm = 0;
+            for (int I = 0; I < nc; I++) maxm = std::max(maxm, mp[I + 1] - mp[I]);
+            if (maxm > 8) { c->err = "aggregate larger than 8"; return MC_ERR_STATE; }
             std::vector<int> pos(mp.begin(), mp.end() - 1);
             for (int i = 0; i < L.n; i++) mem[pos[agg[i]]++] = i;
             MC_TRY(dalloc(c, &L.agg_ptr, nc + 1));
@@ -724,6 +855,9 @@ int amg_update_values(Ctx *c, const double *vals)
         MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
         hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col, L.vals,
                            L.dinv, L.dinv32, L.vals32, L.nnz);
+        if (L.nd)
+            hipLaunchKernelGGL(k_csr_to_dia, dim3(div_up(L.nnz, 256)), dim3(256), 0, c->stream, L.nnz, L.dia_map,
+                               L.vals, L.dia, L.dia32);
         MC_LAUNCH_CHECK(c);
         if (l + 1 < c->amg.size()) {
             AmgLevel &Lc = c->amg[l + 1];
@@ -741,80 +875,94 @@ int amg_update_values(Ctx *c, const double *vals)
     return MC_OK;
 }
 
+template <typename T>
+static Mat<T> level_mat(AmgLevel &L)
+{
+    Mat<T> A;
+    A.n = L.n; A.rp = L.row_ptr; A.col = L.col;
+    if constexpr (sizeof(T) == 8) { A.csr = (const T *)L.vals; A.dia = L.nd ? (const T *)L.dia : nullptr; }
+    else { A.csr = (const T *)L.vals32; A.dia = L.nd ? (const T *)L.dia32 : nullptr; }
+    A.nd = L.nd;
+    for (int d = 0; d < DIA_MAX; d++) A.off[d] = d < L.nd ? L.off[d] : 0;
+    return A;
+}
+
 // ------------------------------------------------------ V-cycle kernels --
 // first two pre-smoothing sweeps from x = 0: x1 = w D^-1 b, x2 = x1 + w D^-1 (b - A x1)
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_jac_first2(int n, const int *rp, const int *col, const T *vals,
-                                                      const T *dinv, const T *b, T *xout, T omega, int chunk, int two)
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_jac_first2(Mat<T> A, const T *dinv, const T *b, T *xout, T omega,
+                                                      int chunk, int two, const KState *stop)
 {
+    if (stop && stop->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
-    const int lane8 = threadIdx.x & 7;
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc = 0.0;
-        if (r < r1 && two) {
-            for (int e = rp[r] + lane8; e < rp[r + 1]; e += 8) {
-                const int j = col[e];
-                acc += (double)vals[e] * (double)(omega * dinv[j] * b[j]);
-            }
-        }
-        acc += __shfl_xor(acc, 1, 64);
-        acc += __shfl_xor(acc, 2, 64);
-        acc += __shfl_xor(acc, 4, 64);
-        if (lane8 == 0 && r < r1) {
-            const T x1 = omega * dinv[r] * b[r];
-            xout[r] = two ? (T)(x1 + omega * dinv[r] * (b[r] - (T)acc)) : x1;
-        }
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    if (!two) {
+        for (int i = r0 + threadIdx.x; i < r1; i += BLOCK) xout[i] = omega * dinv[i] * b[i];
+        return;
+    }
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)(omega * dinv[j] * b[j]); },
+                    [&](int r, const double (&ax)[1]) {
+                        const T x1 = omega * dinv[r] * b[r];
+                        xout[r] = (T)(x1 + omega * dinv[r] * (b[r] - (T)ax[0]));
+                    });
+}
+
+// x_out = y + w D^-1 (b - A y), y_j = x_j + cs * xc[agg_j] (agg == nullptr: y = x).
+// With dotv != nullptr the last sweep also forms (dotv, x_out): the PCG's
+// (r, z), finished into rho/beta by the last block.
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_jac(Mat<T> A, const T *dinv, const T *b, const T *x, const int *agg,
+                                               const T *xc, T cs, T *xout, T omega, int chunk, const T *dotv,
+                                               KState *ks, double *partials, unsigned *cnt, const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    double acc[1] = {0.0};
+    auto yv = [&](int j) -> T { return agg ? (T)(x[j] + cs * xc[agg[j]]) : x[j]; };
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)yv(j); }, [&](int r, const double (&ax)[1]) {
+        const T y = yv(r);
+        const T o = y + omega * dinv[r] * (b[r] - (T)ax[0]);
+        xout[r] = o;
+        if (dotv) acc[0] += (double)dotv[r] * (double)o;
+    });
+    if (dotv && block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
+        ks->rho[0] = acc[0];
+        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
+        ks->rho_old[0] = acc[0];
     }
 }
 
-// x_out = y + w D^-1 (b - A y), y_j = x_j + cs * xc[agg_j] (agg == nullptr: y = x)
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_jac(int n, const int *rp, const int *col, const T *vals, const T *dinv,
-                                               const T *b, const T *x, const int *agg, const T *xc, T cs, T *xout,
-                                               T omega, int chunk)
+// bc[I] = sum_{i in aggregate I} (b_i - (A x)_i): residual fused with
+// restriction.  8 lanes per coarse row; DIA: lane m evaluates member m's row;
+// CSR: the 8 lanes stride over each member's entries.
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_resid_restrict(Mat<T> A, int nc, const int *aptr, const int *amem,
+                                                          const T *b, const T *x, T *bc, const KState *stop)
 {
-    const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
-    const int lane8 = threadIdx.x & 7;
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc = 0.0;
-        if (r < r1) {
-            for (int e = rp[r] + lane8; e < rp[r + 1]; e += 8) {
-                const int j = col[e];
-                T y = x[j];
-                if (agg) y += cs * xc[agg[j]];
-                acc += (double)vals[e] * (double)y;
-            }
-        }
-        acc += __shfl_xor(acc, 1, 64);
-        acc += __shfl_xor(acc, 2, 64);
-        acc += __shfl_xor(acc, 4, 64);
-        if (lane8 == 0 && r < r1) {
-            T y = x[r];
-            if (agg) y += cs * xc[agg[r]];
-            xout[r] = y + omega * dinv[r] * (b[r] - (T)acc);
-        }
-    }
-}
-
-// bc[I] = sum_{i in aggregate I} (b_i - (A x)_i): residual fused with restriction
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_resid_restrict(int nc, const int *aptr, const int *amem, const int *rp,
-                                                          const int *col, const T *vals, const T *b, const T *x, T *bc)
-{
+    if (stop && stop->all_done) return;
     const int I = blockIdx.x * (BLOCK / 8) + (threadIdx.x >> 3);
     const int lane8 = threadIdx.x & 7;
     double s = 0.0;
     if (I < nc) {
-        for (int t = aptr[I]; t < aptr[I + 1]; t++) {
-            const int i = amem[t];
-            double acc = 0.0;
-            for (int e = rp[i] + lane8; e < rp[i + 1]; e += 8) acc += (double)vals[e] * (double)x[col[e]];
-            if (lane8 == 0) s += (double)b[i];
-            s -= acc;
+        const int t0 = aptr[I], t1 = aptr[I + 1];
+        if constexpr (ND > 0) {
+            if (t0 + lane8 < t1) {
+                const int i = amem[t0 + lane8];
+                double ax = 0.0;
+#pragma unroll
+                for (int d = 0; d < ND; d++)
+                    ax += (double)A.dia[(size_t)d * A.n + i] * (double)x[clampi(i + A.off[d], A.n)];
+                s = (double)b[i] - ax;
+            }
+        } else {
+            for (int t = t0; t < t1; t++) {
+                const int i = amem[t];
+                double acc = 0.0;
+                for (int e = A.rp[i] + lane8; e < A.rp[i + 1]; e += 8) acc += (double)A.csr[e] * (double)x[A.col[e]];
+                if (lane8 == 0) s += (double)b[i];
+                s -= acc;
+            }
         }
     }
     s += __shfl_xor(s, 1, 64);
@@ -825,9 +973,10 @@ __global__ void __launch_bounds__(BLOCK) k_resid_restrict(int nc, const int *apt
 
 // coarsest level: x = Ainv b in FP64
 template <typename T>
-__global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, const T *b, T *x)
+__global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, const T *b, T *x, const KState *stop)
 {
-    __shared__ double bs[1024];
+    if (stop && stop->all_done) return;
+    __shared__ double bs[DENSE_MAX];
     for (int i = threadIdx.x; i < n; i += blockDim.x) bs[i] = (double)b[i];
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -837,62 +986,83 @@ __global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, c
     x[i] = (T)s;
 }
 
+// rho = (r, z), beta = rho / rho_old (PCG), when no sweep can fuse it
 template <typename T>
-struct LevelPtrs {
-    static T *vals(AmgLevel &L);
-    static T *dinv(AmgLevel &L);
-    static T *x(AmgLevel &L);
-    static T *b(AmgLevel &L);
-    static T *t(AmgLevel &L);
-};
-template <> double *LevelPtrs<double>::vals(AmgLevel &L) { return L.vals; }
-template <> double *LevelPtrs<double>::dinv(AmgLevel &L) { return L.dinv; }
-template <> double *LevelPtrs<double>::x(AmgLevel &L) { return L.x; }
-template <> double *LevelPtrs<double>::b(AmgLevel &L) { return L.b; }
-template <> double *LevelPtrs<double>::t(AmgLevel &L) { return L.t; }
-template <> float *LevelPtrs<float>::vals(AmgLevel &L) { return L.vals32; }
-template <> float *LevelPtrs<float>::dinv(AmgLevel &L) { return L.dinv32; }
-template <> float *LevelPtrs<float>::x(AmgLevel &L) { return L.x32; }
-template <> float *LevelPtrs<float>::b(AmgLevel &L) { return L.b32; }
-template <> float *LevelPtrs<float>::t(AmgLevel &L) { return L.t32; }
+__global__ void __launch_bounds__(BLOCK) k_rz(int n, const T *r, const T *z, KState *ks, double *partials,
+                                              unsigned *cnt, int chunk)
+{
+    if (ks->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[1] = {0.0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += (double)r[i] * (double)z[i];
+    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
+        ks->rho[0] = acc[0];
+        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
+        ks->rho_old[0] = acc[0];
+    }
+}
 
-// V-cycle on level l: x_l = M^-1 b_l (b given in 'b', result written to 'x').
+template <typename T> static T *lv_x(AmgLevel &L);
+template <typename T> static T *lv_b(AmgLevel &L);
+template <typename T> static T *lv_t(AmgLevel &L);
+template <typename T> static T *lv_dinv(AmgLevel &L);
+template <> double *lv_x<double>(AmgLevel &L) { return L.x; }
+template <> double *lv_b<double>(AmgLevel &L) { return L.b; }
+template <> double *lv_t<double>(AmgLevel &L) { return L.t; }
+template <> double *lv_dinv<double>(AmgLevel &L) { return L.dinv; }
+template <> float *lv_x<float>(AmgLevel &L) { return L.x32; }
+template <> float *lv_b<float>(AmgLevel &L) { return L.b32; }
+template <> float *lv_t<float>(AmgLevel &L) { return L.t32; }
+template <> float *lv_dinv<float>(AmgLevel &L) { return L.dinv32; }
+
+// V-cycle on level l: x_l = M^-1 b_l.  On level 0 with ks != nullptr the
+// last sweep also reduces (b, x) into the PCG's rho / beta.
 template <typename T>
-static int vcycle(Ctx *c, size_t l, const T *b, T *x)
+static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *stop)
 {
     AmgLevel &L = c->amg[l];
     const T omega = (T)c->opt.amg_omega;
     const int nu = std::max(1, c->opt.amg_nu);
     if (l + 1 == c->amg.size()) {
-        hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x);
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x, stop);
+        if (ks) {  // one-level hierarchy: no smoothing sweep carries the (r, z) reduction
+            const Grid g = grid_for(L.n);
+            hipLaunchKernelGGL(k_rz<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, b, x, ks, c->red, c->red_cnt,
+                               g.chunk);
+        }
         return MC_OK;
     }
-    const Grid g = grid_for(L.n, ROWS_PER_BLOCK);
-    T *vals = LevelPtrs<T>::vals(L), *dinv = LevelPtrs<T>::dinv(L), *tmp = LevelPtrs<T>::t(L);
-    // pre-smoothing: x <- S^nu 0
+    const Grid g = grid_for(L.n);
+    const Mat<T> A = level_mat<T>(L);
+    T *dinv = lv_dinv<T>(L), *tmp = lv_t<T>(L);
     T *cur = x, *oth = tmp;
-    hipLaunchKernelGGL(k_jac_first2<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv,
-                       b, cur, omega, g.chunk, nu >= 2 ? 1 : 0);
+    MC_DISPATCH_ND(A.nd, (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, omega,
+                   g.chunk, nu >= 2 ? 1 : 0, stop);
     for (int s = 2; s < nu; s++) {
-        hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
-                           cur, (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk);
+        MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+                       (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk, (const T *)nullptr,
+                       (KState *)nullptr, (double *)nullptr, (unsigned *)nullptr, stop);
         std::swap(cur, oth);
     }
-    // residual + restriction
     AmgLevel &Lc = c->amg[l + 1];
-    T *bc = LevelPtrs<T>::b(Lc), *xc = LevelPtrs<T>::x(Lc);
-    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, L.nc, L.agg_ptr,
-                       L.agg_members, L.row_ptr, L.col, vals, b, cur, bc);
-    MC_TRY(vcycle<T>(c, l + 1, bc, xc));
-    // prolongation fused with the first post-smoothing sweep
+    T *bc = lv_b<T>(Lc), *xc = lv_x<T>(Lc);
+    MC_DISPATCH_ND(A.nd, (k_resid_restrict<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, A, L.nc,
+                   L.agg_ptr, L.agg_members, b, cur, bc, stop);
+    MC_TRY(vcycle<T>(c, l + 1, bc, xc, nullptr, stop));
     const T cs = (T)c->opt.amg_cscale;
-    hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
-                       cur, L.agg, xc, cs, oth, omega, g.chunk);
-    std::swap(cur, oth);
-    for (int s = 1; s < nu; s++) {
-        hipLaunchKernelGGL(k_jac<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, L.row_ptr, L.col, vals, dinv, b,
-                           cur, (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk);
-        std::swap(cur, oth);
+    // post-smoothing: the first sweep applies the prolongation on the fly; the
+    // last sweep writes x (and the fused dot on level 0)
+    for (int s = 0; s < nu; s++) {
+        const bool first = s == 0, last = s == nu - 1;
+        T *dst = last ? x : oth;
+        if (dst == cur) dst = tmp;
+        const T *dotv = (last && ks) ? b : nullptr;
+        MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+                       first ? L.agg : (const int *)nullptr, first ? xc : (const T *)nullptr, first ? cs : (T)0, dst,
+                       omega, g.chunk, dotv, ks, c->red, c->red_cnt, stop);
+        oth = cur;
+        cur = dst;
     }
     if (cur != x) MC_HIP(c, hipMemcpyAsync(x, cur, (size_t)L.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     return MC_OK;
@@ -901,7 +1071,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x)
 // ------------------------------------------------------------ PCG --
 template <typename T>
 struct PcgVecs {
-    T *x, *r, *z, *p, *q;
+    T *x, *r, *z, *p, *q, *p2;
 };
 
 template <typename T>
@@ -929,57 +1099,31 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const dou
     }
 }
 
-// rho = (r, z); p = z + beta p
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_rz(int n, PcgVecs<T> w, KState *ks, double *partials, unsigned *cnt,
-                                                  int chunk)
+// p_new = z + beta p_old ; q = A p_new ; alpha = rho / (p_new, q).  The
+// product reads z + beta p_old at the neighbours, so the p update needs no
+// pass of its own (p_old and p_new are separate buffers).
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_pcg_pq(Mat<T> A, const T *z, const T *pold, T *pnew, T *q, KState *ks,
+                                                  double *partials, unsigned *cnt, int chunk)
 {
     if (ks->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
-    double acc[1] = {0.0};
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += (double)w.r[i] * (double)w.z[i];
-    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
-        ks->rho[0] = acc[0];
-        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
-        ks->rho_old[0] = acc[0];
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_p(int n, PcgVecs<T> w, const KState *ks, int chunk)
-{
-    if (ks->all_done) return;
-    const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
     const T beta = (T)ks->beta[0];
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) w.p[i] = w.z[i] + beta * w.p[i];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_q(int n, const int *rp, const int *col, const T *vals, PcgVecs<T> w,
-                                                 KState *ks, double *partials, unsigned *cnt, int chunk)
-{
-    if (ks->all_done) return;
-    const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
     double dsum[1] = {0.0};
-    const T *xs[1] = {w.p};
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int r = base + (threadIdx.x >> 3);
-        double acc[1];
-        csr_row8<1, T, T>(r < r1 ? r : n, n, rp, col, vals, xs, acc);
-        if ((threadIdx.x & 7) == 0 && r < r1) {
-            w.q[r] = (T)acc[0];
-            dsum[0] += (double)w.p[r] * acc[0];
-        }
-    }
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)(z[j] + beta * pold[j]); },
+                    [&](int r, const double (&ax)[1]) {
+                        const T p = z[r] + beta * pold[r];
+                        pnew[r] = p;
+                        q[r] = (T)ax[0];
+                        dsum[0] += (double)p * ax[0];
+                    });
     if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) ks->alpha[0] = ks->rho[0] / dsum[0];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, PcgVecs<T> w, KState *ks, double *partials, unsigned *cnt,
-                                                  int chunk)
+__global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, T *x, T *r, const T *p, const T *q, KState *ks,
+                                                  double *partials, unsigned *cnt, int chunk)
 {
     if (ks->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
@@ -987,10 +1131,10 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, PcgVecs<T> w, KState *k
     const T al = (T)ks->alpha[0];
     double acc[1] = {0.0};
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-        w.x[i] += al * w.p[i];
-        const T r = w.r[i] - al * w.q[i];
-        w.r[i] = r;
-        acc[0] += (double)r * (double)r;
+        x[i] += al * p[i];
+        const T rr = r[i] - al * q[i];
+        r[i] = rr;
+        acc[0] += (double)rr * (double)rr;
     }
     if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
         const double rn = sqrt(acc[0]);
@@ -1003,48 +1147,51 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, PcgVecs<T> w, KState *k
 }
 
 template <typename T>
-static int pcg_core(Ctx *c, const T *vals, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit,
-                    KState *ks, KState *hs, int *iters_out)
+static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit, KState *ks,
+                    KState *hs, int *iters_out)
 {
     const int n = c->N;
-    const Grid gv = grid_for(n, BLOCK);
-    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
+    const Grid g = grid_for(n);
+    const Mat<T> A = level_mat<T>(c->amg[0]);
     const double *mean = nullptr;
     if (project) {
-        // mean of b (as double) for the projection onto the range of A
         double *dm = c->dscal + 600;
-        if (sizeof(T) == 8) {
-            hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)b, dm, c->red,
-                               c->red_cnt + 10, gv.chunk, 1.0 / (double)n);
+        if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)b, dm, c->red,
+                               c->red_cnt + 10, g.chunk, 1.0 / (double)n);
             mean = dm;
         }
     }
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, ks, c->red,
-                       c->red_cnt, gv.chunk, rtol, atol, maxit);
+    hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, ks, c->red, c->red_cnt,
+                       g.chunk, rtol, atol, maxit);
     MC_LAUNCH_CHECK(c);
-    const int check = std::max(1, c->opt.check_every);
-    int it = 0;
+    const double vbytes = A.nd ? (double)sizeof(T) * c->nnz + 3.0 * sizeof(T) * n
+                               : (4.0 + sizeof(T)) * c->nnz + (4.0 + 3 * sizeof(T)) * n;
+    T *pa = w.p, *pb = w.p2;
+    int it = 0, batch = 2;
+    double rn_prev = -1.0;
     for (;;) {
-        for (int j = 0; j < check && it < maxit; j++, it++) {
+        for (int j = 0; j < batch && it < maxit; j++, it++) {
             {
                 KTimer kt(c, MC_KC_VCYCLE, 0.0);
-                MC_TRY(vcycle<T>(c, 0, w.r, w.z));
+                MC_TRY(vcycle<T>(c, 0, w.r, w.z, ks, ks));
             }
-            hipLaunchKernelGGL(k_pcg_rz<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                               gv.chunk);
-            hipLaunchKernelGGL(k_pcg_p<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, gv.chunk);
             {
-                KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32,
-                          (4.0 + sizeof(T)) * c->nnz + (4.0 + 2 * sizeof(T)) * n);
-                hipLaunchKernelGGL(k_pcg_q<T>, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col, vals,
-                                   w, ks, c->red, c->red_cnt, gm.chunk);
+                KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32, vbytes);
+                MC_DISPATCH_ND(A.nd, (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb,
+                               w.q, ks, c->red, c->red_cnt, g.chunk);
             }
-            hipLaunchKernelGGL(k_pcg_xr<T>, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                               gv.chunk);
+            hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w.x, w.r, pb, w.q, ks,
+                               c->red, c->red_cnt, g.chunk);
+            std::swap(pa, pb);
         }
         MC_LAUNCH_CHECK(c);
         MC_TRY(poll_done(c, ks, hs));
+        c->st.host_polls++;
         if (hs->all_done || it >= maxit) break;
+        const double rn = hs->rnorm[0] / std::max(hs->tol[0], 1e-300);
+        batch = predict_batch(rn_prev, rn, batch, 1.0, 8);
+        rn_prev = rn;
     }
     *iters_out = hs->iters[0];
     return MC_OK;
@@ -1059,10 +1206,10 @@ __global__ void k_sub_mean(int n, double *x, const double *mean)
 static int project_mean_zero(Ctx *c, double *x)
 {
     const int n = c->N;
-    const Grid gv = grid_for(n, BLOCK);
+    const Grid g = grid_for(n);
     double *dm = c->dscal + 610;
-    hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)x, dm, c->red,
-                       c->red_cnt + 11, gv.chunk, 1.0 / (double)n);
+    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)x, dm, c->red,
+                       c->red_cnt + 11, g.chunk, 1.0 / (double)n);
     hipLaunchKernelGGL(k_sub_mean, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, dm);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -1071,11 +1218,12 @@ static int project_mean_zero(Ctx *c, double *x)
 int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
             mc_solve_status *st)
 {
+    (void)vals;  // the hierarchy's level 0 holds these values
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
-    PcgVecs<double> w{x, c->wk[0], c->wk[1], c->wk[2], c->wk[3]};
+    PcgVecs<double> w{x, c->wk[0], c->wk[1], c->wk[2], c->wk[3], c->wk[5]};
     int iters = 0;
-    MC_TRY(pcg_core<double>(c, vals, b, true, w, rtol, atol, maxit, ks, hs, &iters));
+    MC_TRY(pcg_core<double>(c, b, true, w, rtol, atol, maxit, ks, hs, &iters));
     MC_TRY(project_mean_zero(c, x));
     if (st) {
         st->iterations = iters;
@@ -1088,25 +1236,19 @@ int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol,
 }
 
 // ---------------------------------------------- mixed: FP32 inside FP64 IR --
-__global__ void __launch_bounds__(BLOCK) k_ir_resid(int n, const int *rp, const int *col, const double *vals,
-                                                    const double *b, const double *mean, const double *x, double *r,
-                                                    double *partials, unsigned *cnt, int chunk, double *out)
+template <int ND>
+__global__ void __launch_bounds__(BLOCK) k_ir_resid(Mat<double> A, const double *b, const double *mean, const double *x,
+                                                    double *r, double *partials, unsigned *cnt, int chunk, double *out)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
     double dsum[1] = {0.0};
-    const double *xs[1] = {x};
     const double mu = *mean;
-    for (int base = r0; base < r1; base += ROWS_PER_BLOCK) {
-        const int row = base + (threadIdx.x >> 3);
-        double acc[1];
-        csr_row8<1, double, double>(row < r1 ? row : n, n, rp, col, vals, xs, acc);
-        if ((threadIdx.x & 7) == 0 && row < r1) {
-            const double rr = (b[row] - mu) - acc[0];
-            r[row] = rr;
-            dsum[0] += rr * rr;
-        }
-    }
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return x[j]; }, [&](int row, const double (&ax)[1]) {
+        const double rr = (b[row] - mu) - ax[0];
+        r[row] = rr;
+        dsum[0] += rr * rr;
+    });
     if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) *out = sqrt(dsum[0]);
 }
 
@@ -1133,40 +1275,43 @@ __global__ void k_zero(int n, double *x)
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
                    mc_solve_status *st)
 {
+    (void)vals;
     const int n = c->N;
-    const Grid gv = grid_for(n, BLOCK);
-    const Grid gm = grid_for(n, ROWS_PER_BLOCK);
+    const Grid g = grid_for(n);
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     double *mean = c->dscal + 620, *rn = c->dscal + 621;
     double *r = c->wk[4];
     float *r32 = c->wk32[0], *e32 = c->wk32[1];
-    PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5]};
-    hipLaunchKernelGGL(k_sum, dim3(gv.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 12,
-                       gv.chunk, 1.0 / (double)n);
+    PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5], c->wk32[6]};
+    const Mat<double> A = level_mat<double>(c->amg[0]);
+    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 12, g.chunk,
+                       1.0 / (double)n);
     hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
-    // FP32 copy of the fine matrix values (the AMG levels already hold theirs)
-    const float *vals32 = c->amg[0].vals32;
     double bnorm = -1.0, rnorm = 0.0;
     int total = 0, outer = 0;
     bool conv = false;
     const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
+    const double rbytes = A.nd ? 8.0 * c->nnz + 32.0 * n : 12.0 * c->nnz + 36.0 * n;
     for (outer = 0; outer < 50; outer++) {
         {
-            KTimer kt(c, MC_KC_SPMV64, 12.0 * c->nnz + 28.0 * n);
-            hipLaunchKernelGGL(k_ir_resid, dim3(gm.blocks), dim3(BLOCK), 0, c->stream, n, c->row_ptr, c->col, vals, b,
-                               mean, x, r, c->red, c->red_cnt + 13, gm.chunk, rn);
+            KTimer kt(c, MC_KC_SPMV64, rbytes);
+            MC_DISPATCH_ND(A.nd, (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
+                           c->red_cnt + 13, g.chunk, rn);
         }
         MC_LAUNCH_CHECK(c);
         MC_HIP(c, hipMemcpyAsync(c->hscal + 621, rn, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         MC_HIP(c, hipStreamSynchronize(c->stream));
+        c->st.host_polls++;
         rnorm = c->hscal[621];
         if (bnorm < 0) bnorm = rnorm;
         if (rnorm <= std::max(rtol * bnorm, atol) || !(rnorm == rnorm)) { conv = rnorm == rnorm; break; }
         if (total >= maxit) break;
         hipLaunchKernelGGL(k_to32_scaled, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, r, rn, r32);
+        // the last refinement step needs only the remaining reduction
+        const double need = std::max(rtol * bnorm, atol) / rnorm;
         int it = 0;
-        MC_TRY(pcg_core<float>(c, vals32, r32, false, w, inner, 0.0, maxit - total, ks, hs, &it));
+        MC_TRY(pcg_core<float>(c, r32, false, w, std::max(inner, 0.5 * need), 0.0, maxit - total, ks, hs, &it));
         total += it;
         hipLaunchKernelGGL(k_ir_update, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, e32, rn);
         MC_LAUNCH_CHECK(c);

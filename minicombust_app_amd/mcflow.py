@@ -53,7 +53,7 @@ class Stats(C.Structure):
         ("p_iters", C.c_int64), ("p_solves", C.c_int64), ("p_loops", C.c_int64), ("p_outer", C.c_int64),
         ("last_p_relres", C.c_double), ("p_solve_ms", C.c_double), ("last_p_max", C.c_double),
         ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
-        ("clip_passes", C.c_int64),
+        ("clip_passes", C.c_int64), ("host_polls", C.c_int64),
     ]
 
 
@@ -91,7 +91,8 @@ def lib():
         L.mc_ctx_sizes.argtypes = [P_] + [C.POINTER(C.c_int64)] * 4
         L.mc_set_options.argtypes = [P_, C.POINTER(Options)]
         for n in ["mc_timestep", "mc_get_phi_gradients", "mc_set_up_field", "mc_momentum_assemble",
-                  "mc_pressure_assemble", "mc_calculate_pressure", "mc_update_P", "mc_sync", "mc_profile_reset"]:
+                  "mc_pressure_assemble", "mc_calculate_pressure", "mc_update_P", "mc_sync", "mc_profile_reset",
+                  "mc_scalar_batch_solve"]:
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
         L.mc_get_phi_gradient.argtypes = [P_, C.c_int, C.c_int]
@@ -217,6 +218,9 @@ class Context:
 
     def scalar_solve(self, t):
         self._check(self.L.mc_scalar_solve(self.h, t), "mc_scalar_solve")
+
+    def scalar_batch_solve(self):
+        self._check(self.L.mc_scalar_batch_solve(self.h), "mc_scalar_batch_solve")
 
     def update_boundary_slots(self, field):
         self._check(self.L.mc_update_boundary_slots(self.h, field), "mc_update_boundary_slots")

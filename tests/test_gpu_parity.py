@@ -173,3 +173,64 @@ def test_timesteps(m, mode):
         assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "ts %d mass flux" % ts, 1e-8)
     o.close()
     g.close()
+
+
+@pytest.mark.parametrize("psolve", [mc.PSOLVE_FP64, mc.PSOLVE_MIXED])
+@pytest.mark.parametrize("m", [16, 32])
+def test_pressure_solve_forms(m, psolve):
+    """Both pressure forms reach 1e-8 on the assembled pressure-correction system:
+    residual checked with an independent SpMV, solution against the oracle's
+    Jacobi-PCG solve at 1e-12 (mean-zero representatives)."""
+    print("pressure solve m=%d psolve=%d" % (m, psolve), flush=True)
+    o = po.Oracle(m, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=200000)
+    o.get_phi_gradients()
+    o.set_up_field()
+    o.calculate_UVW()
+    oracle_pressure_assemble(o)
+    b = o.cell("S_U").copy()
+    o.solve_pressure_matrix()
+    x_ref = o.phi(po.PP)[:o.N].copy()
+    g = mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8)
+    push_state(o, g)
+    x, st = g.solve_pressure(b)
+    assert st.converged
+    bp = b - b.mean()
+    res = np.linalg.norm(bp - g.spmv(x, which=1)) / np.linalg.norm(bp)
+    assert res <= 1.05e-8, res
+    assert abs(x.mean()) <= 1e-12 * max(1.0, np.abs(x).max())
+    assert_close(x, x_ref, "pressure correction vs oracle", 1e-6)
+    o.close()
+    g.close()
+
+
+@pytest.mark.parametrize("mode", [po.MODE_COMPAT, po.MODE_SIMPLE])
+def test_timesteps_m16(mode):
+    test_timesteps(16, mode)
+
+
+def test_scalar_batch_equals_single():
+    """The 4-RHS passive-scalar solve reproduces four single solves bit for bit."""
+    m = 8
+    o = po.Oracle(m, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    o.get_phi_gradients()
+    o.set_up_field()
+    o.calculate_UVW()
+    o.calculate_pressure()
+    o.scalar_solve(po.TERBTE, po.TE)
+    o.scalar_solve(po.TERBED, po.ED)
+    o.scalar_solve(po.TEMP, po.TEM)
+    g1 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    g2 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    push_state(o, g1)
+    push_state(o, g2)
+    for t in [po.FUEL, po.PROG, po.VARFU, po.VARPR]:
+        g1.scalar_solve(t)
+    g2.scalar_batch_solve()
+    for f in [po.FUL, po.PRO, po.VARF, po.VARP]:
+        assert_bits(g2.phi(f), g1.phi(f), "batched scalar %s" % po.FIELD_NAMES[f])
+    for t, f in [(po.FUEL, po.FUL), (po.PROG, po.PRO), (po.VARFU, po.VARF), (po.VARPR, po.VARP)]:
+        o.scalar_solve(t, f)
+        assert_close(g2.phi(f), o.phi(f), "batched scalar vs oracle %s" % po.FIELD_NAMES[f], 1e-9)
+    g1.close()
+    g2.close()
+    o.close()
