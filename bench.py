@@ -10,9 +10,13 @@ input already resident in HBM.  Each pressure-solve form is timed on its own
 context: FP64 PCG-AMG, and FP32 PCG-AMG inside FP64 iterative refinement.
 `value` is the iterations/s of the fastest form meeting the 1e-8 residual.
 
-A second, profiled pass brackets every kernel class with HIP events on the
-context's stream and divides its algorithmic bytes by its time: `roofline` is
-the dominant class against the 8 TB/s HBM3E peak.  `cpu_baseline` is the CPU
+A profiled pass (before the timed one) brackets every kernel class with HIP
+events on the context's stream (no synchronisation) and divides its
+algorithmic bytes by its time.  The dominant single-kernel class is then timed
+again with HIP events inside the timed region itself: `roofline` is that
+kernel's algorithmic bytes per launch over its average launch duration,
+against the 8 TB/s HBM3E peak; `traffic` is its HBM bytes per launch from the
+FETCH_SIZE/WRITE_SIZE counter summary in profiles/ (null when absent).  `cpu_baseline` is the CPU
 oracle (one core) timed on a bounded sample of the same workload, run in a
 child process while the GPU is measured.
 """
@@ -30,14 +34,38 @@ METRIC = "SIMPLE iter/s at 4.2M cells: learned+mixed vs FP64 GPU at equal residu
 HBM_PEAK_GBS = 8000.0
 
 
-def run_form(mc, m, form, steps, warmup, profile_steps):
+def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
     psolve = mc.PSOLVE_MIXED if form.startswith("mixed") else mc.PSOLVE_FP64
     ctx = mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5)
     it = 0
     for _ in range(warmup):
         ctx.simple_iteration(it % 3)
         it += 1
+    # profiled pass: every kernel class bracketed by HIP events (no syncs)
+    kern = {}
+    if profile_steps > 0:
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        for _ in range(profile_steps):
+            ctx.simple_iteration(it % 3)
+            it += 1
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        for name, v in prof.items():
+            if v["calls"] == 0:
+                continue
+            ms_avg = v["ms"] / v["calls"]
+            gbs = (v["bytes"] / v["calls"]) / (ms_avg * 1e-3) / 1e9 if v["bytes"] > 0 and ms_avg > 0 else None
+            kern[name] = {"ms_per_step": v["ms"] / profile_steps, "calls_per_step": v["calls"] / profile_steps,
+                          "avg_launch_ms": ms_avg, "bytes_per_launch": v["bytes"] / v["calls"],
+                          "achieved_GBs": gbs}
+    if roof_class is None and kern:
+        cands = [(v["ms_per_step"], k) for k, v in kern.items() if v["achieved_GBs"] and k in mc.KC_KERNELS]
+        roof_class = max(cands)[1] if cands else None
+    # timed pass: only the roofline kernel's launches carry events
     ctx.sync()
+    ctx.profile(roof_class is not None, classes=[roof_class] if roof_class else None)
     ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -46,6 +74,14 @@ def run_form(mc, m, form, steps, warmup, profile_steps):
     ctx.sync()
     t1 = time.perf_counter()
     st = ctx.stats()
+    live = None
+    if roof_class is not None:
+        v = ctx.profile_read()[roof_class]
+        if v["calls"]:
+            live = {"kernel_class": roof_class, "kernel": mc.KC_KERNELS.get(roof_class),
+                    "avg_launch_ms": v["ms"] / v["calls"], "launches": v["calls"],
+                    "bytes_per_launch": v["bytes"] / v["calls"]}
+    ctx.profile(False)
     out = {
         "iter_per_s": steps / (t1 - t0),
         "ms_per_step": 1000.0 * (t1 - t0) / steps,
@@ -56,39 +92,49 @@ def run_form(mc, m, form, steps, warmup, profile_steps):
         "last_rel_residual": st["p_relres_max"],
         "momentum_iters_per_step": st["mom_iters"] / steps,
         "scalar_iters_per_step": st["scalar_iters"] / steps,
+        "kernels": kern,
+        "roof_live": live,
     }
-    # profiled pass: per kernel class HIP-event time and algorithmic bytes
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(profile_steps):
-        ctx.simple_iteration(it % 3)
-        it += 1
-    ctx.sync()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    kern = {}
-    for name, v in prof.items():
-        if v["calls"] == 0:
-            continue
-        ms_avg = v["ms"] / v["calls"]
-        gbs = (v["bytes"] / v["calls"]) / (ms_avg * 1e-3) / 1e9 if v["bytes"] > 0 and ms_avg > 0 else None
-        kern[name] = {"ms_per_step": v["ms"] / profile_steps, "calls_per_step": v["calls"] / profile_steps,
-                      "avg_launch_ms": ms_avg, "bytes_per_launch": v["bytes"] / v["calls"],
-                      "achieved_GBs": gbs}
-    out["kernels"] = kern
     ctx.close()
     return out
 
 
-def roofline_of(kern):
-    cands = [(v["ms_per_step"], k) for k, v in kern.items() if v["achieved_GBs"]]
-    if not cands:
+def traffic_of(kernel, m):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (rocprofv3 --pmc runs, scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "traffic_m%d.txt" % m)
+    if not kernel or not os.path.exists(path):
         return None
-    _, k = max(cands)
-    v = kern[k]
-    return {"kernel": k, "bound": "hbm", "achieved": round(v["achieved_GBs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(v["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": v["bytes_per_launch"], "avg_launch_ms": v["avg_launch_ms"]}
+    for ln in open(path):
+        f = ln.split()
+        if len(f) >= 2 and not ln.startswith("#") and f[0] == kernel.replace(" ", ""):
+            try:
+                return float(f[-1])
+            except ValueError:
+                return None
+    return None
+
+
+def roofline_of(live, m):
+    if not live:
+        return None
+    gbs = live["bytes_per_launch"] / (live["avg_launch_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_of(live["kernel"], m),
+            "kernel": live["kernel"], "kernel_class": live["kernel_class"],
+            "bytes_per_launch": live["bytes_per_launch"], "avg_launch_ms": round(live["avg_launch_ms"], 5),
+            "launches_timed": live["launches"]}
+
+
+def copy_probe(mc, mb, reps):
+    """A plain device copy of known size: the on-box HBM bandwidth probe and
+    the known-size kernel (mc::k_copy_probe) that scales the traffic counters."""
+    ctx = mc.Context(4, mode=mc.MODE_SIMPLE)
+    nbytes = mb << 20
+    ms = ctx.copy_probe(nbytes, reps)
+    ctx.close()
+    return {"kernel": "mc::k_copy_probe", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+            "avg_launch_ms": round(ms, 5), "GBs": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
 def main():
@@ -101,6 +147,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-m", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-class", default=None, help="kernel class for the roofline (default: dominant)")
+    ap.add_argument("--copy-probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -114,10 +162,12 @@ def main():
     from minicombust_app_amd import mcflow as mc
     forms = {}
     for form in args.forms.split(","):
-        forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps)
+        forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps, args.roofline_class)
         print("# %s: %.2f SIMPLE iter/s, pressure %.2f ms/iter (%.1f PCG iters), rel.res %.2e" % (
             form, forms[form]["iter_per_s"], forms[form]["pressure_ms_per_step"],
             forms[form]["pressure_iters_per_step"], forms[form]["last_rel_residual"]), file=sys.stderr, flush=True)
+
+    probe = copy_probe(mc, args.copy_probe_mb, 10) if args.copy_probe_mb > 0 else None
 
     ok = {k: v for k, v in forms.items() if v["last_rel_residual"] <= 1e-8}
     head = max(ok or forms, key=lambda k: forms[k]["iter_per_s"])
@@ -155,8 +205,9 @@ def main():
         "speedup_vs_fp64": (round(h["iter_per_s"] / forms["fp64"]["iter_per_s"], 3) if "fp64" in forms else None),
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in h["kernels"].items()},
-        "roofline": roofline_of(h["kernels"]),
+        "roofline": roofline_of(h["roof_live"], args.m),
         "cpu_baseline": cpu,
+        "hbm_copy_probe": probe,
     }
     print(json.dumps(line))
 

@@ -129,18 +129,26 @@ enum {
     MC_ARR_CELLS, MC_ARR_POINTS, MC_ARR_CELLS_PER_POINT              /* u64/f64/u8 */
 };
 
-/* Kernel classes timed with HIP events when profiling is on. */
+/* Kernel classes timed with HIP events when profiling is on.  Every class but
+   MC_KC_VCYCLE is one kernel (fine level for the solver kernels), so its
+   event time is that kernel's launch duration. */
 enum {
-    MC_KC_GRAD = 0,      /* 11-field LSQ gradient                      */
-    MC_KC_GRAD1,         /* 1-field LSQ gradient                       */
-    MC_KC_MOM_ASM,       /* momentum fluxes + sources + CSR assembly   */
-    MC_KC_P_ASM,         /* Rhie-Chow mass flux + pressure assembly    */
-    MC_KC_SCALAR_ASM,    /* scalar fluxes + sources + CSR assembly     */
-    MC_KC_SPMV64,        /* FP64 CSR SpMV (+fused dots)                */
-    MC_KC_SPMV32,        /* FP32 CSR SpMV                              */
-    MC_KC_VCYCLE,        /* AMG V-cycle (all levels)                   */
-    MC_KC_CORRECT,       /* flux / cell correction                     */
-    MC_KC_MPNN,          /* message-passing net inference              */
+    MC_KC_GRAD = 0,      /* k_grad<11>: 11-field LSQ gradient                    */
+    MC_KC_GRAD1,         /* k_grad<1>: 1-field LSQ gradient                      */
+    MC_KC_MOM_ASM,       /* k_momentum: fluxes + sources + matrix assembly       */
+    MC_KC_P_ASM,         /* k_mass_pressure: Rhie-Chow flux + pressure assembly  */
+    MC_KC_SCALAR_ASM,    /* k_scalar: one scalar's fluxes + sources + assembly   */
+    MC_KC_SPMV64,        /* k_pcg_pq<.,double>: fine PCG p-update + SpMV + dot   */
+    MC_KC_SPMV32,        /* k_pcg_pq<.,float>: same, FP32 inner PCG              */
+    MC_KC_VCYCLE,        /* AMG V-cycle, all levels (several kernels)            */
+    MC_KC_CORRECT,       /* k_flux_correct: mass-flux correction                 */
+    MC_KC_MPNN,          /* message-passing net forward layer                    */
+    MC_KC_JAC64,         /* k_jac<.,double>: fine-level Jacobi sweep             */
+    MC_KC_JAC32,         /* k_jac<.,float>: fine-level Jacobi sweep              */
+    MC_KC_BICG,          /* k_bi_spmv_v: BiCGStab SpMV + (r0,v) dot              */
+    MC_KC_SCALAR_BATCH,  /* k_scalar_batch: 4 passive scalars assembled at once  */
+    MC_KC_IR_RESID,      /* k_ir_resid: FP64 residual of the refinement loop     */
+    MC_KC_MPNN_BWD,      /* message-passing net backward layer                   */
     MC_KC_COUNT
 };
 
@@ -228,12 +236,20 @@ int  mc_set_state_flags(mc_ctx *ctx, int first_pressure, int64_t timestep_count)
 int  mc_sync(mc_ctx *ctx);
 
 /* ---- measurement --------------------------------------------------------- */
-/* Profiling on: each kernel class is bracketed by HIP events on the context
-   stream; mc_profile_read returns accumulated milliseconds, launch counts and
-   algorithmic bytes per class (arrays of MC_KC_COUNT).                       */
-int  mc_profile_enable(mc_ctx *ctx, int on);
+/* Profiling: each launch of a class whose bit (1 << MC_KC_x) is set in `mask`
+   is bracketed by HIP events on the context stream, with no synchronisation
+   (events come from a pool and are resolved when read).  mc_profile_read
+   synchronises the stream and returns accumulated milliseconds, launch counts
+   and algorithmic bytes per class (arrays of MC_KC_COUNT).  mask 0 = off.    */
+int  mc_profile_enable(mc_ctx *ctx, int mask);
 int  mc_profile_read(mc_ctx *ctx, double *ms, int64_t *calls, double *bytes);
 int  mc_profile_reset(mc_ctx *ctx);
+/* Streaming copy of `bytes` (multiple of 16) between two fresh device buffers,
+   `reps` launches on the context stream; *ms_out = average launch time.  The
+   HBM bandwidth probe and the known-size kernel that scales the FETCH_SIZE /
+   WRITE_SIZE counters (kernel symbol mc::k_copy_probe).  No counterpart in the
+   reference: measurement only. */
+int  mc_copy_probe(mc_ctx *ctx, int64_t bytes, int reps, double *ms_out);
 
 /* ---- learned pressure-correction initial guess -------------------------- */
 int  mc_mpnn_load(mc_ctx *ctx, const float *weights, int64_t count, int hidden, int layers);

@@ -35,29 +35,57 @@ int dfree_all(Ctx *c)
     return MC_OK;
 }
 
-KTimer::KTimer(Ctx *c_, int kc_, double bytes_) : c(c_), kc(kc_), bytes(bytes_)
+static constexpr size_t EV_PAIRS_MAX = 4096;   // pending pairs before a fold
+
+KTimer::KTimer(Ctx *c_, int kc_, double bytes_) : c(c_)
 {
-    if (!c->profile) return;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, c->stream);
+    if (!(c->profile & (1u << kc_))) return;
+    if (c->ev_used == EV_PAIRS_MAX && profile_fold(c) != MC_OK) return;
+    const size_t i = c->ev_used;
+    if (2 * i + 2 > c->ev_pool.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess) return;
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+        c->ev_pool.push_back(a);
+        c->ev_pool.push_back(b);
+        c->ev_kc.push_back(0);
+        c->ev_bytes.push_back(0.0);
+    }
+    if (hipEventRecord(c->ev_pool[2 * i], c->stream) != hipSuccess) return;
+    c->ev_kc[i] = kc_;
+    c->ev_bytes[i] = bytes_;
+    c->ev_used++;
+    slot = (long)i;
 }
 
 KTimer::~KTimer()
 {
-    if (!c->profile || !a) return;
-    (void)hipEventRecord(b, c->stream);
-    (void)hipEventSynchronize(b);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, a, b);
-    c->kt_ms[kc] += ms;
-    c->kt_calls[kc] += 1;
-    c->kt_bytes[kc] += bytes;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (slot >= 0) (void)hipEventRecord(c->ev_pool[2 * slot + 1], c->stream);
 }
 
-void profile_flush(Ctx *) {}
+int profile_fold(Ctx *c)
+{
+    if (!c->ev_used) return MC_OK;
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->ev_used; i++) {
+        float ms = 0.f;
+        MC_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[2 * i], c->ev_pool[2 * i + 1]));
+        c->kt_ms[c->ev_kc[i]] += ms;
+        c->kt_calls[c->ev_kc[i]] += 1;
+        c->kt_bytes[c->ev_kc[i]] += c->ev_bytes[i];
+    }
+    c->ev_used = 0;
+    return MC_OK;
+}
+
+void profile_release(Ctx *c)
+{
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    c->ev_kc.clear();
+    c->ev_bytes.clear();
+    c->ev_used = 0;
+}
 
 static void set_constants(Ctx *c)
 {
@@ -79,15 +107,13 @@ static void set_constants(Ctx *c)
 static int alloc_workspace(Ctx *c)
 {
     const int N = c->N;
-    for (int i = 0; i < 16; i++) MC_TRY(dalloc(c, &c->wk[i], N));
+    for (int i = 0; i < 24; i++) MC_TRY(dalloc(c, &c->wk[i], N));
     for (int i = 0; i < 8; i++) {
         double *p = nullptr;
         MC_TRY(dalloc(c, &p, N));  // sized for doubles, used as float or double scratch
         c->wk32[i] = (float *)p;
     }
-    MC_TRY(dalloc(c, &c->red, 64 * 2048));
-    MC_TRY(dalloc(c, &c->red_cnt, 64));
-    MC_HIP(c, hipMemset(c->red_cnt, 0, 64 * sizeof(unsigned)));
+    MC_TRY(dalloc(c, &c->red, RED_PARTIALS));
     MC_TRY(dalloc(c, &c->dscal, 1024));
     MC_HIP(c, hipMemset(c->dscal, 0, 1024 * sizeof(double)));
     MC_HIP(c, hipHostMalloc((void **)&c->hscal, 1024 * sizeof(double), hipHostMallocDefault));
@@ -199,6 +225,7 @@ int mc_ctx_destroy(mc_ctx *x)
     Ctx *c = &x->c;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     mpnn_destroy(c);
+    profile_release(c);
     dfree_all(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -486,16 +513,18 @@ int mc_sync(mc_ctx *x)
     return MC_OK;
 }
 
-int mc_profile_enable(mc_ctx *x, int on)
+int mc_profile_enable(mc_ctx *x, int mask)
 {
     CTX_OR_FAIL
-    c->profile = on != 0;
+    MC_TRY(profile_fold(c));
+    c->profile = (unsigned)mask & ((1u << MC_KC_COUNT) - 1u);
     return MC_OK;
 }
 
 int mc_profile_read(mc_ctx *x, double *ms, int64_t *calls, double *bytes)
 {
     CTX_OR_FAIL
+    MC_TRY(profile_fold(c));
     for (int k = 0; k < MC_KC_COUNT; k++) {
         if (ms) ms[k] = c->kt_ms[k];
         if (calls) calls[k] = c->kt_calls[k];
@@ -507,8 +536,20 @@ int mc_profile_read(mc_ctx *x, double *ms, int64_t *calls, double *bytes)
 int mc_profile_reset(mc_ctx *x)
 {
     CTX_OR_FAIL
+    MC_TRY(profile_fold(c));
     for (int k = 0; k < MC_KC_COUNT; k++) { c->kt_ms[k] = 0; c->kt_calls[k] = 0; c->kt_bytes[k] = 0; }
     memset(&c->st, 0, sizeof(c->st));
+    return MC_OK;
+}
+
+int mc_copy_probe(mc_ctx *x, int64_t bytes, int reps, double *ms_out)
+{
+    CTX_OR_FAIL
+    if (bytes < 16 || bytes % 16 || reps < 1 || !ms_out) {
+        c->err = "copy probe: bytes a positive multiple of 16, reps >= 1, ms_out set";
+        return MC_ERR_ARG;
+    }
+    MC_TRY(copy_probe(c, bytes, reps, ms_out));
     return MC_OK;
 }
 

@@ -745,9 +745,11 @@ int calculate_pressure(Ctx *c)
         MC_TRY(pressure_boundary_slots(c, PP));
         MC_TRY(grad_single(c, PP, true));
         {
-            KTimer kt(c, MC_KC_CORRECT, 0.0);
-            hipLaunchKernelGGL(k_flux_correct, dim3(div_up(c->F, 256)), dim3(256), 0, c->stream, geo(c), c->phi[PP],
-                               c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
+            {
+                KTimer kt(c, MC_KC_CORRECT, 56.0 * c->N + 88.0 * c->F);
+                hipLaunchKernelGGL(k_flux_correct, dim3(div_up(c->F, 256)), dim3(256), 0, c->stream, geo(c),
+                                   c->phi[PP], c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
+            }
             const int which = simple ? 2 : (c->first_pressure ? 1 : 0);
             hipLaunchKernelGGL(k_cell_correct, dim3(div_up(N, 256)), dim3(256), 0, c->stream, geo(c), c->fcorr, c->SU,
                                c->phi[P], c->phi[PP], c->phi[U], c->phi[V], c->phi[W], c->gx[PP], c->gy[PP],
@@ -1090,7 +1092,7 @@ int scalar_batch_solve(Ctx *c)
     a.type = FUEL;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
     {
-        KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * SB * c->N + 56.0 * c->F + 8.0 * c->nnz);
+        KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * SB * c->N + 56.0 * c->F + 8.0 * c->nnz);
         hipLaunchKernelGGL(k_scalar_batch, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
         MC_LAUNCH_CHECK(c);
     }

@@ -40,6 +40,7 @@ struct Consts {
 };
 
 constexpr int DIA_MAX = 32;     // most diagonals a diagonal-storage matrix may have
+constexpr int RED_PARTIALS = 16 * 8192;   // doubles in the reduction partials buffer
 constexpr int DENSE_MAX = 128;  // largest coarsest AMG level (inverted in LDS)
 
 // Per-level storage of the aggregation-AMG hierarchy.
@@ -128,10 +129,9 @@ struct Ctx {
     int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
     void *cub_tmp = nullptr;       // hipcub scratch for the compaction
     size_t cub_tmp_bytes = 0;
-    double *wk[16] = {};           // N-length work vectors
+    double *wk[24] = {};           // N-length work vectors
     float *wk32[8] = {};
     double *red = nullptr;         // reduction partials
-    unsigned *red_cnt = nullptr;   // last-block counters
     double *dscal = nullptr;       // device scalars for the solvers
     double *hscal = nullptr;       // pinned host mirror
     int red_blocks = 0;
@@ -151,9 +151,11 @@ struct Ctx {
 
     // ---- stats / timing ----
     mc_stats st{};
-    bool profile = false;
-    hipEvent_t ev[64] = {};
-    int n_ev = 0;
+    unsigned profile = 0;                // class mask of timed kernels
+    std::vector<hipEvent_t> ev_pool;     // 2 events per pending launch
+    std::vector<int> ev_kc;              // class of each pending pair
+    std::vector<double> ev_bytes;
+    size_t ev_used = 0;                  // pairs recorded since the last fold
     double kt_ms[MC_KC_COUNT] = {};      // accumulated kernel-class time
     int64_t kt_calls[MC_KC_COUNT] = {};
     double kt_bytes[MC_KC_COUNT] = {};   // accumulated algorithmic bytes
@@ -219,41 +221,61 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*4 *
     }
 }
 
-// Deterministic two-level reduction: every block stores NV partials, the
-// last block to finish (vector atomic on a global counter) sums them in a
-// fixed order and returns true in thread 0 with the totals in v.
+// Deterministic two-kernel reduction.  Each block of the producing kernel
+// stores its NV block sums with plain stores (partials[k * nblocks + block]);
+// the end of that kernel makes them visible, and a one-block finishing kernel
+// (k_finish) sums them in a fixed order and applies an epilogue.  A "last
+// block finishes" scheme inside the producer would need a device-scope
+// release fence and a counter atomic in every block, which on a multi-XCD
+// part costs far more than the second launch.
 template <int NV>
-__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double *partials, unsigned *counter,
-                                                  int bid, int nblocks)
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partials, int bid, int nblocks)
 {
     __shared__ double lds[NV * 4];
-    __shared__ bool am_last;
     block_sum<NV>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) partials[(size_t)k * nblocks + bid] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        am_last = (prev == (unsigned)(nblocks - 1));
     }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double acc[NV];
+}
+
+// Fixed-order sum of the partials of `nblocks` blocks (one block of 256
+// threads); the totals are valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *partials, int nblocks, double (&v)[NV])
+{
+    __shared__ double lds[NV * 4];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
-            s += __builtin_nontemporal_load(&partials[(size_t)k * nblocks + i]);
-        acc[k] = s;
+        const double *pk = partials + (size_t)k * nblocks;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int step = blockDim.x;
+        int i = threadIdx.x;
+        for (; i + 3 * step < nblocks; i += 4 * step) {
+            s0 += pk[i];
+            s1 += pk[i + step];
+            s2 += pk[i + 2 * step];
+            s3 += pk[i + 3 * step];
+        }
+        for (; i < nblocks; i += step) s0 += pk[i];
+        v[k] = (s0 + s1) + (s2 + s3);
     }
-    block_sum<NV>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) v[k] = acc[k];
-        *counter = 0u;  // re-arm for the next launch (stream order)
-    }
-    return threadIdx.x == 0;
+    block_sum<NV>(v, lds);
+}
+
+// Finishing kernel: one block of 256 threads; thread 0 runs epi(totals).
+template <int NV, class Epi>
+__global__ void __launch_bounds__(256) k_finish(const double *partials, int nblocks, Epi epi)
+{
+    double v[NV];
+    sum_partials<NV>(partials, nblocks, v);
+    if (threadIdx.x == 0) epi(v);
+}
+
+template <int NV, class Epi>
+inline void launch_finish(hipStream_t s, const double *partials, int nblocks, const Epi &epi)
+{
+    hipLaunchKernelGGL((k_finish<NV, Epi>), dim3(1), dim3(256), 0, s, partials, nblocks, epi);
 }
 
 // ------------------------------------------------------------ host utils --
@@ -265,16 +287,18 @@ inline int dalloc(Ctx *c, T **p, size_t count)
 }
 int dfree_all(Ctx *c);
 
-// profiling of kernel classes with HIP events on the context stream
+// profiling of kernel classes with HIP events on the context stream: the
+// constructor records the start event, the destructor the end event; pairs
+// are resolved by profile_fold (which synchronises the stream)
 struct KTimer {
     Ctx *c;
-    int kc;
-    double bytes;
-    hipEvent_t a = nullptr, b = nullptr;
+    long slot = -1;
     KTimer(Ctx *c_, int kc_, double bytes_);
     ~KTimer();
 };
-void profile_flush(Ctx *c);
+int profile_fold(Ctx *c);
+int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out);   // mc_linalg.hip
+void profile_release(Ctx *c);
 
 // stages (mc_mesh.hip)
 int mesh_box_generate(Ctx *c, int64_t m);

@@ -39,7 +39,8 @@
 namespace mc {
 
 // ------------------------------------------------------------ layout --
-constexpr int RED_MAX_BLOCKS = 2048;
+constexpr int RED_MAX_BLOCKS = 8192;   // partial sums: at most 12 per block (RED_PARTIALS)
+static_assert(12 * RED_MAX_BLOCKS <= RED_PARTIALS, "reduction partials buffer");
 constexpr int BLOCK = 256;
 
 struct Grid {
@@ -79,6 +80,14 @@ struct Mat {
     int nd;
     int off[DIA_MAX];
 };
+
+// bytes of one pass over a matrix: DIA values, or CSR values + columns + rows
+template <typename T>
+static double mat_bytes(const Mat<T> &A, int64_t nnz, size_t vsize)
+{
+    return A.nd ? (double)vsize * A.nd * A.n : (double)(vsize + 4) * nnz + 4.0 * (A.n + 1);
+}
+
 
 __device__ __forceinline__ int clampi(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
 
@@ -233,23 +242,32 @@ int spmv(Ctx *c, const double *vals, const double *x, double *y)
 }
 
 // ------------------------------------------------------- BiCGStab --
+// One iteration is three streaming kernels, each followed by a one-block
+// finishing kernel for its dot products:
+//   pv: p = r + beta (p' - omega v'), formed on the fly (also where a row
+//       reads it at a neighbour); v = A p; (r^, v)       -> alpha
+//   st: s = r - alpha v, formed on the fly; t = A s;
+//       (t, s), (t, t), (s, s)                           -> half-step test, omega
+//   xr: x += alpha p + omega s; r = s - omega t;
+//       (r, r), (r^, r)                                  -> convergence, rho, beta
+// p and v alternate between two buffers, since rows read the previous
+// iteration's values at their neighbours while others write the new ones.
 struct BiVecs {
-    double *x[4], *r[4], *p[4], *v[4], *s[4], *t[4];
-    const double *rh[4];
+    double *x[4], *r[4], *s[4], *t[4];
+    double *p[4], *v[4];            // this iteration's
+    const double *po[4], *vo[4];    // the previous iteration's
+    const double *rh[4];            // r^ (= b, never written)
     const double *b[4];
 };
 
 template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt,
-                                                   int chunk, double rtol, int maxit)
+__global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, double *partials, int chunk)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = b * chunk, i1 = min(n, i0 + chunk);
     double acc[NR];
 #pragma unroll
     for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    // r^ is b itself (never written during the solve); p and v start at zero
-    // implicitly (k_bi_p reads them only after the first iteration)
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
 #pragma unroll
         for (int k = 0; k < NR; k++) {
@@ -258,15 +276,24 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, 
             acc[k] += bi * bi;
         }
     }
-    if (block_reduce_last<NR>(acc, partials, cnt, b, gridDim.x)) {
+    block_partials<NR>(acc, partials, b, gridDim.x);
+}
+
+template <int NR>
+struct EpiBiInit {
+    KState *ks;
+    double rtol;
+    int maxit;
+    __device__ void operator()(const double (&v)[NR]) const
+    {
         int all = 1;
         for (int k = 0; k < NR; k++) {
-            const double bn = sqrt(acc[k]);
+            const double bn = sqrt(v[k]);
             ks->bnorm[k] = bn; ks->tol[k] = rtol * bn; ks->rnorm[k] = bn;
             ks->rho_old[k] = 1.0; ks->alpha[k] = 1.0; ks->omega[k] = 1.0;
-            ks->rho[k] = acc[k];  // (r^, r) with r^ = r = b
-            ks->beta[k] = (ks->rho[k] / 1.0) * (1.0 / 1.0);
-            ks->iters[k] = 0; ks->half[k] = 0;
+            ks->rho[k] = v[k];  // (r^, r) with r^ = r = b
+            ks->beta[k] = 0.0;
+            ks->iters[k] = 0; ks->half[k] = 0; ks->sn[k] = 0.0;
             ks->done[k] = (bn == 0.0) ? 1 : 0;
             ks->conv[k] = (bn == 0.0) ? 1 : 0;
             all &= ks->done[k];
@@ -274,135 +301,145 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, KState *ks, 
         ks->nr = NR; ks->maxit = maxit;
         ks->all_done = all;
     }
-}
+};
 
-template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_p(int n, BiVecs w, const KState *ks, int chunk)
-{
-    if (ks->all_done) return;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            if (ks->done[k]) continue;
-            const double beta = ks->beta[k], om = ks->omega[k];
-            const bool first = ks->iters[k] == 0;
-            const double pold = first ? 0.0 : w.p[k][i], vold = first ? 0.0 : w.v[k][i];
-            w.p[k][i] = w.r[k][i] + beta * (pold - om * vold);
-        }
-    }
-}
-
-// v = A p ; alpha = rho / (r^, v)
+// v = A p with p formed on the fly ; partials of (r^, v)
 template <int ND, int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_spmv_v(Mat<double> A, BiVecs w, KState *ks, double *partials,
-                                                     unsigned *cnt, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_bi_pv(Mat<double> A, BiVecs w, const KState *ks, double *partials,
+                                                 int chunk)
 {
     if (ks->all_done) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
-    double dsum[NR];
+    bool skip[NR], first[NR];
+    double beta[NR], om[NR], dsum[NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) dsum[k] = 0.0;
-    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return w.p[k][j]; }, [&](int r, const double (&ax)[NR]) {
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            if (ks->done[k]) continue;
-            w.v[k][r] = ax[k];
-            dsum[k] += w.rh[k][r] * ax[k];
-        }
-    });
-    if (block_reduce_last<NR>(dsum, partials, cnt, b, gridDim.x)) {
-        for (int k = 0; k < NR; k++)
-            if (!ks->done[k]) ks->alpha[k] = ks->rho[k] / dsum[k];
+    for (int k = 0; k < NR; k++) {
+        skip[k] = ks->done[k] != 0;
+        first[k] = ks->iters[k] == 0;
+        beta[k] = ks->beta[k];
+        om[k] = ks->omega[k];
+        dsum[k] = 0.0;
     }
+    auto P = [&](int k, int j) -> double {
+        if (first[k]) return w.r[k][j];
+        return fma(beta[k], fma(-om[k], w.vo[k][j], w.po[k][j]), w.r[k][j]);
+    };
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return skip[k] ? 0.0 : P(k, j); },
+                     [&](int r, const double (&ax)[NR]) {
+#pragma unroll
+                         for (int k = 0; k < NR; k++) {
+                             if (skip[k]) continue;
+                             w.p[k][r] = P(k, r);
+                             w.v[k][r] = ax[k];
+                             dsum[k] += w.rh[k][r] * ax[k];
+                         }
+                     });
+    block_partials<NR>(dsum, partials, b, gridDim.x);
 }
 
-// s = r - alpha v ; half-step convergence test on ||s||
 template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_s(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt, int chunk)
+struct EpiBiAlpha {
+    KState *ks;
+    __device__ void operator()(const double (&v)[NR]) const
+    {
+        if (ks->all_done) return;
+        for (int k = 0; k < NR; k++)
+            if (!ks->done[k]) ks->alpha[k] = ks->rho[k] / v[k];
+    }
+};
+
+// t = A s with s = r - alpha v formed on the fly ; partials of (t,s), (t,t), (s,s)
+template <int ND, int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_st(Mat<double> A, BiVecs w, const KState *ks, double *partials,
+                                                 int chunk)
 {
     if (ks->all_done) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = b * chunk, i1 = min(n, i0 + chunk);
-    double acc[NR];
+    const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
+    bool skip[NR];
+    double al[NR], dsum[3 * NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            if (ks->done[k]) continue;
-            const double s = w.r[k][i] - ks->alpha[k] * w.v[k][i];
-            w.s[k][i] = s;
-            acc[k] += s * s;
-        }
+    for (int k = 0; k < NR; k++) {
+        skip[k] = ks->done[k] != 0;
+        al[k] = ks->alpha[k];
+        dsum[3 * k] = dsum[3 * k + 1] = dsum[3 * k + 2] = 0.0;
     }
-    if (block_reduce_last<NR>(acc, partials, cnt, b, gridDim.x)) {
+    auto S = [&](int k, int j) -> double { return fma(-al[k], w.v[k][j], w.r[k][j]); };
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return skip[k] ? 0.0 : S(k, j); },
+                     [&](int r, const double (&ax)[NR]) {
+#pragma unroll
+                         for (int k = 0; k < NR; k++) {
+                             if (skip[k]) continue;
+                             const double sv = S(k, r);
+                             w.s[k][r] = sv;
+                             w.t[k][r] = ax[k];
+                             dsum[3 * k] += ax[k] * sv;
+                             dsum[3 * k + 1] += ax[k] * ax[k];
+                             dsum[3 * k + 2] += sv * sv;
+                         }
+                     });
+    block_partials<3 * NR>(dsum, partials, b, gridDim.x);
+}
+
+template <int NR>
+struct EpiBiOmega {
+    KState *ks;
+    __device__ void operator()(const double (&v)[3 * NR]) const
+    {
+        if (ks->all_done) return;
         for (int k = 0; k < NR; k++) {
             if (ks->done[k]) continue;
-            const double sn = sqrt(acc[k]);
+            const double sn = sqrt(v[3 * k + 2]);
             ks->sn[k] = sn;
             ks->half[k] = (sn <= ks->tol[k]) ? 1 : 0;
+            if (!ks->half[k]) ks->omega[k] = v[3 * k] / v[3 * k + 1];
         }
     }
-}
+};
 
-// t = A s ; omega = (t,s)/(t,t)
-template <int ND, int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_spmv_t(Mat<double> A, BiVecs w, KState *ks, double *partials,
-                                                     unsigned *cnt, int chunk)
-{
-    if (ks->all_done) return;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = b * chunk, r1 = min(A.n, r0 + chunk);
-    double dsum[2 * NR];
-#pragma unroll
-    for (int k = 0; k < 2 * NR; k++) dsum[k] = 0.0;
-    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return w.s[k][j]; }, [&](int r, const double (&ax)[NR]) {
-#pragma unroll
-        for (int k = 0; k < NR; k++) {
-            if (ks->done[k] || ks->half[k]) continue;
-            w.t[k][r] = ax[k];
-            dsum[2 * k] += ax[k] * w.s[k][r];
-            dsum[2 * k + 1] += ax[k] * ax[k];
-        }
-    });
-    if (block_reduce_last<2 * NR>(dsum, partials, cnt, b, gridDim.x)) {
-        for (int k = 0; k < NR; k++)
-            if (!ks->done[k] && !ks->half[k]) ks->omega[k] = dsum[2 * k] / dsum[2 * k + 1];
-    }
-}
-
-// x += alpha p + omega s ; r = s - omega t ; next rho ; convergence
+// x += alpha p + omega s ; r = s - omega t ; partials of (r,r), (r^,r)
 template <int NR>
-__global__ void __launch_bounds__(BLOCK) k_bi_xr(int n, BiVecs w, KState *ks, double *partials, unsigned *cnt, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_bi_xr(int n, BiVecs w, const KState *ks, double *partials, int chunk)
 {
     if (ks->all_done) return;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = b * chunk, i1 = min(n, i0 + chunk);
-    double acc[2 * NR];
+    bool skip[NR], half[NR];
+    double al[NR], om[NR], acc[2 * NR];
 #pragma unroll
-    for (int k = 0; k < 2 * NR; k++) acc[k] = 0.0;
+    for (int k = 0; k < NR; k++) {
+        skip[k] = ks->done[k] != 0;
+        half[k] = ks->half[k] != 0;
+        al[k] = ks->alpha[k];
+        om[k] = ks->omega[k];
+        acc[2 * k] = acc[2 * k + 1] = 0.0;
+    }
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
 #pragma unroll
         for (int k = 0; k < NR; k++) {
-            if (ks->done[k]) continue;
-            const double al = ks->alpha[k];
-            if (ks->half[k]) {
-                w.x[k][i] += al * w.p[k][i];
+            if (skip[k]) continue;
+            if (half[k]) {
+                w.x[k][i] = fma(al[k], w.p[k][i], w.x[k][i]);
             } else {
-                const double om = ks->omega[k];
-                const double s = w.s[k][i];
-                w.x[k][i] += al * w.p[k][i] + om * s;
-                const double r = s - om * w.t[k][i];
+                const double sv = w.s[k][i];
+                w.x[k][i] = fma(om[k], sv, fma(al[k], w.p[k][i], w.x[k][i]));
+                const double r = fma(-om[k], w.t[k][i], sv);
                 w.r[k][i] = r;
                 acc[2 * k] += r * r;
                 acc[2 * k + 1] += w.rh[k][i] * r;
             }
         }
     }
-    if (block_reduce_last<2 * NR>(acc, partials, cnt, b, gridDim.x)) {
+    block_partials<2 * NR>(acc, partials, b, gridDim.x);
+}
+
+template <int NR>
+struct EpiBiXr {
+    KState *ks;
+    __device__ void operator()(const double (&v)[2 * NR]) const
+    {
+        if (ks->all_done) return;
         int all = 1;
         for (int k = 0; k < NR; k++) {
             if (!ks->done[k]) {
@@ -411,12 +448,12 @@ __global__ void __launch_bounds__(BLOCK) k_bi_xr(int n, BiVecs w, KState *ks, do
                     ks->rnorm[k] = ks->sn[k];
                     ks->done[k] = 1; ks->conv[k] = 1;
                 } else {
-                    const double rn = sqrt(acc[2 * k]);
+                    const double rn = sqrt(v[2 * k]);
                     ks->rnorm[k] = rn;
                     ks->rho_old[k] = ks->rho[k];
-                    ks->rho[k] = acc[2 * k + 1];
+                    ks->rho[k] = v[2 * k + 1];
                     if (rn <= ks->tol[k]) { ks->done[k] = 1; ks->conv[k] = 1; }
-                    else if (ks->rho[k] == 0.0 || ks->iters[k] >= ks->maxit) ks->done[k] = 1;
+                    else if (ks->rho[k] == 0.0 || ks->iters[k] >= ks->maxit || !(rn == rn)) ks->done[k] = 1;
                     else ks->beta[k] = (ks->rho[k] / ks->rho_old[k]) * (ks->alpha[k] / ks->omega[k]);
                 }
             }
@@ -424,7 +461,7 @@ __global__ void __launch_bounds__(BLOCK) k_bi_xr(int n, BiVecs w, KState *ks, do
         }
         ks->all_done = all;
     }
-}
+};
 
 static int poll_done(Ctx *c, KState *ks, KState *host)
 {
@@ -450,46 +487,50 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
 {
     static_assert(NR <= 4, "NR");
     const int n = c->N;
-    BiVecs w{};
-    // five work vectors per RHS (r, p, v, s, t) from the 16 FP64 work vectors
-    // and the 8 scratch vectors (sized for doubles); r^ aliases b
-    double *pool[24];
+    // seven work vectors per RHS (r, s, t and two each of p, v) from the 24
+    // FP64 work vectors and the 8 scratch vectors (sized for doubles)
+    double *pool[32];
     int np = 0;
-    for (int i = 0; i < 16; i++) pool[np++] = c->wk[i];
+    for (int i = 0; i < 24; i++) pool[np++] = c->wk[i];
     for (int i = 0; i < 8; i++) pool[np++] = (double *)c->wk32[i];
+    BiVecs w[2] = {};
     int q = 0;
     for (int k = 0; k < NR; k++) {
-        w.b[k] = b[k]; w.x[k] = x[k]; w.rh[k] = b[k];
-        w.r[k] = pool[q++]; w.p[k] = pool[q++]; w.v[k] = pool[q++]; w.s[k] = pool[q++]; w.t[k] = pool[q++];
+        double *r = pool[q++], *s = pool[q++], *t = pool[q++];
+        double *pa = pool[q++], *pb = pool[q++], *va = pool[q++], *vb = pool[q++];
+        for (int e = 0; e < 2; e++) {
+            w[e].b[k] = b[k]; w[e].x[k] = x[k]; w[e].rh[k] = b[k];
+            w[e].r[k] = r; w[e].s[k] = s; w[e].t[k] = t;
+        }
+        w[0].p[k] = pa; w[0].v[k] = va; w[0].po[k] = pb; w[0].vo[k] = vb;
+        w[1].p[k] = pb; w[1].v[k] = vb; w[1].po[k] = pa; w[1].vo[k] = va;
     }
     MC_TRY(fine_dia_from(c, vals, c->A_dia));
     const Mat<double> A = fine_mat(c, vals, c->A_dia);
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     const Grid g = grid_for(n);
-    hipLaunchKernelGGL(k_bi_init<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt, g.chunk,
-                       rtol, maxit);
+    hipLaunchKernelGGL(k_bi_init<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w[0], c->red, g.chunk);
+    launch_finish<NR>(c->stream, c->red, g.blocks, EpiBiInit<NR>{ks, rtol, maxit});
     MC_LAUNCH_CHECK(c);
-    const double mbytes = (c->fine_nd ? 8.0 * c->nnz + 16.0 * n : 12.0 * c->nnz + 20.0 * n) + 8.0 * n * (NR - 1) * 2;
+    // A; r, p', v', r^ read and p, v written per RHS
+    const double mbytes = mat_bytes(A, c->nnz, 8) + 48.0 * NR * n;
     int it = 0, batch = 1;
     double rn_prev = -1.0;
     for (;;) {
         for (int j = 0; j < batch && it < maxit; j++, it++) {
-            hipLaunchKernelGGL(k_bi_p<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, g.chunk);
+            const BiVecs &W = w[it & 1];
             {
-                KTimer kt(c, MC_KC_SPMV64, mbytes);
-                MC_DISPATCH_ND(A.nd, (k_bi_spmv_v<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w, ks,
-                               c->red, c->red_cnt, g.chunk);
-            }
-            hipLaunchKernelGGL(k_bi_s<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
+                KTimer kt(c, MC_KC_BICG, mbytes);
+                MC_DISPATCH_ND(A.nd, (k_bi_pv<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
                                g.chunk);
-            {
-                KTimer kt(c, MC_KC_SPMV64, mbytes);
-                MC_DISPATCH_ND(A.nd, (k_bi_spmv_t<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w, ks,
-                               c->red, c->red_cnt, g.chunk);
             }
-            hipLaunchKernelGGL(k_bi_xr<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w, ks, c->red, c->red_cnt,
-                               g.chunk);
+            launch_finish<NR>(c->stream, c->red, g.blocks, EpiBiAlpha<NR>{ks});
+            MC_DISPATCH_ND(A.nd, (k_bi_st<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
+                           g.chunk);
+            launch_finish<3 * NR>(c->stream, c->red, g.blocks, EpiBiOmega<NR>{ks});
+            hipLaunchKernelGGL(k_bi_xr<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, W, ks, c->red, g.chunk);
+            launch_finish<2 * NR>(c->stream, c->red, g.blocks, EpiBiXr<NR>{ks});
         }
         MC_LAUNCH_CHECK(c);
         MC_TRY(poll_done(c, ks, hs));
@@ -527,14 +568,27 @@ int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double 
 }
 
 // ------------------------------------------------- generic reductions --
-__global__ void __launch_bounds__(BLOCK) k_sum(int n, const double *v, double *out, double *partials, unsigned *cnt,
-                                               int chunk, double scale)
+__global__ void __launch_bounds__(BLOCK) k_sum(int n, const double *v, double *partials, int chunk)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = b * chunk, i1 = min(n, i0 + chunk);
     double acc[1] = {0.0};
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += v[i];
-    if (block_reduce_last<1>(acc, partials, cnt, b, gridDim.x)) *out = acc[0] * scale;
+    block_partials<1>(acc, partials, b, gridDim.x);
+}
+
+struct EpiScale {
+    double *out;
+    double scale;
+    __device__ void operator()(const double (&v)[1]) const { *out = v[0] * scale; }
+};
+
+// *out = scale * sum(v[0..n)) on the stream
+static void sum_scaled(Ctx *c, const double *v, int n, double *out, double scale)
+{
+    const Grid g = grid_for(n);
+    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, c->red, g.chunk);
+    launch_finish<1>(c->stream, c->red, g.blocks, EpiScale{out, scale});
 }
 
 __global__ void k_sub_scalar(int n, const double *in, const double *s, double *out)
@@ -545,40 +599,34 @@ __global__ void k_sub_scalar(int n, const double *in, const double *s, double *o
 
 int mean_project(Ctx *c, const double *b, double *out, int n)
 {
-    const Grid g = grid_for(n);
     double *mean = c->dscal + 512;
-    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 8, g.chunk,
-                       1.0 / (double)n);
+    sum_scaled(c, b, n, mean, 1.0 / (double)n);
     hipLaunchKernelGGL(k_sub_scalar, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n, b, mean, out);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
 
 // max(PP[0], max_{i>=1} |PP[i]|), the reference's Pressure_correction_max
-// (FlowSolver.inl:2371-2378)
-__global__ void __launch_bounds__(BLOCK) k_maxabs(int n, const double *v, double *out, double *partials, unsigned *cnt,
-                                                  int chunk)
+// (FlowSolver.inl:2371-2378): block maxima, then one block combines them
+__global__ void __launch_bounds__(BLOCK) k_maxabs(int n, const double *v, double *partials, int chunk)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = b * chunk, i1 = min(n, i0 + chunk);
     double m = 0.0;
     for (int i = max(i0, 1) + threadIdx.x; i < i1; i += BLOCK) m = fmax(m, fabs(v[i]));
     __shared__ double lds[BLOCK / 64];
-    __shared__ bool am_last;
     for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        m = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
-        partials[b] = m;
-        __threadfence();
-        am_last = atomicAdd(cnt, 1u) == (unsigned)(gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
+    if (threadIdx.x == 0) partials[b] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+__global__ void __launch_bounds__(BLOCK) k_maxabs_finish(const double *v, const double *partials, int nblocks,
+                                                         double *out)
+{
     double mm = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK) mm = fmax(mm, __builtin_nontemporal_load(&partials[i]));
+    for (int i = threadIdx.x; i < nblocks; i += BLOCK) mm = fmax(mm, partials[i]);
+    __shared__ double lds[BLOCK / 64];
     for (int off = 32; off > 0; off >>= 1) mm = fmax(mm, __shfl_down(mm, off, 64));
     if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mm;
     __syncthreads();
@@ -586,7 +634,6 @@ __global__ void __launch_bounds__(BLOCK) k_maxabs(int n, const double *v, double
         mm = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
         const double p0 = v[0];
         *out = (p0 != p0) ? p0 : ((mm > p0) ? mm : p0);
-        *cnt = 0u;
     }
 }
 
@@ -594,7 +641,8 @@ int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host)
 {
     const Grid g = grid_for(c->N);
     double *d = c->dscal + 520;
-    hipLaunchKernelGGL(k_maxabs, dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, v, d, c->red, c->red_cnt + 9, g.chunk);
+    hipLaunchKernelGGL(k_maxabs, dim3(g.blocks), dim3(BLOCK), 0, c->stream, c->N, v, c->red, g.chunk);
+    hipLaunchKernelGGL(k_maxabs_finish, dim3(1), dim3(BLOCK), 0, c->stream, v, c->red, g.blocks, d);
     MC_LAUNCH_CHECK(c);
     MC_HIP(c, hipMemcpyAsync(c->hscal + 520, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
@@ -908,12 +956,12 @@ __global__ void __launch_bounds__(BLOCK) k_jac_first2(Mat<T> A, const T *dinv, c
 }
 
 // x_out = y + w D^-1 (b - A y), y_j = x_j + cs * xc[agg_j] (agg == nullptr: y = x).
-// With dotv != nullptr the last sweep also forms (dotv, x_out): the PCG's
-// (r, z), finished into rho/beta by the last block.
+// With dotv != nullptr the last sweep also forms the block partials of
+// (dotv, x_out): the PCG's (r, z), finished into rho / beta by EpiRz.
 template <int ND, typename T>
 __global__ void __launch_bounds__(BLOCK) k_jac(Mat<T> A, const T *dinv, const T *b, const T *x, const int *agg,
                                                const T *xc, T cs, T *xout, T omega, int chunk, const T *dotv,
-                                               KState *ks, double *partials, unsigned *cnt, const KState *stop)
+                                               double *partials, const KState *stop)
 {
     if (stop && stop->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
@@ -926,12 +974,20 @@ __global__ void __launch_bounds__(BLOCK) k_jac(Mat<T> A, const T *dinv, const T 
         xout[r] = o;
         if (dotv) acc[0] += (double)dotv[r] * (double)o;
     });
-    if (dotv && block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
-        ks->rho[0] = acc[0];
-        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
-        ks->rho_old[0] = acc[0];
-    }
+    if (dotv) block_partials<1>(acc, partials, bb, gridDim.x);
 }
+
+// rho = (r, z), beta = rho / rho_old of the PCG
+struct EpiRz {
+    KState *ks;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        if (ks->all_done) return;
+        ks->rho[0] = v[0];
+        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : v[0] / ks->rho_old[0];
+        ks->rho_old[0] = v[0];
+    }
+};
 
 // bc[I] = sum_{i in aggregate I} (b_i - (A x)_i): residual fused with
 // restriction.  8 lanes per coarse row; DIA: lane m evaluates member m's row;
@@ -986,21 +1042,17 @@ __global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, c
     x[i] = (T)s;
 }
 
-// rho = (r, z), beta = rho / rho_old (PCG), when no sweep can fuse it
+// partials of (r, z) when no smoothing sweep can carry them
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_rz(int n, const T *r, const T *z, KState *ks, double *partials,
-                                              unsigned *cnt, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_rz(int n, const T *r, const T *z, const KState *ks, double *partials,
+                                              int chunk)
 {
     if (ks->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
     double acc[1] = {0.0};
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) acc[0] += (double)r[i] * (double)z[i];
-    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
-        ks->rho[0] = acc[0];
-        ks->beta[0] = (ks->iters[0] == 0) ? 0.0 : acc[0] / ks->rho_old[0];
-        ks->rho_old[0] = acc[0];
-    }
+    block_partials<1>(acc, partials, bb, gridDim.x);
 }
 
 template <typename T> static T *lv_x(AmgLevel &L);
@@ -1028,8 +1080,8 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x, stop);
         if (ks) {  // one-level hierarchy: no smoothing sweep carries the (r, z) reduction
             const Grid g = grid_for(L.n);
-            hipLaunchKernelGGL(k_rz<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, b, x, ks, c->red, c->red_cnt,
-                               g.chunk);
+            hipLaunchKernelGGL(k_rz<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, b, x, ks, c->red, g.chunk);
+            launch_finish<1>(c->stream, c->red, g.blocks, EpiRz{ks});
         }
         return MC_OK;
     }
@@ -1042,7 +1094,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     for (int s = 2; s < nu; s++) {
         MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
                        (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk, (const T *)nullptr,
-                       (KState *)nullptr, (double *)nullptr, (unsigned *)nullptr, stop);
+                       (double *)nullptr, stop);
         std::swap(cur, oth);
     }
     AmgLevel &Lc = c->amg[l + 1];
@@ -1058,9 +1110,18 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         T *dst = last ? x : oth;
         if (dst == cur) dst = tmp;
         const T *dotv = (last && ks) ? b : nullptr;
-        MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
-                       first ? L.agg : (const int *)nullptr, first ? xc : (const T *)nullptr, first ? cs : (T)0, dst,
-                       omega, g.chunk, dotv, ks, c->red, c->red_cnt, stop);
+        // A, dinv, b, x read, x' written; the first sweep also reads agg and
+        // the coarse correction
+        const double jbytes = (l == 0) ? mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n +
+                                             (first ? 4.0 * L.n + (double)sizeof(T) * L.nc : 0.0)
+                                       : 0.0;
+        {
+            KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
+            MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+                           first ? L.agg : (const int *)nullptr, first ? xc : (const T *)nullptr, first ? cs : (T)0,
+                           dst, omega, g.chunk, dotv, c->red, stop);
+        }
+        if (dotv) launch_finish<1>(c->stream, c->red, g.blocks, EpiRz{ks});
         oth = cur;
         cur = dst;
     }
@@ -1075,9 +1136,8 @@ struct PcgVecs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const double *mean, PcgVecs<T> w, KState *ks,
-                                                    double *partials, unsigned *cnt, int chunk, double rtol,
-                                                    double atol, int maxit)
+__global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const double *mean, PcgVecs<T> w,
+                                                    double *partials, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
@@ -1088,8 +1148,16 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const dou
         w.r[i] = r; w.x[i] = (T)0; w.p[i] = (T)0;
         acc[0] += (double)r * (double)r;
     }
-    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
-        const double bn = sqrt(acc[0]);
+    block_partials<1>(acc, partials, bb, gridDim.x);
+}
+
+struct EpiPcgInit {
+    KState *ks;
+    double rtol, atol;
+    int maxit;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        const double bn = sqrt(v[0]);
         ks->bnorm[0] = bn; ks->rnorm[0] = bn;
         ks->tol[0] = fmax(rtol * bn, atol);
         ks->rho_old[0] = 0.0; ks->iters[0] = 0; ks->maxit = maxit;
@@ -1097,14 +1165,14 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const dou
         ks->conv[0] = ks->done[0];
         ks->all_done = ks->done[0];
     }
-}
+};
 
 // p_new = z + beta p_old ; q = A p_new ; alpha = rho / (p_new, q).  The
 // product reads z + beta p_old at the neighbours, so the p update needs no
 // pass of its own (p_old and p_new are separate buffers).
 template <int ND, typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_pq(Mat<T> A, const T *z, const T *pold, T *pnew, T *q, KState *ks,
-                                                  double *partials, unsigned *cnt, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_pcg_pq(Mat<T> A, const T *z, const T *pold, T *pnew, T *q,
+                                                  const KState *ks, double *partials, int chunk)
 {
     if (ks->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
@@ -1118,12 +1186,20 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_pq(Mat<T> A, const T *z, const T 
                         q[r] = (T)ax[0];
                         dsum[0] += (double)p * ax[0];
                     });
-    if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) ks->alpha[0] = ks->rho[0] / dsum[0];
+    block_partials<1>(dsum, partials, bb, gridDim.x);
 }
 
+struct EpiPcgAlpha {
+    KState *ks;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        if (!ks->all_done) ks->alpha[0] = ks->rho[0] / v[0];
+    }
+};
+
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, T *x, T *r, const T *p, const T *q, KState *ks,
-                                                  double *partials, unsigned *cnt, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, T *x, T *r, const T *p, const T *q, const KState *ks,
+                                                  double *partials, int chunk)
 {
     if (ks->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
@@ -1136,15 +1212,22 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_xr(int n, T *x, T *r, const T *p,
         r[i] = rr;
         acc[0] += (double)rr * (double)rr;
     }
-    if (block_reduce_last<1>(acc, partials, cnt, bb, gridDim.x)) {
-        const double rn = sqrt(acc[0]);
+    block_partials<1>(acc, partials, bb, gridDim.x);
+}
+
+struct EpiPcgXr {
+    KState *ks;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        if (ks->all_done) return;
+        const double rn = sqrt(v[0]);
         ks->rnorm[0] = rn;
         ks->iters[0] += 1;
         if (rn <= ks->tol[0]) { ks->done[0] = 1; ks->conv[0] = 1; }
         else if (ks->iters[0] >= ks->maxit || !(rn == rn)) ks->done[0] = 1;
         ks->all_done = ks->done[0];
     }
-}
+};
 
 template <typename T>
 static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit, KState *ks,
@@ -1157,16 +1240,15 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
     if (project) {
         double *dm = c->dscal + 600;
         if constexpr (sizeof(T) == 8) {
-            hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)b, dm, c->red,
-                               c->red_cnt + 10, g.chunk, 1.0 / (double)n);
+            sum_scaled(c, (const double *)b, n, dm, 1.0 / (double)n);
             mean = dm;
         }
     }
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, ks, c->red, c->red_cnt,
-                       g.chunk, rtol, atol, maxit);
+    hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, c->red, g.chunk);
+    launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgInit{ks, rtol, atol, maxit});
     MC_LAUNCH_CHECK(c);
-    const double vbytes = A.nd ? (double)sizeof(T) * c->nnz + 3.0 * sizeof(T) * n
-                               : (4.0 + sizeof(T)) * c->nnz + (4.0 + 3 * sizeof(T)) * n;
+    // A; z and p_old read, p and q written
+    const double vbytes = mat_bytes(A, c->nnz, sizeof(T)) + 4.0 * sizeof(T) * n;
     T *pa = w.p, *pb = w.p2;
     int it = 0, batch = 2;
     double rn_prev = -1.0;
@@ -1179,10 +1261,12 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
             {
                 KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32, vbytes);
                 MC_DISPATCH_ND(A.nd, (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb,
-                               w.q, ks, c->red, c->red_cnt, g.chunk);
+                               w.q, ks, c->red, g.chunk);
             }
+            launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgAlpha{ks});
             hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w.x, w.r, pb, w.q, ks,
-                               c->red, c->red_cnt, g.chunk);
+                               c->red, g.chunk);
+            launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgXr{ks});
             std::swap(pa, pb);
         }
         MC_LAUNCH_CHECK(c);
@@ -1206,10 +1290,8 @@ __global__ void k_sub_mean(int n, double *x, const double *mean)
 static int project_mean_zero(Ctx *c, double *x)
 {
     const int n = c->N;
-    const Grid g = grid_for(n);
     double *dm = c->dscal + 610;
-    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, (const double *)x, dm, c->red,
-                       c->red_cnt + 11, g.chunk, 1.0 / (double)n);
+    sum_scaled(c, x, n, dm, 1.0 / (double)n);
     hipLaunchKernelGGL(k_sub_mean, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, dm);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -1238,7 +1320,7 @@ int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol,
 // ---------------------------------------------- mixed: FP32 inside FP64 IR --
 template <int ND>
 __global__ void __launch_bounds__(BLOCK) k_ir_resid(Mat<double> A, const double *b, const double *mean, const double *x,
-                                                    double *r, double *partials, unsigned *cnt, int chunk, double *out)
+                                                    double *r, double *partials, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
@@ -1249,8 +1331,13 @@ __global__ void __launch_bounds__(BLOCK) k_ir_resid(Mat<double> A, const double 
         r[row] = rr;
         dsum[0] += rr * rr;
     });
-    if (block_reduce_last<1>(dsum, partials, cnt, bb, gridDim.x)) *out = sqrt(dsum[0]);
+    block_partials<1>(dsum, partials, bb, gridDim.x);
 }
+
+struct EpiSqrt {
+    double *out;
+    __device__ void operator()(const double (&v)[1]) const { *out = sqrt(v[0]); }
+};
 
 __global__ void k_to32_scaled(int n, const double *r, const double *nrm, float *r32)
 {
@@ -1285,20 +1372,20 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     float *r32 = c->wk32[0], *e32 = c->wk32[1];
     PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5], c->wk32[6]};
     const Mat<double> A = level_mat<double>(c->amg[0]);
-    hipLaunchKernelGGL(k_sum, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, c->red, c->red_cnt + 12, g.chunk,
-                       1.0 / (double)n);
+    sum_scaled(c, b, n, mean, 1.0 / (double)n);
     hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
     double bnorm = -1.0, rnorm = 0.0;
     int total = 0, outer = 0;
     bool conv = false;
     const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
-    const double rbytes = A.nd ? 8.0 * c->nnz + 32.0 * n : 12.0 * c->nnz + 36.0 * n;
+    const double rbytes = mat_bytes(A, c->nnz, 8) + 24.0 * n;   // A; b, x read, r written
     for (outer = 0; outer < 50; outer++) {
         {
-            KTimer kt(c, MC_KC_SPMV64, rbytes);
+            KTimer kt(c, MC_KC_IR_RESID, rbytes);
             MC_DISPATCH_ND(A.nd, (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
-                           c->red_cnt + 13, g.chunk, rn);
+                           g.chunk);
         }
+        launch_finish<1>(c->stream, c->red, g.blocks, EpiSqrt{rn});
         MC_LAUNCH_CHECK(c);
         MC_HIP(c, hipMemcpyAsync(c->hscal + 621, rn, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         MC_HIP(c, hipStreamSynchronize(c->stream));
@@ -1354,6 +1441,52 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
         return MC_ERR_NOCONV;
     }
     return MC_OK;
+}
+
+// ------------------------------------------------------ copy probe --
+// Plain streaming copy, 16 B per lane per access (grid-stride): the known-size
+// kernel for HBM bandwidth and counter calibration.
+__global__ void __launch_bounds__(256) k_copy_probe(int64_t n16, const double2 *__restrict__ src,
+                                                   double2 *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out)
+{
+    void *src = nullptr, *dst = nullptr;
+    MC_HIP(c, hipMalloc(&src, (size_t)bytes));
+    if (hipMalloc(&dst, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(src);
+        c->err = "copy probe: hipMalloc failed";
+        return MC_ERR_NOMEM;
+    }
+    int rc = MC_OK;
+    hipEvent_t a = nullptr, b = nullptr;
+    const int64_t n16 = bytes / 16;
+    const int blocks = (int)std::min<int64_t>(div_up(n16, 256), 256 * 64);
+    if (hipMemsetAsync(src, 0, (size_t)bytes, c->stream) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
+        hipEventCreate(&b) != hipSuccess) {
+        rc = MC_ERR_HIP;
+    } else {
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const double2 *)src,
+                           (double2 *)dst);   // untimed first touch
+        (void)hipEventRecord(a, c->stream);
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const double2 *)src,
+                               (double2 *)dst);
+        (void)hipEventRecord(b, c->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = MC_ERR_HIP;
+        *ms_out = (double)ms / reps;
+    }
+    if (rc) c->err = "copy probe: HIP error";
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return rc;
 }
 
 }  // namespace mc

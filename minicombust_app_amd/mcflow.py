@@ -28,7 +28,13 @@ ARR_FF0, ARR_FF1, ARR_MFLUX = 40, 41, 42
 ARR_ROW_PTR, ARR_COL, ARR_A_VALS, ARR_PA_VALS, ARR_PB = range(64, 69)
 ARR_FACES, ARR_CELL_FACES, ARR_CELL_NEIGHBOURS, ARR_CELLS, ARR_POINTS, ARR_CELLS_PER_POINT = range(72, 78)
 
-KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "spmv64", "spmv32", "vcycle", "correct", "mpnn"]
+KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "pcg_spmv64", "pcg_spmv32", "vcycle",
+            "flux_correct", "mpnn_fwd", "jac64", "jac32", "bicg_spmv", "scalar_batch", "ir_resid", "mpnn_bwd"]
+# kernel symbol (as a profiler lists it) of each single-kernel class
+KC_KERNELS = {"grad11": "mc::k_grad<11>", "grad1": "mc::k_grad<1>", "mom_asm": "mc::k_momentum",
+              "p_asm": "mc::k_mass_pressure", "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<7, double>",
+              "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
+              "scalar_batch": "mc::k_scalar_batch", "ir_resid": "mc::k_ir_resid<7>"}
 KC_COUNT = len(KC_NAMES)
 
 
@@ -110,6 +116,7 @@ def lib():
         L.mc_set_state_flags.argtypes = [P_, C.c_int, C.c_int64]
         L.mc_profile_enable.argtypes = [P_, C.c_int]
         L.mc_profile_read.argtypes = [P_, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mc_copy_probe.argtypes = [P_, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.mc_mpnn_load.argtypes = [P_, C.c_void_p, C.c_int64, C.c_int, C.c_int]
         L.mc_mpnn_infer.argtypes = [P_, C.c_void_p, C.c_void_p]
         _lib = L
@@ -307,8 +314,21 @@ class Context:
         self._check(self.L.mc_set_state_flags(self.h, int(first_pressure), int(timestep_count)), "flags")
 
     # ---- profiling ----
-    def profile(self, on=True):
-        self._check(self.L.mc_profile_enable(self.h, 1 if on else 0), "mc_profile_enable")
+    def profile(self, on=True, classes=None):
+        """Time kernel classes with HIP events: all classes, or those named."""
+        if not on:
+            mask = 0
+        elif classes is None:
+            mask = (1 << KC_COUNT) - 1
+        else:
+            mask = sum(1 << KC_NAMES.index(k) for k in classes)
+        self._check(self.L.mc_profile_enable(self.h, mask), "mc_profile_enable")
+
+    def copy_probe(self, nbytes, reps=10):
+        """Average ms of a plain device copy of `nbytes` (read + write)."""
+        ms = C.c_double(0.0)
+        self._check(self.L.mc_copy_probe(self.h, int(nbytes), int(reps), C.byref(ms)), "mc_copy_probe")
+        return ms.value
 
     def profile_reset(self):
         self._check(self.L.mc_profile_reset(self.h), "mc_profile_reset")
