@@ -81,6 +81,8 @@ struct Ctx {
     int *fc0 = nullptr, *fc1 = nullptr;     // [F] face cells; boundary: fc1 = N + side
     int *cell_faces = nullptr;               // [6N] side order (FRONT..UP)
     int *forder = nullptr;                   // [6N] the cell's faces in ascending face id
+    int *fother = nullptr;                   // [6N] other cell of each forder face (boundary: N + side),
+                                             //      bit 31 set when the cell is the face's cell1
     int *nbr = nullptr;                      // [6N] neighbour per side, -1 at the boundary
     int *bfaces = nullptr;                   // [Fb] boundary faces, ascending
     int btype[6] = {0, 0, 0, 0, 0, 0};

@@ -262,6 +262,17 @@ __global__ void k_row_fill(int N, const int *nbr, const int *cell_faces, const i
     for (int s = 0; s < 6; s++) forder[6 * c + s] = fo[s];
 }
 
+// fother[6c + k]: the cell across face forder[6c + k] (boundary faces:
+// N + side), with bit 31 set when c is that face's cell1.
+__global__ void k_face_other(int N, const int *forder, const int *fc0, const int *fc1, int *fother)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 6 * N) return;
+    const int c = i / 6, f = forder[i];
+    const int c0 = fc0[f], c1 = fc1[f];
+    fother[i] = (c0 == c) ? c1 : (int)((unsigned)c0 | 0x80000000u);
+}
+
 __global__ void k_boundary_flags(int F, int N, const int *fc1, uint8_t *flag, int *last_face)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,6 +481,9 @@ int mesh_finish(Ctx *c)
     MC_HIP(c, hipMemsetAsync(c->PA_vals, 0, (size_t)nnz * 8, c->stream));
     hipLaunchKernelGGL(k_row_fill, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->nbr, c->cell_faces, c->row_ptr,
                        c->col, c->diag_idx, c->forder);
+    MC_TRY(dalloc(c, &c->fother, 6 * (int64_t)N));
+    hipLaunchKernelGGL(k_face_other, dim3(div_up(6 * (int64_t)N, 256)), dim3(256), 0, c->stream, N, c->forder, c->fc0,
+                       c->fc1, c->fother);
     MC_LAUNCH_CHECK(c);
 
     // boundary faces, last face per side, wall cells

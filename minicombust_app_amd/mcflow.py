@@ -2,8 +2,9 @@
 
 This is the Python-side FFI of the C-ABI, the same binding a ctypes user of
 the library would write.  It loads the in-tree build
-``minicombust_app_amd/_lib/libmcflow.so`` and raises if that file is missing:
-there is no fallback path.
+``minicombust_app_amd/_lib/libmcflow.so`` (or the file named by the MCFLOW_LIB
+environment variable) and raises if that file is missing: there is no
+fallback path.
 """
 import ctypes as C
 import os
@@ -11,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libmcflow.so")
+LIB_PATH = os.environ.get("MCFLOW_LIB") or os.path.join(HERE, "_lib", "libmcflow.so")
 
 MC_OK, MC_ERR_ARG, MC_ERR_HIP, MC_ERR_NOMEM, MC_ERR_STATE, MC_ERR_NOCONV, MC_ERR_NODEV = range(7)
 (U, V, W, P, PP, TE, ED, TP, TEM, FUL, PRO, VARF, VARP) = range(13)

@@ -119,6 +119,9 @@ def lib():
                                        _DP]
         L.mco_bicgstab.restype = C.c_int
         L.mco_bicgstab.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP, C.c_double, C.c_int, _DP]
+        L.mco_pcg_amg.restype = C.c_int
+        L.mco_pcg_amg.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP, C.c_double, C.c_double, C.c_int, _DP, _DP]
+        L.mco_spmv.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP]
         _lib = L
     return _lib
 

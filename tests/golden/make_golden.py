@@ -1,0 +1,34 @@
+"""Writes tests/golden/ref_mesh_m{1,2,3}.npz: the box meshes produced by the
+reference's own mesh generator (src/examples/mesh_examples.cpp:120-571, built
+from its sources by `make -C oracle ref` into oracle/_ref/ref_mesh_dump), so
+the oracle's mesh is pinned by the reference even where the reference tree is
+absent.  Run from the repository root:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import pyoracle as po  # noqa: E402
+
+
+def main():
+    if not os.path.exists(po.REF_MESH_DUMP):
+        raise SystemExit("oracle/_ref/ref_mesh_dump missing: run `make -C oracle ref` first")
+    with tempfile.TemporaryDirectory() as d:
+        for m in (1, 2, 3):
+            mesh = po.ref_mesh(m, d)
+            arrays = {k: np.ascontiguousarray(v) for k, v in mesh.items() if k != "m"}
+            out = os.path.join(HERE, "ref_mesh_m%d.npz" % m)
+            np.savez_compressed(out, **arrays)
+            print("wrote", out, {k: v.shape for k, v in arrays.items()})
+
+
+if __name__ == "__main__":
+    main()
