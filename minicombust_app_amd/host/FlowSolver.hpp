@@ -1,0 +1,143 @@
+// FlowSolver.hpp -- C++ host side of the MI355X flow solver, keeping the call
+// surface of the reference's FlowSolver<T> (include/flow/FlowSolver.hpp:8-1049,
+// include/flow/gpu/FlowSolver.hpp:45-1489) as src/minicombust.cpp uses it:
+//
+//   FlowSolver<double>(MPI_Config*, Mesh<double>*, double delta, FILE*)  (minicombust.cpp:174)
+//   timestep()                                                          (:221)
+//   output_data(t)                                                      (:226)
+//   print_logger_stats(ntimesteps, program_time)                        (:249)
+//   public compute_time                                                 (:277-282)
+//
+// Everything below the class goes through the C-ABI of include/mcflow.h (one
+// context = one GPU).  One flow rank owns the whole mesh, so the MPI_Config
+// argument has no counterpart; the mesh is either the reference's box
+// (modifier m: 2m x m x m cells, generated on the device) or host arrays in
+// the layout of the reference Mesh<T> (mc_mesh_desc).  As in the reference
+// (gpuErrchk, include/flow/gpu/gpu_kernels.cu:24-32), a failed call prints the
+// error and exits.
+#pragma once
+
+#include <chrono>
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "mcflow.h"
+
+namespace minicombust_amd {
+
+template <typename T>
+class FlowSolver {
+    static_assert(sizeof(T) == 8, "the flow solver computes in double precision");
+
+  public:
+    double compute_time = 0.0;   // seconds inside timestep(), as the reference's
+
+    // box mesh of the given modifier (src/minicombust.cpp:123-124)
+    FlowSolver(uint64_t modifier, double delta, FILE *output_file, const mc_options *opts = nullptr)
+        : out_(output_file ? output_file : stdout)
+    {
+        mc_options o = options(delta, opts);
+        check(mc_ctx_create_box((int64_t)modifier, &o, &ctx_), "mc_ctx_create_box");
+    }
+
+    // a host mesh in the reference Mesh<T> layout (include/geometry/Mesh.hpp:70-107)
+    FlowSolver(const mc_mesh_desc &mesh, double delta, FILE *output_file, const mc_options *opts = nullptr)
+        : out_(output_file ? output_file : stdout)
+    {
+        mc_options o = options(delta, opts);
+        check(mc_ctx_create(&mesh, &o, &ctx_), "mc_ctx_create");
+    }
+
+    ~FlowSolver()
+    {
+        if (ctx_) mc_ctx_destroy(ctx_);
+    }
+
+    FlowSolver(const FlowSolver &) = delete;
+    FlowSolver &operator=(const FlowSolver &) = delete;
+
+    // FlowSolver<T>::timestep() (FlowSolver.inl:3148-3356): three SIMPLE passes
+    void timestep()
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        check(mc_timestep(ctx_), "mc_timestep");
+        check(mc_sync(ctx_), "mc_sync");
+        compute_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        timesteps_++;
+    }
+
+    // FlowSolver<T>::output_data (FlowSolver.inl:18): the reference writes
+    // VTK files; this build reports per-field range and mean (VTK output is
+    // outside the accelerated path).
+    void output_data(uint64_t timestep)
+    {
+        static const char *names[] = {"U", "V", "W", "P", "PP", "TE", "ED", "TP", "TEM", "FUL", "PRO", "VARF", "VARP"};
+        fprintf(out_, "Flow fields at timestep %" PRIu64 ":\n", timestep);
+        for (int f : {MC_U, MC_V, MC_W, MC_P, MC_TE, MC_ED, MC_TEM, MC_FUL, MC_PRO, MC_VARF, MC_VARP}) {
+            const std::vector<double> v = field(f);
+            double lo = INFINITY, hi = -INFINITY, sum = 0.0;
+            for (size_t i = 0; i < n_cells_; i++) { lo = std::fmin(lo, v[i]); hi = std::fmax(hi, v[i]); sum += v[i]; }
+            fprintf(out_, "\t%-5s min % .6e  max % .6e  mean % .6e\n", names[f], lo, hi, sum / (double)n_cells_);
+        }
+    }
+
+    // FlowSolver<T>::print_logger_stats (FlowSolver.inl:3079): solver work and time
+    void print_logger_stats(uint64_t timesteps, double runtime)
+    {
+        mc_stats s{};
+        check(mc_get_stats(ctx_, &s), "mc_get_stats");
+        fprintf(out_, "Flow Solver Stats:\n");
+        fprintf(out_, "\tTimesteps:                 %" PRIu64 "\n", timesteps);
+        fprintf(out_, "\tSIMPLE iterations:         %" PRId64 "\n", s.simple_iterations);
+        fprintf(out_, "\tMomentum BiCGStab iters:   %" PRId64 "\n", s.mom_iters);
+        fprintf(out_, "\tScalar BiCGStab iters:     %" PRId64 "\n", s.scalar_iters);
+        fprintf(out_, "\tPressure solves / iters:   %" PRId64 " / %" PRId64 "\n", s.p_solves, s.p_iters);
+        fprintf(out_, "\tPressure solve time:       %.6fs\n", s.p_solve_ms * 1e-3);
+        fprintf(out_, "\tMax pressure rel. residual %.3e\n", s.p_relres_max);
+        fprintf(out_, "\tFlow Compute Time:         %.6fs\n", compute_time);
+        fprintf(out_, "\tProgram time:              %.6fs\n", runtime);
+    }
+
+    // host copy of a field's N cell values (field ids: include/mcflow.h)
+    std::vector<double> field(int f)
+    {
+        std::vector<double> v(n_cells_ + 6);
+        check(mc_array_get(ctx_, MC_ARR_PHI + f, v.data(), (int64_t)v.size()), "mc_array_get");
+        v.resize(n_cells_);
+        return v;
+    }
+
+    uint64_t cells() const { return n_cells_; }
+    mc_ctx *context() { return ctx_; }
+
+  private:
+    mc_ctx *ctx_ = nullptr;
+    FILE *out_;
+    uint64_t n_cells_ = 0;
+    uint64_t timesteps_ = 0;
+
+    mc_options options(double delta, const mc_options *opts)
+    {
+        mc_options o;
+        if (opts) o = *opts; else mc_options_default(&o);
+        o.delta = delta;
+        return o;
+    }
+
+    void check(int rc, const char *what)
+    {
+        if (rc != MC_OK) {
+            fprintf(stderr, "mcflow error %d in %s: %s\n", rc, what, ctx_ ? mc_last_error(ctx_) : "");
+            exit(EXIT_FAILURE);
+        }
+        if (!n_cells_ && ctx_) {
+            int64_t sizes[4] = {0, 0, 0, 0};
+            if (mc_ctx_sizes(ctx_, &sizes[0], &sizes[1], &sizes[2], &sizes[3]) == MC_OK) n_cells_ = (uint64_t)sizes[0];
+        }
+    }
+};
+
+}  // namespace minicombust_amd
