@@ -75,6 +75,8 @@ typedef struct mc_options {
     int    amg_coarse_max; /* dense coarsest level size bound                      */
     double mixed_inner_rtol; /* FP32 inner solve reduction per refinement step     */
     int    check_every;    /* Krylov iterations between host convergence checks    */
+    double inlet_scale;    /* multiplies the reference gas velocity (Mesh.hpp:114,
+                              inlet and initial field); 1 = the reference case    */
 } mc_options;
 
 typedef struct mc_solve_status {
@@ -99,6 +101,7 @@ typedef struct mc_stats {
     int64_t clip_cells;       /* TE/ED cells repaired to non-negative values     */
     int64_t clip_passes;      /* relaxation passes of that repair                */
     int64_t host_polls;       /* host waits on solver convergence flags          */
+    int64_t p_guesses;        /* pressure solves started from the learned guess  */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>
@@ -169,6 +172,11 @@ int  mc_ctx_destroy(mc_ctx *ctx);
 const char *mc_last_error(mc_ctx *ctx);
 int  mc_ctx_sizes(mc_ctx *ctx, int64_t *n_cells, int64_t *n_faces, int64_t *nnz, int64_t *n_points);
 int  mc_set_options(mc_ctx *ctx, const mc_options *opt);
+/* Back to the constructor's state (FlowSolver.hpp:301-539): initial fields and
+   boundary slots from the current options (inlet_scale), zero mass fluxes and
+   matrices, first-call flags set; mesh, geometry and AMG hierarchy are kept.
+   No counterpart in the reference (which builds a new FlowSolver). */
+int  mc_reset_fields(mc_ctx *ctx);
 
 /* ---- SIMPLE step ---------------------------------------------------------
  * mc_timestep        FlowSolver<T>::timestep() (FlowSolver.inl:3148-3356):
@@ -251,9 +259,33 @@ int  mc_profile_reset(mc_ctx *ctx);
    reference: measurement only. */
 int  mc_copy_probe(mc_ctx *ctx, int64_t bytes, int reps, double *ms_out);
 
-/* ---- learned pressure-correction initial guess -------------------------- */
-int  mc_mpnn_load(mc_ctx *ctx, const float *weights, int64_t count, int hidden, int layers);
-int  mc_mpnn_infer(mc_ctx *ctx, const double *rhs_dev_or_null, double *x_host_or_null);
+/* ---- learned pressure-correction initial guess -------------------------- *
+ * No counterpart in the reference: its pressure solves start from zero
+ * (FlowSolver.inl:2203-2273, gpu/FlowSolver.inl:943-969).  A message-passing
+ * net over the cell graph (3 layers, hidden 64, BF16 weights on MFMA with FP32
+ * accumulation; definition and parameter order in mc_mpnn.hip) maps the
+ * pressure system (RHS, diagonal, cell volume, net mass outflow; the
+ * off-diagonals as edge features) to a guess g.  With mc_options.learned = 1
+ * and weights loaded, each pressure solve starts from x0 = g (g,b)/(g,Ag),
+ * kept only when it lowers the residual (else x0 = 0, the plain solve);
+ * convergence is judged by the unchanged PCG-AMG residual test.           */
+int64_t mc_mpnn_nparams(void);
+int  mc_mpnn_load(mc_ctx *ctx, const float *weights, int64_t count);
+int  mc_mpnn_save(mc_ctx *ctx, float *weights, int64_t count);
+/* Collect (features, solution) pairs from the following pressure solves into
+   a device data set of max_pairs (0 = stop collecting).                    */
+int  mc_mpnn_collect(mc_ctx *ctx, int64_t max_pairs);
+int  mc_mpnn_dataset_size(mc_ctx *ctx, int64_t *pairs);
+/* Host copies of a stored pair: node features [N x 4], edge features [N x 6]
+   (side order of MC_ARR_CELL_NEIGHBOURS), normalised target [N].          */
+int  mc_mpnn_pair_get(mc_ctx *ctx, int64_t pair, float *feat, float *edge, float *target);
+/* Net output on a stored pair's features [N].                              */
+int  mc_mpnn_forward_pair(mc_ctx *ctx, int64_t pair, float *y_host);
+/* Loss (mean squared error of the mean-removed output against the target)
+   and its gradient for one stored pair [mc_mpnn_nparams()].                */
+int  mc_mpnn_grad(mc_ctx *ctx, int64_t pair, float *grad_host, double *loss);
+/* One Adam step (learning rate lr) on one stored pair; *loss before it.    */
+int  mc_mpnn_train_step(mc_ctx *ctx, int64_t pair, double lr, double *loss);
 
 #ifdef __cplusplus
 }

@@ -6,9 +6,6 @@
 
 #include "mc_internal.hpp"
 
-struct mc_ctx {
-    mc::Ctx c;
-};
 
 namespace mc {
 
@@ -91,7 +88,7 @@ static void set_constants(Ctx *c)
 {
     Consts &k = c->k;
     // Mesh.hpp:114-120
-    k.gas_vel_x = 50.; k.gas_vel_y = 0.00; k.gas_vel_z = 0.00;
+    k.gas_vel_x = 50. * c->opt.inlet_scale; k.gas_vel_y = 0.00; k.gas_vel_z = 0.00;
     k.gas_pre = 4000.0; k.gas_tem = 2000.0; k.gas_te = 0.0001; k.gas_ed = 0.0; k.gas_ful = 0.1; k.gas_pro = 0.001;
     // FlowSolver.hpp:296-297, 421-432 (host libm, the reference's expressions)
     k.ev = 0.000014;
@@ -162,6 +159,7 @@ int mc_options_default(mc_options *o)
     o->amg_coarse_max = 128;
     o->mixed_inner_rtol = 1.0e-3;
     o->check_every = 2;
+    o->inlet_scale = 1.0;
     return MC_OK;
 }
 
@@ -175,6 +173,7 @@ static int validate(const mc_options *o, std::string &err)
     }
     if (o->amg_coarse_max < 8 || o->amg_coarse_max > 128) { err = "amg_coarse_max must be in [8, 128]"; return MC_ERR_ARG; }
     if (o->amg_nu < 1 || o->amg_nu > 8) { err = "amg_nu must be in [1, 8]"; return MC_ERR_ARG; }
+    if (!(o->inlet_scale > 0.0)) { err = "inlet_scale must be positive"; return MC_ERR_ARG; }
     return MC_OK;
 }
 
@@ -252,8 +251,27 @@ int mc_set_options(mc_ctx *x, const mc_options *opt)
     MC_TRY(validate(opt, c->err));
     const bool amg_change = opt->amg_coarse_max != c->opt.amg_coarse_max;
     c->opt = *opt;
-    c->k.rdelta = 1.0 / c->opt.delta;
+    set_constants(c);
     if (amg_change) c->amg_ready = false;
+    return MC_OK;
+}
+
+int mc_reset_fields(mc_ctx *x)
+{
+    if (!x) return MC_ERR_ARG;
+    Ctx *c = &x->c;
+    MC_TRY(init_fields(c));
+    MC_HIP(c, hipMemsetAsync(c->mflux, 0, (size_t)c->F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->ff0, 0, (size_t)c->F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->ff1, 0, (size_t)c->F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->fcorr, 0, (size_t)c->F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->A_vals, 0, (size_t)c->nnz * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->PA_vals, 0, (size_t)c->nnz * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->Pb, 0, (size_t)c->N * 8, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    c->first_pressure = true;
+    c->field_set_up = false;
+    c->timestep_count = 0;
     return MC_OK;
 }
 

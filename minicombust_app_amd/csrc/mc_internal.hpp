@@ -104,6 +104,7 @@ struct Ctx {
 
     // ---- fields ----
     double *phi[NFIELDS] = {};
+    double **field_ptrs = nullptr;   // device table: phi, gx, gy, gz of every field
     double *gx[NFIELDS] = {}, *gy[NFIELDS] = {}, *gz[NFIELDS] = {};
     double *AU = nullptr, *AV = nullptr, *AW = nullptr;
     double *SU = nullptr, *SV = nullptr, *SW = nullptr;
@@ -305,7 +306,8 @@ void profile_release(Ctx *c);
 // stages (mc_mesh.hip)
 int mesh_box_generate(Ctx *c, int64_t m);
 int mesh_from_host(Ctx *c, const mc_mesh_desc *d);
-int mesh_finish(Ctx *c);   // geometry, init, CSR pattern, face orders
+int mesh_finish(Ctx *c);
+int init_fields(Ctx *c);   // constructor fields and boundary slots   // geometry, init, CSR pattern, face orders
 
 // stages (mc_flow.hip)
 int grad_multi(Ctx *c);                                  // 11 fields
@@ -333,15 +335,24 @@ int amg_setup(Ctx *c, const double *vals);
 int amg_update_values(Ctx *c, const double *vals);
 int spmv(Ctx *c, const double *vals, const double *x, double *y);
 int dia_setup_fine(Ctx *c);
+// x0 = true: x holds the initial guess (else the solve starts from zero);
+// bnorm0 = ||b - mean b|| when starting from a guess
 int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
-            double atol, mc_solve_status *st);
+            double atol, mc_solve_status *st, bool x0 = false);
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
-                   double atol, mc_solve_status *st);
+                   double atol, mc_solve_status *st, bool x0 = false, double bnorm0 = 0.0);
 int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host);
 int mean_project(Ctx *c, const double *b, double *out, int n);
 
 // learned initial guess (mc_mpnn.hip)
 int mpnn_guess(Ctx *c, const double *b, double *x0, int *ok);
+int mpnn_collect_pair(Ctx *c, const double *b, const double *x);   // data set, when collecting
 void mpnn_destroy(Ctx *c);
 
 }  // namespace mc
+
+// the opaque handle of the C-ABI
+struct mc_ctx {
+    mc::Ctx c;
+};
+

@@ -1135,33 +1135,55 @@ struct PcgVecs {
     T *x, *r, *z, *p, *q, *p2;
 };
 
+// r = b - mean, x = 0, p = 0; partials of (r, r) and (b - mean, b - mean)
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_pcg_init(int n, const T *b, const double *mean, PcgVecs<T> w,
                                                     double *partials, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
-    double acc[1] = {0.0};
+    double acc[2] = {0.0, 0.0};
     const double mu = mean ? *mean : 0.0;
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
         const T r = (T)((double)b[i] - mu);
         w.r[i] = r; w.x[i] = (T)0; w.p[i] = (T)0;
         acc[0] += (double)r * (double)r;
     }
-    block_partials<1>(acc, partials, bb, gridDim.x);
+    acc[1] = acc[0];
+    block_partials<2>(acc, partials, bb, gridDim.x);
 }
 
+// from a given x: r = (b - mean) - A x, p = 0; same partials
+template <int ND>
+__global__ void __launch_bounds__(BLOCK) k_pcg_init_x0(Mat<double> A, const double *b, const double *mean,
+                                                       PcgVecs<double> w, double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    double acc[2] = {0.0, 0.0};
+    const double mu = *mean;
+    const double *x = w.x;
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return x[j]; }, [&](int i, const double (&ax)[1]) {
+        const double bt = b[i] - mu, r = bt - ax[0];
+        w.r[i] = r; w.p[i] = 0.0;
+        acc[0] += r * r;
+        acc[1] += bt * bt;
+    });
+    block_partials<2>(acc, partials, bb, gridDim.x);
+}
+
+// tolerance from ||b - mean||, start from ||r||
 struct EpiPcgInit {
     KState *ks;
     double rtol, atol;
     int maxit;
-    __device__ void operator()(const double (&v)[1]) const
+    __device__ void operator()(const double (&v)[2]) const
     {
-        const double bn = sqrt(v[0]);
-        ks->bnorm[0] = bn; ks->rnorm[0] = bn;
+        const double rn = sqrt(v[0]), bn = sqrt(v[1]);
+        ks->bnorm[0] = bn; ks->rnorm[0] = rn;
         ks->tol[0] = fmax(rtol * bn, atol);
         ks->rho_old[0] = 0.0; ks->iters[0] = 0; ks->maxit = maxit;
-        ks->done[0] = (bn <= ks->tol[0]) ? 1 : 0;
+        ks->done[0] = (rn <= ks->tol[0]) ? 1 : 0;
         ks->conv[0] = ks->done[0];
         ks->all_done = ks->done[0];
     }
@@ -1231,7 +1253,7 @@ struct EpiPcgXr {
 
 template <typename T>
 static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit, KState *ks,
-                    KState *hs, int *iters_out)
+                    KState *hs, int *iters_out, bool x0 = false)
 {
     const int n = c->N;
     const Grid g = grid_for(n);
@@ -1244,8 +1266,18 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
             mean = dm;
         }
     }
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, c->red, g.chunk);
-    launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgInit{ks, rtol, atol, maxit});
+    if constexpr (sizeof(T) == 8) {
+        if (x0 && mean) {
+            MC_DISPATCH_ND(A.nd, (k_pcg_init_x0<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A,
+                           (const double *)b, mean, w, c->red, g.chunk);
+        } else {
+            hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, c->red,
+                               g.chunk);
+        }
+    } else {
+        hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, c->red, g.chunk);
+    }
+    launch_finish<2>(c->stream, c->red, g.blocks, EpiPcgInit{ks, rtol, atol, maxit});
     MC_LAUNCH_CHECK(c);
     // A; z and p_old read, p and q written
     const double vbytes = mat_bytes(A, c->nnz, sizeof(T)) + 4.0 * sizeof(T) * n;
@@ -1298,14 +1330,14 @@ static int project_mean_zero(Ctx *c, double *x)
 }
 
 int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
-            mc_solve_status *st)
+            mc_solve_status *st, bool x0)
 {
     (void)vals;  // the hierarchy's level 0 holds these values
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     PcgVecs<double> w{x, c->wk[0], c->wk[1], c->wk[2], c->wk[3], c->wk[5]};
     int iters = 0;
-    MC_TRY(pcg_core<double>(c, b, true, w, rtol, atol, maxit, ks, hs, &iters));
+    MC_TRY(pcg_core<double>(c, b, true, w, rtol, atol, maxit, ks, hs, &iters, x0));
     MC_TRY(project_mean_zero(c, x));
     if (st) {
         st->iterations = iters;
@@ -1360,7 +1392,7 @@ __global__ void k_zero(int n, double *x)
 }
 
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
-                   mc_solve_status *st)
+                   mc_solve_status *st, bool x0, double bnorm0)
 {
     (void)vals;
     const int n = c->N;
@@ -1373,8 +1405,8 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5], c->wk32[6]};
     const Mat<double> A = level_mat<double>(c->amg[0]);
     sum_scaled(c, b, n, mean, 1.0 / (double)n);
-    hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
-    double bnorm = -1.0, rnorm = 0.0;
+    if (!x0) hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
+    double bnorm = x0 ? bnorm0 : -1.0, rnorm = 0.0;
     int total = 0, outer = 0;
     bool conv = false;
     const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
@@ -1414,18 +1446,110 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     return MC_OK;
 }
 
+// ------------------------------------------------------ learned guess --
+// x0 = alpha g with g the net's mean-zero guess and alpha = (g, b) / (g, A g):
+// the A-norm-optimal multiple of g (a one-vector Galerkin / deflation step).
+// Kept only if it lowers the residual below ||b - mean b||; otherwise x0 = 0.
+__global__ void __launch_bounds__(BLOCK) k_guess_dots(int n, const double *b, const double *gv, const double *Ag,
+                                                      double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        acc[0] += b[i];
+        acc[1] += gv[i] * b[i];
+        acc[2] += gv[i] * Ag[i];
+    }
+    block_partials<3>(acc, partials, bb, gridDim.x);
+}
+struct EpiGuess1 {
+    double *sc;
+    double inv_n;
+    __device__ void operator()(const double (&v)[3]) const
+    {
+        sc[0] = v[0] * inv_n;                              // mean b
+        sc[1] = (v[2] > 0.0) ? v[1] / v[2] : 0.0;          // alpha
+    }
+};
+__global__ void __launch_bounds__(BLOCK) k_guess_res(int n, const double *b, const double *Ag, const double *sc,
+                                                     double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[2] = {0.0, 0.0};
+    const double mu = sc[0], al = sc[1];
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        const double bt = b[i] - mu, r = bt - al * Ag[i];
+        acc[0] += bt * bt;
+        acc[1] += r * r;
+    }
+    block_partials<2>(acc, partials, bb, gridDim.x);
+}
+struct EpiGuess2 {
+    double *sc;
+    __device__ void operator()(const double (&v)[2]) const
+    {
+        sc[2] = sqrt(v[0]);                                // ||b - mean||
+        sc[3] = sqrt(v[1]);                                // ||r0||
+        sc[4] = (sc[3] < sc[2]) ? 1.0 : 0.0;               // accept
+    }
+};
+__global__ void k_guess_set(int n, const double *gv, const double *sc, double *x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (sc[4] > 0.0) ? sc[1] * gv[i] : 0.0;
+}
+
+static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double *bnorm)
+{
+    *used = false;
+    *bnorm = 0.0;
+    if (!c->opt.learned || !c->mpnn) return MC_OK;
+    const int n = c->N;
+    double *gv = c->wk[6], *Ag = c->wk[7];
+    double *sc = c->dscal + 640;
+    int ok = 0;
+    MC_TRY(mpnn_guess(c, b, gv, &ok));
+    if (!ok) return MC_OK;
+    MC_TRY(project_mean_zero(c, gv));
+    const Grid g = grid_for(n);
+    const Mat<double> A = level_mat<double>(c->amg[0]);
+    MC_DISPATCH_ND(A.nd, (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, (const double *)gv, Ag, g.chunk);
+    hipLaunchKernelGGL(k_guess_dots, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, (const double *)gv,
+                       (const double *)Ag, c->red, g.chunk);
+    launch_finish<3>(c->stream, c->red, g.blocks, EpiGuess1{sc, 1.0 / n});
+    hipLaunchKernelGGL(k_guess_res, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, (const double *)Ag,
+                       (const double *)sc, c->red, g.chunk);
+    launch_finish<2>(c->stream, c->red, g.blocks, EpiGuess2{sc});
+    hipLaunchKernelGGL(k_guess_set, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, (const double *)gv,
+                       (const double *)sc, x);
+    MC_LAUNCH_CHECK(c);
+    MC_HIP(c, hipMemcpyAsync(c->hscal + 640, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    *used = c->hscal[644] > 0.0;
+    *bnorm = c->hscal[642];
+    return MC_OK;
+}
+
 int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol)
 {
     const auto t0 = std::chrono::steady_clock::now();
     mc_solve_status s{};
     int rc;
+    bool used = false;
+    double bn0 = 0.0;
+    MC_TRY(learned_guess(c, b, x, &used, &bn0));
     if (c->opt.psolve == MC_PSOLVE_MIXED)
-        rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s);
+        rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used, bn0);
     else
-        rc = pcg_amg(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s);
+        rc = pcg_amg(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used);
+    s.guess_used = used ? 1 : 0;
     const auto t1 = std::chrono::steady_clock::now();
     c->st.p_solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (rc) return rc;
+    c->st.p_guesses += used ? 1 : 0;
+    if (c->mpnn) MC_TRY(mpnn_collect_pair(c, b, x));
     c->st.p_iters += s.iterations;
     c->st.p_solves += 1;
     c->st.p_outer += s.outer;

@@ -371,48 +371,14 @@ int mesh_from_host(Ctx *c, const mc_mesh_desc *d)
     return MC_OK;
 }
 
-int mesh_finish(Ctx *c)
+// Initial fields and boundary slots of the FlowSolver constructor
+// (FlowSolver.hpp:301-485, values Mesh.hpp:114-120); used at construction and
+// by mc_reset_fields.
+int init_fields(Ctx *c)
 {
-    const int N = c->N, F = c->F;
-    const int NS = N + 6;
-    int *d_btype = nullptr;
-    MC_TRY(dalloc(c, &d_btype, 6));
-    MC_HIP(c, hipMemcpyAsync(d_btype, c->btype, sizeof(c->btype), hipMemcpyHostToDevice, c->stream));
-    // geometry
-    MC_TRY(dalloc(c, &c->cx, N)); MC_TRY(dalloc(c, &c->cy, N)); MC_TRY(dalloc(c, &c->cz, N));
-    MC_TRY(dalloc(c, &c->vol, N)); MC_TRY(dalloc(c, &c->rho, N));
-    MC_TRY(dalloc(c, &c->fcx, F)); MC_TRY(dalloc(c, &c->fcy, F)); MC_TRY(dalloc(c, &c->fcz, F));
-    MC_TRY(dalloc(c, &c->fnx, F)); MC_TRY(dalloc(c, &c->fny, F)); MC_TRY(dalloc(c, &c->fnz, F));
-    MC_TRY(dalloc(c, &c->farea, F)); MC_TRY(dalloc(c, &c->flam, F)); MC_TRY(dalloc(c, &c->frlc, F));
-    MC_TRY(dalloc(c, &c->ff0, F)); MC_TRY(dalloc(c, &c->ff1, F)); MC_TRY(dalloc(c, &c->mflux, F));
-    MC_TRY(dalloc(c, &c->fcorr, F));
-    hipLaunchKernelGGL(k_cell_geom, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->cells, c->px, c->py, c->pz,
-                       c->cx, c->cy, c->cz, c->vol, c->rho);
-    MC_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_face_geom, dim3(div_up(F, 256)), dim3(256), 0, c->stream, F, N, c->fc0, c->fc1, c->cells,
-                       c->px, c->py, c->pz, c->cx, c->cy, c->cz, d_btype, c->farea, c->flam, c->frlc, c->fnx, c->fny,
-                       c->fnz, c->fcx, c->fcy, c->fcz, c->mflux);
-    MC_LAUNCH_CHECK(c);
-    MC_HIP(c, hipMemsetAsync(c->ff0, 0, (size_t)F * 8, c->stream));
-    MC_HIP(c, hipMemsetAsync(c->ff1, 0, (size_t)F * 8, c->stream));
-
-    // fields
-    for (int f = 0; f < NFIELDS; f++) {
-        MC_TRY(dalloc(c, &c->phi[f], NS));
-        MC_TRY(dalloc(c, &c->gx[f], NS)); MC_TRY(dalloc(c, &c->gy[f], NS)); MC_TRY(dalloc(c, &c->gz[f], NS));
-    }
-    MC_TRY(dalloc(c, &c->AU, NS)); MC_TRY(dalloc(c, &c->AV, NS)); MC_TRY(dalloc(c, &c->AW, NS));
-    MC_TRY(dalloc(c, &c->SU, NS)); MC_TRY(dalloc(c, &c->SV, NS)); MC_TRY(dalloc(c, &c->SW, NS));
-    double **dptr = nullptr;
-    MC_TRY(dalloc(c, &dptr, 4 * NFIELDS));
-    {
-        double *hp[4 * NFIELDS];
-        for (int f = 0; f < NFIELDS; f++) {
-            hp[f] = c->phi[f]; hp[NFIELDS + f] = c->gx[f]; hp[2 * NFIELDS + f] = c->gy[f]; hp[3 * NFIELDS + f] = c->gz[f];
-        }
-        MC_HIP(c, hipMemcpyAsync(dptr, hp, sizeof(hp), hipMemcpyHostToDevice, c->stream));
-        MC_HIP(c, hipStreamSynchronize(c->stream));
-    }
+    const int N = c->N;
+    double *const *dptr = c->field_ptrs;
+    MC_HIP(c, hipStreamSynchronize(c->stream));   // nothing in flight touches the fields
     const Consts &k = c->k;
     InitVals iv;
     iv.v[U] = k.gas_vel_x; iv.v[V] = k.gas_vel_y; iv.v[W] = k.gas_vel_z; iv.v[P] = k.gas_pre; iv.v[PP] = 0.0;
@@ -451,6 +417,52 @@ int mesh_finish(Ctx *c)
         for (double *a : {c->AU, c->AV, c->AW, c->SU, c->SV, c->SW})
             MC_HIP(c, hipMemcpy(a + N, zeros, 6 * sizeof(double), hipMemcpyHostToDevice));
     }
+
+    return MC_OK;
+}
+
+int mesh_finish(Ctx *c)
+{
+    const int N = c->N, F = c->F;
+    const int NS = N + 6;
+    int *d_btype = nullptr;
+    MC_TRY(dalloc(c, &d_btype, 6));
+    MC_HIP(c, hipMemcpyAsync(d_btype, c->btype, sizeof(c->btype), hipMemcpyHostToDevice, c->stream));
+    // geometry
+    MC_TRY(dalloc(c, &c->cx, N)); MC_TRY(dalloc(c, &c->cy, N)); MC_TRY(dalloc(c, &c->cz, N));
+    MC_TRY(dalloc(c, &c->vol, N)); MC_TRY(dalloc(c, &c->rho, N));
+    MC_TRY(dalloc(c, &c->fcx, F)); MC_TRY(dalloc(c, &c->fcy, F)); MC_TRY(dalloc(c, &c->fcz, F));
+    MC_TRY(dalloc(c, &c->fnx, F)); MC_TRY(dalloc(c, &c->fny, F)); MC_TRY(dalloc(c, &c->fnz, F));
+    MC_TRY(dalloc(c, &c->farea, F)); MC_TRY(dalloc(c, &c->flam, F)); MC_TRY(dalloc(c, &c->frlc, F));
+    MC_TRY(dalloc(c, &c->ff0, F)); MC_TRY(dalloc(c, &c->ff1, F)); MC_TRY(dalloc(c, &c->mflux, F));
+    MC_TRY(dalloc(c, &c->fcorr, F));
+    hipLaunchKernelGGL(k_cell_geom, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->cells, c->px, c->py, c->pz,
+                       c->cx, c->cy, c->cz, c->vol, c->rho);
+    MC_LAUNCH_CHECK(c);
+    hipLaunchKernelGGL(k_face_geom, dim3(div_up(F, 256)), dim3(256), 0, c->stream, F, N, c->fc0, c->fc1, c->cells,
+                       c->px, c->py, c->pz, c->cx, c->cy, c->cz, d_btype, c->farea, c->flam, c->frlc, c->fnx, c->fny,
+                       c->fnz, c->fcx, c->fcy, c->fcz, c->mflux);
+    MC_LAUNCH_CHECK(c);
+    MC_HIP(c, hipMemsetAsync(c->ff0, 0, (size_t)F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->ff1, 0, (size_t)F * 8, c->stream));
+
+    // fields
+    for (int f = 0; f < NFIELDS; f++) {
+        MC_TRY(dalloc(c, &c->phi[f], NS));
+        MC_TRY(dalloc(c, &c->gx[f], NS)); MC_TRY(dalloc(c, &c->gy[f], NS)); MC_TRY(dalloc(c, &c->gz[f], NS));
+    }
+    MC_TRY(dalloc(c, &c->AU, NS)); MC_TRY(dalloc(c, &c->AV, NS)); MC_TRY(dalloc(c, &c->AW, NS));
+    MC_TRY(dalloc(c, &c->SU, NS)); MC_TRY(dalloc(c, &c->SV, NS)); MC_TRY(dalloc(c, &c->SW, NS));
+    MC_TRY(dalloc(c, &c->field_ptrs, 4 * NFIELDS));
+    {
+        double *hp[4 * NFIELDS];
+        for (int f = 0; f < NFIELDS; f++) {
+            hp[f] = c->phi[f]; hp[NFIELDS + f] = c->gx[f]; hp[2 * NFIELDS + f] = c->gy[f]; hp[3 * NFIELDS + f] = c->gz[f];
+        }
+        MC_HIP(c, hipMemcpyAsync(c->field_ptrs, hp, sizeof(hp), hipMemcpyHostToDevice, c->stream));
+        MC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    MC_TRY(init_fields(c));
 
     // CSR pattern, face orders
     int *cnt = nullptr;

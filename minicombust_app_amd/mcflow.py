@@ -45,7 +45,7 @@ class Options(C.Structure):
         ("delta", C.c_double), ("mom_rtol", C.c_double), ("mom_maxit", C.c_int),
         ("p_rtol", C.c_double), ("p_maxit", C.c_int),
         ("amg_nu", C.c_int), ("amg_omega", C.c_double), ("amg_cscale", C.c_double), ("amg_coarse_max", C.c_int),
-        ("mixed_inner_rtol", C.c_double), ("check_every", C.c_int),
+        ("mixed_inner_rtol", C.c_double), ("check_every", C.c_int), ("inlet_scale", C.c_double),
     ]
 
 
@@ -60,7 +60,7 @@ class Stats(C.Structure):
         ("p_iters", C.c_int64), ("p_solves", C.c_int64), ("p_loops", C.c_int64), ("p_outer", C.c_int64),
         ("last_p_relres", C.c_double), ("p_solve_ms", C.c_double), ("last_p_max", C.c_double),
         ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
-        ("clip_passes", C.c_int64), ("host_polls", C.c_int64),
+        ("clip_passes", C.c_int64), ("host_polls", C.c_int64), ("p_guesses", C.c_int64),
     ]
 
 
@@ -99,7 +99,7 @@ def lib():
         L.mc_set_options.argtypes = [P_, C.POINTER(Options)]
         for n in ["mc_timestep", "mc_get_phi_gradients", "mc_set_up_field", "mc_momentum_assemble",
                   "mc_pressure_assemble", "mc_calculate_pressure", "mc_update_P", "mc_sync", "mc_profile_reset",
-                  "mc_scalar_batch_solve"]:
+                  "mc_scalar_batch_solve", "mc_reset_fields"]:
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
         L.mc_get_phi_gradient.argtypes = [P_, C.c_int, C.c_int]
@@ -118,8 +118,16 @@ def lib():
         L.mc_profile_enable.argtypes = [P_, C.c_int]
         L.mc_profile_read.argtypes = [P_, C.c_void_p, C.c_void_p, C.c_void_p]
         L.mc_copy_probe.argtypes = [P_, C.c_int64, C.c_int, C.POINTER(C.c_double)]
-        L.mc_mpnn_load.argtypes = [P_, C.c_void_p, C.c_int64, C.c_int, C.c_int]
-        L.mc_mpnn_infer.argtypes = [P_, C.c_void_p, C.c_void_p]
+        L.mc_mpnn_nparams.restype = C.c_int64
+        L.mc_mpnn_nparams.argtypes = []
+        L.mc_mpnn_load.argtypes = [P_, C.c_void_p, C.c_int64]
+        L.mc_mpnn_save.argtypes = [P_, C.c_void_p, C.c_int64]
+        L.mc_mpnn_collect.argtypes = [P_, C.c_int64]
+        L.mc_mpnn_dataset_size.argtypes = [P_, C.POINTER(C.c_int64)]
+        L.mc_mpnn_pair_get.argtypes = [P_, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mc_mpnn_forward_pair.argtypes = [P_, C.c_int64, C.c_void_p]
+        L.mc_mpnn_grad.argtypes = [P_, C.c_int64, C.c_void_p, C.POINTER(C.c_double)]
+        L.mc_mpnn_train_step.argtypes = [P_, C.c_int64, C.c_double, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -132,6 +140,11 @@ def default_options(**kw):
             raise KeyError(k)
         setattr(o, k, v)
     return o
+
+
+def mpnn_nparams():
+    """Number of FP32 parameters of the learned-guess net."""
+    return int(lib().mc_mpnn_nparams())
 
 
 class Context:
@@ -324,6 +337,53 @@ class Context:
         else:
             mask = sum(1 << KC_NAMES.index(k) for k in classes)
         self._check(self.L.mc_profile_enable(self.h, mask), "mc_profile_enable")
+
+    def reset_fields(self):
+        """Back to the constructor's fields (mesh, geometry, AMG kept)."""
+        self._check(self.L.mc_reset_fields(self.h), "mc_reset_fields")
+
+    # ---- learned initial guess ----
+    def mpnn_load(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.L.mc_mpnn_load(self.h, w.ctypes.data, w.size), "mc_mpnn_load")
+
+    def mpnn_save(self):
+        w = np.empty(mpnn_nparams(), dtype=np.float32)
+        self._check(self.L.mc_mpnn_save(self.h, w.ctypes.data, w.size), "mc_mpnn_save")
+        return w
+
+    def mpnn_collect(self, max_pairs):
+        self._check(self.L.mc_mpnn_collect(self.h, int(max_pairs)), "mc_mpnn_collect")
+
+    def mpnn_dataset_size(self):
+        n = C.c_int64(0)
+        self._check(self.L.mc_mpnn_dataset_size(self.h, C.byref(n)), "mc_mpnn_dataset_size")
+        return n.value
+
+    def mpnn_pair(self, pair):
+        f = np.empty((self.N, 4), dtype=np.float32)
+        e = np.empty((self.N, 6), dtype=np.float32)
+        t = np.empty(self.N, dtype=np.float32)
+        self._check(self.L.mc_mpnn_pair_get(self.h, int(pair), f.ctypes.data, e.ctypes.data, t.ctypes.data),
+                    "mc_mpnn_pair_get")
+        return f, e, t
+
+    def mpnn_forward_pair(self, pair):
+        y = np.empty(self.N, dtype=np.float32)
+        self._check(self.L.mc_mpnn_forward_pair(self.h, int(pair), y.ctypes.data), "mc_mpnn_forward_pair")
+        return y
+
+    def mpnn_grad(self, pair):
+        g = np.empty(mpnn_nparams(), dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._check(self.L.mc_mpnn_grad(self.h, int(pair), g.ctypes.data, C.byref(loss)), "mc_mpnn_grad")
+        return g, loss.value
+
+    def mpnn_train_step(self, pair, lr, want_loss=False):
+        loss = C.c_double(float("nan"))
+        self._check(self.L.mc_mpnn_train_step(self.h, int(pair), float(lr), C.byref(loss) if want_loss else None),
+                    "mc_mpnn_train_step")
+        return loss.value
 
     def copy_probe(self, nbytes, reps=10):
         """Average ms of a plain device copy of `nbytes` (read + write)."""

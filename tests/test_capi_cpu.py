@@ -34,7 +34,7 @@ def test_header_declares_the_entry_points():
     names = _declared_functions()
     for must in ["mc_ctx_create", "mc_ctx_create_box", "mc_ctx_destroy", "mc_timestep", "mc_simple_iteration",
                  "mc_get_phi_gradients", "mc_momentum_assemble", "mc_pressure_assemble", "mc_pressure_solve",
-                 "mc_calculate_pressure", "mc_scalar_solve", "mc_mpnn_infer", "mc_last_error"]:
+                 "mc_calculate_pressure", "mc_scalar_solve", "mc_mpnn_train_step", "mc_last_error"]:
         assert must in names
 
 
