@@ -31,12 +31,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "SIMPLE iter/s at 4.2M cells: learned+mixed vs FP64 GPU at equal residual"
+FIELDS = [("U", 0), ("V", 1), ("W", 2), ("P", 3), ("TE", 5), ("ED", 6), ("TEM", 8), ("FUL", 9), ("PRO", 10),
+          ("VARF", 11), ("VARP", 12)]
 HBM_PEAK_GBS = 8000.0
 
 
 def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
-    psolve = mc.PSOLVE_MIXED if form.startswith("mixed") else mc.PSOLVE_FP64
-    ctx = mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5)
+    psolve = mc.PSOLVE_MIXED if "mixed" in form else mc.PSOLVE_FP64
+    use_net = form.startswith("learned")
+    ctx = mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5, learned=1 if use_net else 0)
+    if use_net:
+        from minicombust_app_amd import learned
+        ctx.mpnn_load(learned.load_weights())
     it = 0
     for _ in range(warmup):
         ctx.simple_iteration(it % 3)
@@ -56,10 +62,14 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
             if v["calls"] == 0:
                 continue
             ms_avg = v["ms"] / v["calls"]
-            gbs = (v["bytes"] / v["calls"]) / (ms_avg * 1e-3) / 1e9 if v["bytes"] > 0 and ms_avg > 0 else None
+            rate = (v["bytes"] / v["calls"]) / (ms_avg * 1e-3) if v["bytes"] > 0 and ms_avg > 0 else None
             kern[name] = {"ms_per_step": v["ms"] / profile_steps, "calls_per_step": v["calls"] / profile_steps,
-                          "avg_launch_ms": ms_avg, "bytes_per_launch": v["bytes"] / v["calls"],
-                          "achieved_GBs": gbs}
+                          "avg_launch_ms": ms_avg, "bytes_per_launch": v["bytes"] / v["calls"]}
+            if name.startswith("mpnn"):   # these classes count MFMA flops, not bytes
+                kern[name]["achieved_TFLOPs"] = rate / 1e12 if rate else None
+                kern[name]["achieved_GBs"] = None
+            else:
+                kern[name]["achieved_GBs"] = rate / 1e9 if rate else None
     if roof_class is None and kern:
         cands = [(v["ms_per_step"], k) for k, v in kern.items() if v["achieved_GBs"] and k in mc.KC_KERNELS]
         roof_class = max(cands)[1] if cands else None
@@ -91,11 +101,41 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
         "refinement_steps_per_step": st["p_outer"] / steps,
         "last_rel_residual": st["p_relres_max"],
         "momentum_iters_per_step": st["mom_iters"] / steps,
+        "learned_guesses_per_step": st["p_guesses"] / steps,
         "scalar_iters_per_step": st["scalar_iters"] / steps,
         "kernels": kern,
         "roof_live": live,
+        # final fields after the same iteration sequence in every form
+        "fields": {name: ctx.phi(f)[:ctx.N].copy() for name, f in FIELDS},
+        "memory": ctx.memory(),
     }
     ctx.close()
+    return out
+
+
+def mfma_of(forms):
+    """MFMA throughput of the learned guess's forward pass (BF16 dense peak)."""
+    for name, v in forms.items():
+        k = v["kernels"].get("mpnn_fwd")
+        if k and k["avg_launch_ms"] > 0:
+            tf = k["bytes_per_launch"] / (k["avg_launch_ms"] * 1e-3) / 1e12
+            return {"form": name, "flops_per_forward": k["bytes_per_launch"], "avg_forward_ms": round(k["avg_launch_ms"], 4),
+                    "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tf / 2500.0, 5)}
+    return None
+
+
+def field_errors(forms):
+    """max |x - x_fp64| / max |x_fp64| per field, for each form against fp64."""
+    import numpy as np
+    if "fp64" not in forms:
+        return None
+    ref = forms["fp64"]["fields"]
+    out = {}
+    for name, v in forms.items():
+        if name == "fp64":
+            continue
+        out[name] = {f: float(np.max(np.abs(v["fields"][f] - ref[f])) / max(np.max(np.abs(ref[f])), 1e-300))
+                     for f in ref}
     return out
 
 
@@ -143,7 +183,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,mixed,learned_mixed")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-m", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -160,8 +200,14 @@ def main():
                                     cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
 
     from minicombust_app_amd import mcflow as mc
+    from minicombust_app_amd import learned
     forms = {}
+    skipped = {}
     for form in args.forms.split(","):
+        if form.startswith("learned") and not os.path.exists(learned.WEIGHTS):
+            skipped[form] = "no trained weights (%s)" % os.path.relpath(learned.WEIGHTS, ROOT)
+            print("# %s skipped: %s" % (form, skipped[form]), file=sys.stderr, flush=True)
+            continue
         forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps, args.roofline_class)
         print("# %s: %.2f SIMPLE iter/s, pressure %.2f ms/iter (%.1f PCG iters), rel.res %.2e" % (
             form, forms[form]["iter_per_s"], forms[form]["pressure_ms_per_step"],
@@ -182,7 +228,7 @@ def main():
             cpu = {"error": "cpu baseline failed: %s" % e}
 
     N = 2 * args.m ** 3
-    dtype = {"fp64": "f64", "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16"}[head]
+    dtype = {"fp64": "f64", "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16", "learned_fp64": "f64+bf16"}[head]
     line = {
         "metric": METRIC,
         "value": round(h["iter_per_s"], 3),
@@ -201,7 +247,12 @@ def main():
                    "modifier": args.m, "cells": N, "form": head},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
+                      "learned_guesses_per_step": v["learned_guesses_per_step"],
                       "last_rel_residual": v["last_rel_residual"]} for k, v in forms.items()},
+        "forms_skipped": skipped or None,
+        "field_rel_error_vs_fp64": field_errors(forms),
+        "device_memory_GB": {k: round(v / 1e9, 3) for k, v in h["memory"].items()},
+        "mpnn_mfma": mfma_of(forms),
         "speedup_vs_fp64": (round(h["iter_per_s"] / forms["fp64"]["iter_per_s"], 3) if "fp64" in forms else None),
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in h["kernels"].items()},

@@ -145,13 +145,13 @@ enum {
     MC_KC_SPMV32,        /* k_pcg_pq<.,float>: same, FP32 inner PCG              */
     MC_KC_VCYCLE,        /* AMG V-cycle, all levels (several kernels)            */
     MC_KC_CORRECT,       /* k_flux_correct: mass-flux correction                 */
-    MC_KC_MPNN,          /* message-passing net forward layer                    */
+    MC_KC_MPNN,          /* learned guess: whole forward pass (work = MFMA flops) */
     MC_KC_JAC64,         /* k_jac<.,double>: fine-level Jacobi sweep             */
     MC_KC_JAC32,         /* k_jac<.,float>: fine-level Jacobi sweep              */
     MC_KC_BICG,          /* k_bi_spmv_v: BiCGStab SpMV + (r0,v) dot              */
     MC_KC_SCALAR_BATCH,  /* k_scalar_batch: 4 passive scalars assembled at once  */
     MC_KC_IR_RESID,      /* k_ir_resid: FP64 residual of the refinement loop     */
-    MC_KC_MPNN_BWD,      /* message-passing net backward layer                   */
+    MC_KC_MPNN_BWD,      /* learned guess: whole backward pass (work = flops)     */
     MC_KC_COUNT
 };
 
@@ -258,6 +258,16 @@ int  mc_profile_reset(mc_ctx *ctx);
    WRITE_SIZE counters (kernel symbol mc::k_copy_probe).  No counterpart in the
    reference: measurement only. */
 int  mc_copy_probe(mc_ctx *ctx, int64_t bytes, int reps, double *ms_out);
+/* Device memory held by the context, per category (array of MC_MEM_COUNT). */
+enum { MC_MEM_MESH = 0,   /* topology, geometry, face orders                */
+       MC_MEM_FIELDS,     /* 13 fields + gradients, A/S vectors, face fields */
+       MC_MEM_MATRIX,     /* CSR pattern and values, diagonal copies         */
+       MC_MEM_AMG,        /* AMG hierarchy                                   */
+       MC_MEM_WORK,       /* solver work vectors, reductions                 */
+       MC_MEM_MPNN,       /* learned guess: weights, activations             */
+       MC_MEM_DATASET,    /* learned guess: training pairs                   */
+       MC_MEM_COUNT };
+int  mc_ctx_memory(mc_ctx *ctx, int64_t *bytes);
 
 /* ---- learned pressure-correction initial guess -------------------------- *
  * No counterpart in the reference: its pressure solves start from zero

@@ -21,6 +21,7 @@ int alloc(Ctx *c, void **p, size_t bytes)
         return MC_ERR_NOMEM;
     }
     allocs_of(c).push_back(q);
+    c->mem_bytes[c->mem_cat] += (int64_t)bytes;
     *p = q;
     return MC_OK;
 }
@@ -103,6 +104,7 @@ static void set_constants(Ctx *c)
 
 static int alloc_workspace(Ctx *c)
 {
+    MemScope ms(c, MC_MEM_WORK);
     const int N = c->N;
     for (int i = 0; i < 24; i++) MC_TRY(dalloc(c, &c->wk[i], N));
     for (int i = 0; i < 8; i++) {
@@ -557,6 +559,14 @@ int mc_profile_reset(mc_ctx *x)
     MC_TRY(profile_fold(c));
     for (int k = 0; k < MC_KC_COUNT; k++) { c->kt_ms[k] = 0; c->kt_calls[k] = 0; c->kt_bytes[k] = 0; }
     memset(&c->st, 0, sizeof(c->st));
+    return MC_OK;
+}
+
+int mc_ctx_memory(mc_ctx *x, int64_t *bytes)
+{
+    CTX_OR_FAIL
+    if (!bytes) return MC_ERR_ARG;
+    for (int k = 0; k < MC_MEM_COUNT; k++) bytes[k] = c->mem_bytes[k];
     return MC_OK;
 }
 

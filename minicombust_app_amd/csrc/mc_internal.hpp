@@ -151,6 +151,8 @@ struct Ctx {
 
     // ---- device allocations owned by the context ----
     std::vector<void *> allocs;
+    int mem_cat = MC_MEM_MESH;             // category charged by alloc()
+    int64_t mem_bytes[MC_MEM_COUNT] = {};
 
     // ---- stats / timing ----
     mc_stats st{};
@@ -289,6 +291,13 @@ inline int dalloc(Ctx *c, T **p, size_t count)
     return alloc(c, (void **)p, count * sizeof(T) + 16);
 }
 int dfree_all(Ctx *c);
+// charge the allocations of a scope to one memory category
+struct MemScope {
+    Ctx *c;
+    int old;
+    MemScope(Ctx *c_, int cat) : c(c_), old(c_->mem_cat) { c->mem_cat = cat; }
+    ~MemScope() { c->mem_cat = old; }
+};
 
 // profiling of kernel classes with HIP events on the context stream: the
 // constructor records the start event, the destructor the end event; pairs

@@ -188,6 +188,7 @@ __global__ void k_csr_to_dia(int nnz, const int *map, const double *csr, double 
 
 int dia_setup_fine(Ctx *c)
 {
+    MemScope ms(c, MC_MEM_MATRIX);
     std::vector<int> rp(c->N + 1), col(c->nnz), offs, map;
     MC_HIP(c, hipMemcpy(rp.data(), c->row_ptr, (size_t)(c->N + 1) * 4, hipMemcpyDeviceToHost));
     MC_HIP(c, hipMemcpy(col.data(), c->col, (size_t)c->nnz * 4, hipMemcpyDeviceToHost));
@@ -794,6 +795,7 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
 
 int amg_setup(Ctx *c, const double *vals)
 {
+    MemScope ms(c, MC_MEM_AMG);
     HostCSR A;
     A.n = c->N;
     A.rp.resize(c->N + 1);

@@ -447,6 +447,7 @@ int mesh_finish(Ctx *c)
     MC_HIP(c, hipMemsetAsync(c->ff1, 0, (size_t)F * 8, c->stream));
 
     // fields
+    MemScope ms_fields(c, MC_MEM_FIELDS);
     for (int f = 0; f < NFIELDS; f++) {
         MC_TRY(dalloc(c, &c->phi[f], NS));
         MC_TRY(dalloc(c, &c->gx[f], NS)); MC_TRY(dalloc(c, &c->gy[f], NS)); MC_TRY(dalloc(c, &c->gz[f], NS));
@@ -465,6 +466,7 @@ int mesh_finish(Ctx *c)
     MC_TRY(init_fields(c));
 
     // CSR pattern, face orders
+    MemScope ms_csr(c, MC_MEM_MATRIX);
     int *cnt = nullptr;
     MC_TRY(dalloc(c, &c->row_ptr, N + 1));
     MC_TRY(dalloc(c, &c->diag_idx, N));

@@ -333,6 +333,7 @@ __global__ void __launch_bounds__(256) k_edge_bwd(int n, const float *PQ, const 
             gb1.x += gz.x; gb1.y += gz.y; gb1.z += gz.z; gb1.w += gz.w;
             // the message i -> j seen from j: z_ji = P_j + Q_i + e_{j, rev} we + b1
             const int rs = rev[6 * i + s];
+            if (rs < 0) continue;   // j does not list i (not a symmetric graph)
             const float ej = e[6 * j + rs];
             const float4 Pj = *(const float4 *)(PQ + (size_t)j * 2 * MH + c4);
             const float4 dj = *(const float4 *)(dS + (size_t)j * MH + c4);
@@ -590,6 +591,7 @@ __global__ void k_guess_out(int N, const float *y, const double *sc, double *g)
 static int ensure(Ctx *c)
 {
     if (c->mpnn) return MC_OK;
+    MemScope ms(c, MC_MEM_MPNN);
     Mpnn *m = new (std::nothrow) Mpnn();
     if (!m) return MC_ERR_NOMEM;
     c->mpnn = m;
@@ -618,6 +620,7 @@ static int ensure_train(Ctx *c)
 {
     Mpnn *m = M(c);
     if (m->train_bufs) return MC_OK;
+    MemScope ms(c, MC_MEM_MPNN);
     const int64_t N = m->n;
     for (int l = 2; l <= ML; l++) MC_TRY(dalloc(c, &m->H[l], N * MH));
     for (int l = 1; l < ML; l++) {
@@ -895,6 +898,7 @@ int mc_mpnn_collect(mc_ctx *x, int64_t max_pairs)
     if (max_pairs > m->cap) {
         if (m->cap) { c->err = "mpnn: data set already allocated with a smaller capacity"; return MC_ERR_STATE; }
         const int64_t N = c->N;
+        MemScope ms(c, MC_MEM_DATASET);
         MC_TRY(dalloc(c, &m->ds_f, max_pairs * 4 * N));
         MC_TRY(dalloc(c, &m->ds_e, max_pairs * 6 * N));
         MC_TRY(dalloc(c, &m->ds_t, max_pairs * N));

@@ -37,6 +37,7 @@ KC_KERNELS = {"grad11": "mc::k_grad<11>", "grad1": "mc::k_grad<1>", "mom_asm": "
               "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
               "scalar_batch": "mc::k_scalar_batch", "ir_resid": "mc::k_ir_resid<7>"}
 KC_COUNT = len(KC_NAMES)
+MEM_NAMES = ["mesh", "fields", "matrix", "amg", "work", "mpnn", "dataset"]
 
 
 class Options(C.Structure):
@@ -118,6 +119,7 @@ def lib():
         L.mc_profile_enable.argtypes = [P_, C.c_int]
         L.mc_profile_read.argtypes = [P_, C.c_void_p, C.c_void_p, C.c_void_p]
         L.mc_copy_probe.argtypes = [P_, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        L.mc_ctx_memory.argtypes = [P_, C.c_void_p]
         L.mc_mpnn_nparams.restype = C.c_int64
         L.mc_mpnn_nparams.argtypes = []
         L.mc_mpnn_load.argtypes = [P_, C.c_void_p, C.c_int64]
@@ -337,6 +339,12 @@ class Context:
         else:
             mask = sum(1 << KC_NAMES.index(k) for k in classes)
         self._check(self.L.mc_profile_enable(self.h, mask), "mc_profile_enable")
+
+    def memory(self):
+        """Device bytes held by the context per category."""
+        b = np.zeros(len(MEM_NAMES), dtype=np.int64)
+        self._check(self.L.mc_ctx_memory(self.h, b.ctypes.data), "mc_ctx_memory")
+        return {k: int(v) for k, v in zip(MEM_NAMES, b)}
 
     def reset_fields(self):
         """Back to the constructor's fields (mesh, geometry, AMG kept)."""
