@@ -62,7 +62,8 @@ enum {
 typedef struct mc_options {
     int    mode;           /* MC_MODE_*                                            */
     int    psolve;         /* MC_PSOLVE_*                                          */
-    int    learned;        /* 1: MPNN initial guess for the pressure solve         */
+    int    learned;        /* learned guess for loop-1 pressure solves: 0 off, 1 while
+                              it pays (measured), 2 always                        */
     int    solve_scalars;  /* 1: the 7 scalar equations of the SIMPLE step         */
     double delta;          /* time step, src/minicombust.cpp:120 (1e-5)            */
     double mom_rtol;       /* BiCGStab rtol, momentum and scalars (PETSc: 1e-5)    */

@@ -789,229 +789,164 @@ __device__ __forceinline__ double scalar_visac(int type, double ev, double Visac
     return (ev + Visac / 0.9) / 0.9;
 }
 
-// One cell's faces in ascending face id -- the order in which the
-// reference's face loops reach this cell -- with the cell across each face
-// (boundary: N + side) and whether this cell is the face's cell0.
-struct FaceRef {
-    int f, o;
-    bool own;
-};
-__device__ __forceinline__ FaceRef face_ref(const Geo &g, int c, int k)
-{
-    const int v = g.fother[6 * c + k];
-    FaceRef r;
-    r.f = g.forder[6 * c + k];
-    r.o = v & 0x7fffffff;
-    r.own = v >= 0;
-    return r;
-}
-
-// Lane groups for face-parallel assembly: 8 lanes per cell, lane k < 6 owns
-// the cell's k-th face in ascending face id.  Each lane issues its own loads
-// (one round trip for the face ids, one for all face and neighbour data), and
-// the group's lane 0 applies the per-face terms in face order, so every sum
-// is formed exactly as the reference's face loop forms it.
-constexpr int CELLS_PER_BLOCK = 256 / 8;
-template <typename T>
-__device__ __forceinline__ T from_lane(T v, int k) { return __shfl(v, k, 8); }
-
-// sorted position of own_nb among {c} u {neighbours}: entries below it
-__device__ __forceinline__ int row_pos8(int c, int nb)
-{
-    int p = (c < nb) ? 1 : 0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        const int o = from_lane(nb, j);
-        p += (o >= 0 && o < nb) ? 1 : 0;
-    }
-    return p;
-}
-
 // FluxScalar (FlowSolver.inl:2516-2744), the unsteady term (:2908-2916),
 // solveTurbulenceModels (:2746-2883) and setup_sparse_matrix (:1186-1297).
-// Each face adds S = ((S + t0) + t1) + t2 and A = A + dA: a subtraction is
-// the addition of the negated term (exact), and unused terms are +0, which
-// leaves S unchanged because S starts at +0 and never holds -0.  GammaBlend
-// = 0 makes the blend term a signed zero for the same reason it is left out.
 __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
 {
-    const int lane = threadIdx.x & 7;
-    const int cell = xcd_block(blockIdx.x, gridDim.x) * CELLS_PER_BLOCK + (threadIdx.x >> 3);
-    const bool valid = cell < g.N;
-    const int c = valid ? cell : g.N - 1;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (c >= g.N) return;
     const int N = g.N;
     const double ev = a.ev;
+    const double GammaBlend = 0.0;
+    const bool wallcell = g.is_wall[c] != 0;
     const bool ed = a.type == TERBED;
-    // ---- per-face terms (lanes 0..5)
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, dA = 0.0, offd = 0.0;
-    int nb = -1;
-    if (lane < 6) {
-        const FaceRef fr = face_ref(g, c, lane);
-        const int f = fr.f;
-        const double m = a.mflux[f];
-        const dv3 sn = ld3(g.fnx, g.fny, g.fnz, f);
-        const double rlc = g.frlc[f];
-        const dv3 gc = ld3(a.gx, a.gy, a.gz, c), cc = ld3(g.cx, g.cy, g.cz, c);
-        if (fr.o < N) {
-            const int o = fr.o;
+    double S = 0.0, A = 0.0;
+    double own_ff[6];
+    int own_nb[6];
+    int nint = 0;
+    int fo[6];
+    for (int k = 0; k < 6; k++) fo[k] = g.forder[6 * c + k];
+    for (int k = 0; k < 6; k++) {
+        const int f = fo[k];
+        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const double mf = a.mflux[f];
+        if (c1 < N) {
             const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
-            const dv3 go = ld3(a.gx, a.gy, a.gz, o), co = ld3(g.cx, g.cy, g.cz, o);
-            const dv3 g0 = fr.own ? gc : go, g1 = fr.own ? go : gc;
-            const dv3 c0v = fr.own ? cc : co, c1v = fr.own ? co : cc;
             double Visac = ev * lambda0 + ev * lambda1;
             Visac -= ev;
             Visac = scalar_visac(a.type, ev, Visac);
-            const dv3 dPhiXac = g0 * lambda0 + g1 * lambda1;
-            const dv3 Xpn = c1v - c0v;
-            const double VisFace = Visac * rlc;
-            const double fde1 = Visac * dot(dPhiXac, sn);
+            const dv3 dPhiXac = ld3(a.gx, a.gy, a.gz, c0) * lambda0 + ld3(a.gx, a.gy, a.gz, c1) * lambda1;
+            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const double VisFace = Visac * g.frlc[f];
+            const double p0 = a.phi[c0], p1 = a.phi[c1];
+            const double PhiFace = (mf >= 0.0) ? p0 : p1;
+            const double fce = mf * PhiFace;
+            const double fde1 = Visac * dot(dPhiXac, ld3(g.fnx, g.fny, g.fnz, f));
+            const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
             const double fdi = VisFace * dot(dPhiXac, Xpn);
-            double ff0 = -VisFace - dmax(m, 0.0);
-            double ff1 = -VisFace + dmin(m, 0.0);
+            double ff0 = -VisFace - dmax(mf, 0.0);
+            double ff1 = -VisFace + dmin(mf, 0.0);
+            const double blend = GammaBlend * (fce - fci);
             if (ed) {  // FlowSolver.inl:2855-2875: rows of wall cells lose their off-diagonals
-                const bool wc = g.is_wall[c] != 0, wn = g.is_wall[o] != 0;
-                if (fr.own ? wc : wn) ff1 = 0.0;
-                if (fr.own ? wn : wc) ff0 = 0.0;
+                if (g.is_wall[c0]) ff1 = 0.0;
+                if (g.is_wall[c1]) ff0 = 0.0;
             }
-            if (fr.own) {
-                t0 = fde1; t1 = -fdi;
-                offd = ff1;
-                if (valid) { a.ff0[f] = ff0; a.ff1[f] = ff1; }
+            if (c == c0) {
+                S = S - blend + fde1 - fdi;
+                own_ff[nint] = ff1; own_nb[nint] = c1;
+                a.ff0[f] = ff0; a.ff1[f] = ff1;
             } else {
-                t0 = -fde1; t1 = fdi;
-                offd = ff0;
+                S = S + blend - fde1 + fdi;
+                own_ff[nint] = ff0; own_nb[nint] = c0;
             }
-            nb = o;
+            nint++;
         } else {
-            const int side = fr.o - N;
+            const int side = c1 - N;
             const int bt = g.btype[side];
-            const bool last = valid && (f == g.last_face[side]);
-            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+            const bool last = (f == g.last_face[side]);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 gP = ld3(a.gx, a.gy, a.gz, c0);
             if (bt == INLET) {
                 const double PhiFace = a.inlet_phi;
                 double Visac = a.inlet_ev;
                 Visac -= ev;
                 Visac = scalar_visac(a.type, ev, Visac);
-                const double VisFace = Visac * rlc;
-                const double fde = Visac * dot(gc, sn);
-                const double fdi = VisFace * dot(gc, Xpn);
-                const double fb = -VisFace + dmin(m, 0.0);
-                dA = -fb;
-                t0 = -(fb * PhiFace); t1 = fde; t2 = -fdi;
+                const double VisFace = Visac * g.frlc[f];
+                const double fde = Visac * dot(gP, ld3(g.fnx, g.fny, g.fnz, f));
+                const double fdi = VisFace * dot(gP, Xpn);
+                const double fb = -VisFace + dmin(mf, 0.0);
+                A = A - fb;
+                S = S - fb * PhiFace + fde - fdi;
                 if (last) a.phi[N + side] = PhiFace;
             } else if (bt == OUTLET) {
                 double Visac = ev;
                 Visac -= ev;
                 Visac = scalar_visac(a.type, ev, Visac);
-                const double PhiFace = a.phi[c] + dot(gc, Xpn);
-                const double VisFace = Visac * rlc;
-                const double fde = Visac * dot(gc, sn);
-                const double fdi = VisFace * dot(gc, Xpn);
-                t0 = fde; t1 = -fdi;
+                const double PhiFace = a.phi[c0] + dot(gP, Xpn);
+                const double VisFace = Visac * g.frlc[f];
+                const double fde = Visac * dot(gP, ld3(g.fnx, g.fny, g.fnz, f));
+                const double fdi = VisFace * dot(gP, Xpn);
+                S = S + fde - fdi;
                 if (last) a.phi[N + side] = PhiFace;
             } else if (bt == WALL) {
-                if (last && !ed) a.phi[N + side] = a.phi[c];
+                if (last && !ed) a.phi[N + side] = a.phi[c0];
             }
         }
     }
-    // ---- ordered accumulation (lane 0)
-    double S = 0.0, A = 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const double u0 = from_lane(t0, k), u1 = from_lane(t1, k), u2 = from_lane(t2, k), ua = from_lane(dA, k);
-        S = ((S + u0) + u1) + u2;
-        A = A + ua;
-    }
-    double offs[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) offs[k] = from_lane(offd, k);
-    const int pos = row_pos8(c, nb);
-    if (lane == 0 && valid) {
-        const double vol = g.vol[c], rho = g.rho[c];
-        double phic = a.phi[c];
-        const double fv = rho * vol * a.rdelta;
-        S += fv * phic;
-        A += fv;
-        const bool wallcell = g.is_wall[c] != 0;
-        const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
-        if (a.type == TERBTE) {
-            const dv3 dUdXp = ld3(a.gUx, a.gUy, a.gUz, c), dVdXp = ld3(a.gVx, a.gVy, a.gVz, c), dWdXp = ld3(a.gWx, a.gWy, a.gWz, c);
-            const double s1 = (dUdXp.x + dUdXp.x) * dUdXp.x + (dUdXp.y + dVdXp.x) * dUdXp.y + (dUdXp.z + dWdXp.x) * dUdXp.z;
-            const double s2 = (dVdXp.x + dUdXp.y) * dVdXp.x + (dVdXp.y + dVdXp.y) * dVdXp.y + (dVdXp.z + dWdXp.y) * dVdXp.z;
-            const double s3 = (dWdXp.x + dUdXp.z) * dWdXp.x + (dWdXp.y + dVdXp.z) * dWdXp.y + (dWdXp.z + dWdXp.z) * dWdXp.z;
-            const double VisT = ev - ev;
-            double TP = VisT * (s1 + s2 + s3);
-            const double Dis = rho * a.ED[c];
-            const double TEc = a.TE[c];
-            S = S + TP * vol;
-            A = A + Dis / (TEc + 0.000000000000000001) * vol;
-            if (wallcell) {
-                for (int k = 0; k < 6; k++) {
-                    const FaceRef fr = face_ref(g, c, k);
-                    if (fr.o < N || g.btype[fr.o - N] != WALL) continue;
-                    const int f = fr.f;
-                    S = S - TP * vol;
-                    const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
-                    const dv3 Up = v3(a.U[c] - 0., a.V[c] - 0., a.W[c] - 0.);
-                    const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
-                    const dv3 nh = n / mag(n);
-                    const double dp = dot(Up, nh);
-                    const dv3 Ut = Up - nh * dp;
-                    const double Uvel = sqrt(dot(Ut, Ut));
-                    const double distance_to_face = mag(Xpn);
-                    const double rkapdn = 1.0 / (0.419 * distance_to_face);
-                    const double Tau_w = ev * Uvel / distance_to_face;
-                    const double Utau = sqrt(Tau_w / rho);
-                    TP = Tau_w * Utau * rkapdn;
-                    if (f == g.last_face[fr.o - N]) a.TE[fr.o] = TEc;
-                    S = S + TP * vol;
-                    const double DisP = a.cmu75 * sqrt(TEc) * rkapdn;
-                    A = A + rho * DisP * vol;
-                }
-            }
-            a.TP[c] = TP;
-        } else if (ed) {
-            const double TEc = a.TE[c];
-            const double fact = a.ED[c] / (TEc + 0.000000000000000001) * vol;
-            S = S + 1.44 * fact * a.TP[c];
-            A = A + 1.92 * fact * rho;
-            if (wallcell) {
-                for (int k = 0; k < 6; k++) {
-                    const FaceRef fr = face_ref(g, c, k);
-                    if (fr.o < N || g.btype[fr.o - N] != WALL) continue;
-                    const int f = fr.f;
-                    const double distance = mag(ld3(g.fcx, g.fcy, g.fcz, f) - cc);
-                    const double Dis = a.cmu75 * pow(TEc, 1.5) / (distance * 0.419);
-                    phic = Dis;
-                    S = Dis;
-                    A = 1;
-                    if (f == g.last_face[fr.o - N]) a.ED[fr.o] = Dis;
-                }
-                // The reference stores Dis into phi.ED[c] here (FlowSolver.inl:2876),
-                // after every face flux has read the old values; the solve that
-                // follows (x0 = 0) overwrites it, so the value only enters through
-                // phic below, and neighbours keep reading the old ED[c].
+    const double vol = g.vol[c], rho = g.rho[c];
+    double phic = a.phi[c];
+    const double fv = rho * vol * a.rdelta;
+    S += fv * phic;
+    A += fv;
+    if (a.type == TERBTE) {
+        const dv3 dUdXp = ld3(a.gUx, a.gUy, a.gUz, c), dVdXp = ld3(a.gVx, a.gVy, a.gVz, c), dWdXp = ld3(a.gWx, a.gWy, a.gWz, c);
+        const double s1 = (dUdXp.x + dUdXp.x) * dUdXp.x + (dUdXp.y + dVdXp.x) * dUdXp.y + (dUdXp.z + dWdXp.x) * dUdXp.z;
+        const double s2 = (dVdXp.x + dUdXp.y) * dVdXp.x + (dVdXp.y + dVdXp.y) * dVdXp.y + (dVdXp.z + dWdXp.y) * dVdXp.z;
+        const double s3 = (dWdXp.x + dUdXp.z) * dWdXp.x + (dWdXp.y + dVdXp.z) * dWdXp.y + (dWdXp.z + dWdXp.z) * dWdXp.z;
+        const double VisT = ev - ev;
+        double TP = VisT * (s1 + s2 + s3);
+        const double Dis = rho * a.ED[c];
+        const double TEc = a.TE[c];
+        S = S + TP * vol;
+        A = A + Dis / (TEc + 0.000000000000000001) * vol;
+        if (wallcell) {
+            for (int k = 0; k < 6; k++) {
+                const int f = fo[k];
+                const int c1 = g.fc1[f];
+                if (c1 < N || g.btype[c1 - N] != WALL) continue;
+                S = S - TP * vol;
+                const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c);
+                const dv3 Up = v3(a.U[c] - 0., a.V[c] - 0., a.W[c] - 0.);
+                const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+                const dv3 nh = n / mag(n);
+                const double dp = dot(Up, nh);
+                const dv3 Ut = Up - nh * dp;
+                const double Uvel = sqrt(dot(Ut, Ut));
+                const double distance_to_face = mag(Xpn);
+                const double rkapdn = 1.0 / (0.419 * distance_to_face);
+                const double Tau_w = ev * Uvel / distance_to_face;
+                const double Utau = sqrt(Tau_w / rho);
+                TP = Tau_w * Utau * rkapdn;
+                if (f == g.last_face[c1 - N]) a.TE[N + c1 - N] = TEc;
+                S = S + TP * vol;
+                const double DisP = a.cmu75 * sqrt(TEc) * rkapdn;
+                A = A + rho * DisP * vol;
             }
         }
-#pragma unroll
-        for (int k = 0; k < 6; k++) A -= offs[k];
-        A *= (1. / a.urf);
-        S = S + (1.0 - a.urf) * A * phic;
-        a.AV[c] = A;
-        a.SU[c] = S;
+        a.TP[c] = TP;
+    } else if (ed) {
+        const double TEc = a.TE[c];
+        const double fact = a.ED[c] / (TEc + 0.000000000000000001) * vol;
+        S = S + 1.44 * fact * a.TP[c];
+        A = A + 1.92 * fact * rho;
+        if (wallcell) {
+            for (int k = 0; k < 6; k++) {
+                const int f = fo[k];
+                const int c1 = g.fc1[f];
+                if (c1 < N || g.btype[c1 - N] != WALL) continue;
+                const double distance = mag(ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c));
+                const double Dis = a.cmu75 * pow(TEc, 1.5) / (distance * 0.419);
+                phic = Dis;
+                S = Dis;
+                A = 1;
+                if (f == g.last_face[c1 - N]) a.ED[N + c1 - N] = Dis;
+            }
+            // The reference stores Dis into phi.ED[c] here (FlowSolver.inl:2876),
+            // after every face flux has read the old values; the solve that
+            // follows (x0 = 0) overwrites it, so the value only enters through
+            // phic below, and neighbours keep reading the old ED[c].
+        }
     }
-    // ---- matrix row: lane k writes its off-diagonal, lane 6 the diagonal
-    // (shuffles stay outside divergent branches: their source lanes must be active)
+    for (int k = 0; k < nint; k++) A -= own_ff[k];
+    A *= (1. / a.urf);
+    S = S + (1.0 - a.urf) * A * phic;
+    a.AV[c] = A;
+    a.SU[c] = S;
+    int pos[6], dpos;
+    row_positions(c, nint, own_nb, pos, dpos);
     const int rp = g.row_ptr[c];
-    const double Ad = from_lane(A, 0);
-    int dpos = 0;
-#pragma unroll
-    for (int j = 0; j < 6; j++) {
-        const int o = from_lane(nb, j);
-        dpos += (o >= 0 && o < c) ? 1 : 0;
-    }
-    if (valid && lane < 6 && nb >= 0) a.vals[rp + pos] = offd;
-    if (valid && lane == 6) a.vals[rp + dpos] = Ad;
+    a.vals[rp + dpos] = A;
+    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
 }
 
 // The passive scalars FUL, PRO, VARF and VARP share one Visac rule
@@ -1206,7 +1141,7 @@ int scalar_assemble(Ctx *c, int type)
     default: a.inlet_phi = 0.0; break;
     }
     KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
-    hipLaunchKernelGGL(k_scalar, dim3(div_up(c->N, CELLS_PER_BLOCK)), dim3(256), 0, c->stream, geo(c), a);
+    hipLaunchKernelGGL(k_scalar, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }

@@ -148,6 +148,12 @@ struct Ctx {
 
     // ---- learned initial guess ----
     void *mpnn = nullptr;
+    // measured use policy (loop-1 pressure solves): running means of the
+    // iterations with / without the guess, the guess's cost and the cost of
+    // one Krylov iteration; the guess stays on while it pays for itself
+    double lg_it_on = -1.0, lg_it_off = -1.0, lg_cost_ms = -1.0, lg_iter_ms = -1.0;
+    int64_t lg_count = 0;
+    bool lg_on = true;
 
     // ---- device allocations owned by the context ----
     std::vector<void *> allocs;

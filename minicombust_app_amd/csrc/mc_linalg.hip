@@ -1507,7 +1507,7 @@ static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double 
 {
     *used = false;
     *bnorm = 0.0;
-    if (!c->opt.learned || !c->mpnn) return MC_OK;
+    if (!c->mpnn) return MC_OK;
     const int n = c->N;
     double *gv = c->wk[6], *Ag = c->wk[7];
     double *sc = c->dscal + 640;
@@ -1534,14 +1534,31 @@ static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double 
     return MC_OK;
 }
 
+static double ema(double old, double v) { return old < 0.0 ? v : 0.7 * old + 0.3 * v; }
+
 int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol)
 {
     const auto t0 = std::chrono::steady_clock::now();
     mc_solve_status s{};
     int rc;
-    bool used = false;
+    bool used = false, tried = false;
     double bn0 = 0.0;
-    MC_TRY(learned_guess(c, b, x, &used, &bn0));
+    // learned = 2: every loop-1 solve starts from the guess; learned = 1: only
+    // while its measured saving (iterations x time per iteration) exceeds its
+    // cost, re-checking the other choice every 16th solve
+    if (c->opt.learned && c->mpnn && atol == 0.0) {
+        bool want = true;
+        if (c->opt.learned == 1) {
+            if (c->lg_it_on < 0.0) want = true;
+            else if (c->lg_it_off < 0.0) want = false;
+            else want = (c->lg_count % 16 == 15) ? !c->lg_on : c->lg_on;
+        }
+        if (want) {
+            MC_TRY(learned_guess(c, b, x, &used, &bn0));
+            tried = true;
+        }
+    }
+    const auto tg = std::chrono::steady_clock::now();
     if (c->opt.psolve == MC_PSOLVE_MIXED)
         rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used, bn0);
     else
@@ -1550,6 +1567,19 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     const auto t1 = std::chrono::steady_clock::now();
     c->st.p_solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (rc) return rc;
+    if (c->opt.learned && c->mpnn && atol == 0.0) {
+        const double solve_ms = std::chrono::duration<double, std::milli>(t1 - tg).count();
+        if (s.iterations > 0) c->lg_iter_ms = ema(c->lg_iter_ms, solve_ms / s.iterations);
+        if (tried) {
+            c->lg_cost_ms = ema(c->lg_cost_ms, std::chrono::duration<double, std::milli>(tg - t0).count());
+            c->lg_it_on = ema(c->lg_it_on, used ? s.iterations : (c->lg_it_off < 0 ? s.iterations : c->lg_it_off));
+        } else {
+            c->lg_it_off = ema(c->lg_it_off, s.iterations);
+        }
+        if (c->lg_it_on >= 0.0 && c->lg_it_off >= 0.0)
+            c->lg_on = (c->lg_it_off - c->lg_it_on) * c->lg_iter_ms > c->lg_cost_ms;
+        c->lg_count++;
+    }
     c->st.p_guesses += used ? 1 : 0;
     if (c->mpnn) MC_TRY(mpnn_collect_pair(c, b, x));
     c->st.p_iters += s.iterations;

@@ -14,7 +14,8 @@
 //               a_i = s_i W2 + deg_i b2
 //               u_i = relu([h_i, a_i] Wn1 + bn1)
 //               h_i = h_i + u_i Wn2 + bn2
-//   y = h wdec + bdec;   guess g = y s_b / s_d
+//   y = h wdec + bdec = the guess g, up to scale (the solve applies the
+//   Galerkin factor); training target: mean-zero solution / its rms
 //
 // The edge MLP's second linear layer is applied after the neighbour sum
 // (sum_j W2 r_ij = W2 sum_j r_ij), so every matrix product is node-level: an
@@ -576,15 +577,32 @@ __global__ void k_features(int N, const double *b, const double *vals, const int
         e[6 * i + s] = (p >= 0) ? (float)(vals[p] / sd) : 0.f;
     }
 }
+// The solve rescales the guess by its Galerkin factor, so the net learns the
+// solution's shape: the target is the mean-zero solution over its rms.
+__global__ void __launch_bounds__(256) k_sumsq(int n, const double *x, double *partials)
+{
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc[0] += x[i] * x[i];
+    block_partials<1>(acc, partials, blockIdx.x, gridDim.x);
+}
+struct EpiRms {
+    double *out;
+    double inv_n;
+    __device__ void operator()(const double (&v)[1]) const
+    {
+        const double r = sqrt(v[0] * inv_n);
+        *out = (r > 0.0) ? r : 1.0;
+    }
+};
 __global__ void k_target(int N, const double *x, const double *sc, float *t)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) t[i] = (float)(x[i] * sc[2] / sc[1]);
+    if (i < N) t[i] = (float)(x[i] / sc[6]);
 }
-__global__ void k_guess_out(int N, const float *y, const double *sc, double *g)
+__global__ void k_guess_out(int N, const float *y, double *g)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) g[i] = (double)y[i] * sc[1] / sc[2];
+    if (i < N) g[i] = (double)y[i];
 }
 
 // --------------------------------------------------------------- host --
@@ -788,7 +806,7 @@ int mpnn_guess(Ctx *c, const double *b, double *g, int *ok)
         KTimer kt(c, MC_KC_MPNN, forward_flops(c->N));
         forward(c, m->f, m->e, false);
     }
-    hipLaunchKernelGGL(k_guess_out, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, c->N, m->y, m->sc, g);
+    hipLaunchKernelGGL(k_guess_out, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, c->N, m->y, g);
     MC_LAUNCH_CHECK(c);
     *ok = 1;
     return MC_OK;
@@ -801,6 +819,9 @@ int mpnn_collect_pair(Ctx *c, const double *b, const double *x)
     if (!m || m->count >= m->cap) return MC_OK;
     const int64_t N = c->N, k = m->count;
     MC_TRY(features(c, b, m->ds_f + k * 4 * N, m->ds_e + k * 6 * N));
+    const int nb = std::min(div_up(N, 256), 1024);
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, c->stream, (int)N, x, c->red);
+    launch_finish<1>(c->stream, c->red, nb, EpiRms{m->sc + 6, 1.0 / (double)N});
     hipLaunchKernelGGL(k_target, dim3(div_up(N, 256)), dim3(256), 0, c->stream, (int)N, x, m->sc, m->ds_t + k * N);
     MC_LAUNCH_CHECK(c);
     m->count++;
