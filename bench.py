@@ -102,6 +102,7 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
         "last_rel_residual": st["p_relres_max"],
         "momentum_iters_per_step": st["mom_iters"] / steps,
         "learned_guesses_per_step": st["p_guesses"] / steps,
+        "learned_guess_residual_ratio": (st["p_guess_ratio_sum"] / st["p_guesses"]) if st["p_guesses"] else None,
         "scalar_iters_per_step": st["scalar_iters"] / steps,
         "kernels": kern,
         "roof_live": live,
@@ -248,6 +249,7 @@ def main():
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
                       "learned_guesses_per_step": v["learned_guesses_per_step"],
+                      "learned_guess_residual_ratio": v["learned_guess_residual_ratio"],
                       "last_rel_residual": v["last_rel_residual"]} for k, v in forms.items()},
         "forms_skipped": skipped or None,
         "field_rel_error_vs_fp64": field_errors(forms),

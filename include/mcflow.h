@@ -103,6 +103,7 @@ typedef struct mc_stats {
     int64_t clip_passes;      /* relaxation passes of that repair                */
     int64_t host_polls;       /* host waits on solver convergence flags          */
     int64_t p_guesses;        /* pressure solves started from the learned guess  */
+    double  p_guess_ratio_sum;/* sum over those of ||b - A x0|| / ||b - mean b||  */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>

@@ -1531,6 +1531,7 @@ static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double 
     MC_HIP(c, hipStreamSynchronize(c->stream));
     *used = c->hscal[644] > 0.0;
     *bnorm = c->hscal[642];
+    if (*used && c->hscal[642] > 0.0) c->st.p_guess_ratio_sum += c->hscal[643] / c->hscal[642];
     return MC_OK;
 }
 

@@ -4,9 +4,9 @@
 // No counterpart in the reference (its pressure solves start from zero,
 // FlowSolver.inl:2203-2273).  Definition (H = 64, 3 layers):
 //
-//   features f_i (4): (b_i - mean b) / s_b, A_ii / s_d, V_i / mean V, q_i / s_b
+//   features f_i (4): (b_i - mean b) / s_b, A_ii / s_d, V_i / mean V, q_i / s_q
 //     b: pressure-correction RHS, A: its matrix, q: net mass outflow of the
-//     cell, s_b = rms(b - mean b), s_d = mean(A_ii); edge feature e_is =
+//     cell, s_b = rms(b - mean b), s_q = rms(q), s_d = mean(A_ii); edge feature e_is =
 //     A_{i, nbr(i,s)} / s_d for the six sides s (0 where there is no neighbour)
 //   h = relu(f Wenc + benc)
 //   per layer:  [P | Q] = h [Wa | Wb]
@@ -537,23 +537,37 @@ struct EpiFeat1 {
         sc[3] = v[2] * inv_n;   // mean V
     }
 };
-__global__ void __launch_bounds__(256) k_feat_var(int N, const double *b, const double *sc, double *partials)
+// net mass outflow of cell i
+__device__ __forceinline__ double cell_outflow(int i, const double *mflux, const int *forder, const int *fother)
 {
-    double acc[1] = {0.0};
+    double q = 0.0;
+    for (int k = 0; k < 6; k++) {
+        const double m = mflux[forder[6 * i + k]];
+        q += (fother[6 * i + k] >= 0) ? m : -m;
+    }
+    return q;
+}
+// partials of (b - mean)^2 and q^2
+__global__ void __launch_bounds__(256) k_feat_var(int N, const double *b, const double *mflux, const int *forder,
+                                                 const int *fother, const double *sc, double *partials)
+{
+    double acc[2] = {0.0, 0.0};
     const double mu = sc[0];
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
-        const double d = b[i] - mu;
+        const double d = b[i] - mu, q = cell_outflow(i, mflux, forder, fother);
         acc[0] += d * d;
+        acc[1] += q * q;
     }
-    block_partials<1>(acc, partials, blockIdx.x, gridDim.x);
+    block_partials<2>(acc, partials, blockIdx.x, gridDim.x);
 }
 struct EpiFeat2 {
     double *sc;
     double inv_n;
-    __device__ void operator()(const double (&v)[1]) const
+    __device__ void operator()(const double (&v)[2]) const
     {
-        const double r = sqrt(v[0] * inv_n);
-        sc[1] = (r > 0.0) ? r : 1.0;   // s_b
+        const double rb = sqrt(v[0] * inv_n), rq = sqrt(v[1] * inv_n);
+        sc[1] = (rb > 0.0) ? rb : 1.0;   // s_b
+        sc[7] = (rq > 0.0) ? rq : 1.0;   // s_q
     }
 };
 __global__ void k_features(int N, const double *b, const double *vals, const int *diag_idx, const int *pmap,
@@ -562,16 +576,12 @@ __global__ void k_features(int N, const double *b, const double *vals, const int
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const double mu = sc[0], sb = sc[1], sd = sc[2], mv = sc[3];
-    double q = 0.0;   // net mass outflow of the cell
-    for (int k = 0; k < 6; k++) {
-        const double m = mflux[forder[6 * i + k]];
-        q += (fother[6 * i + k] >= 0) ? m : -m;
-    }
+    const double mu = sc[0], sb = sc[1], sd = sc[2], mv = sc[3], sq = sc[7];
+    const double q = cell_outflow(i, mflux, forder, fother);
     f[4 * i + 0] = (float)((b[i] - mu) / sb);
     f[4 * i + 1] = (float)(vals[diag_idx[i]] / sd);
     f[4 * i + 2] = (float)(vol[i] / mv);
-    f[4 * i + 3] = (float)(q / sb);
+    f[4 * i + 3] = (float)(q / sq);
     for (int s = 0; s < 6; s++) {
         const int p = pmap[6 * i + s];
         e[6 * i + s] = (p >= 0) ? (float)(vals[p] / sd) : 0.f;
@@ -787,8 +797,9 @@ static int features(Ctx *c, const double *b, float *f, float *e)
     const int nb = std::min(div_up(N, 256), 1024);
     hipLaunchKernelGGL(k_feat_sums, dim3(nb), dim3(256), 0, c->stream, N, b, c->PA_vals, c->diag_idx, c->vol, c->red);
     launch_finish<3>(c->stream, c->red, nb, EpiFeat1{m->sc, 1.0 / N});
-    hipLaunchKernelGGL(k_feat_var, dim3(nb), dim3(256), 0, c->stream, N, b, m->sc, c->red);
-    launch_finish<1>(c->stream, c->red, nb, EpiFeat2{m->sc, 1.0 / N});
+    hipLaunchKernelGGL(k_feat_var, dim3(nb), dim3(256), 0, c->stream, N, b, c->mflux, c->forder, c->fother, m->sc,
+                       c->red);
+    launch_finish<2>(c->stream, c->red, nb, EpiFeat2{m->sc, 1.0 / N});
     hipLaunchKernelGGL(k_features, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, b, c->PA_vals, c->diag_idx,
                        m->pmap, c->vol, c->mflux, c->forder, c->fother, m->sc, f, e);
     MC_LAUNCH_CHECK(c);

@@ -62,6 +62,7 @@ class Stats(C.Structure):
         ("last_p_relres", C.c_double), ("p_solve_ms", C.c_double), ("last_p_max", C.c_double),
         ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
         ("clip_passes", C.c_int64), ("host_polls", C.c_int64), ("p_guesses", C.c_int64),
+        ("p_guess_ratio_sum", C.c_double),
     ]
 
 

@@ -10,17 +10,18 @@ backward is pinned by finite differences (tests/test_mpnn_cpu.py).
 Graph: cells are nodes; each cell has up to six neighbours (side order, -1 at
 the boundary); messages run both ways over every interior face.
 
-    features f_i (4): b_i / s_b, A_ii / s_d, V_i / mean V, q_i / s_b
+    features f_i (4): (b_i - mean b) / s_b, A_ii / s_d, V_i / mean V, q_i / s_q
         (b: pressure-correction RHS, A_ii: its diagonal, q: net mass outflow,
-         s_b = rms(b), s_d = mean(A_ii));  edge feature e_ij = A_ij / s_d
+         s_b = rms(b - mean b), s_q = rms(q), s_d = mean(A_ii));
+        edge feature e_ij = A_ij / s_d
     h = relu(f Wenc + benc)
     3 layers:  P = h Wa,  Q = h Wb
                s_i = sum_j relu(P_i + Q_j + e_ij we + b1)
                a_i = s_i W2 + deg_i b2
                u_i = relu([h_i, a_i] Wn1 + bn1)
                h_i = h_i + u_i Wn2 + bn2
-    y = h wdec + bdec          guess x_i = y_i s_b / s_d
-    loss = mean((y - mean(y) - t)^2),  t = x* s_d / s_b (x* mean-zero)
+    y = h wdec + bdec          (the guess, up to the solve's Galerkin scaling)
+    loss = mean((y - mean(y) - t)^2),  t = x* / rms(x*) (x* mean-zero)
 
 GEMM operands are rounded to bfloat16 (round to nearest even) with float32
 accumulation when `bf16=True`, as the MFMA kernels compute them.
@@ -87,10 +88,13 @@ def _mm(x, w, lowp):
 
 
 def features(b, diag, vol, q):
-    s_b = np.sqrt(np.mean(b * b))
+    bc = b - b.mean()
+    s_b = np.sqrt(np.mean(bc * bc))
     s_b = s_b if s_b > 0 else 1.0
+    s_q = np.sqrt(np.mean(q * q))
+    s_q = s_q if s_q > 0 else 1.0
     s_d = np.mean(diag)
-    f = np.stack([b / s_b, diag / s_d, vol / np.mean(vol), q / s_b], axis=1)
+    f = np.stack([bc / s_b, diag / s_d, vol / np.mean(vol), q / s_q], axis=1)
     return f, s_b, s_d
 
 
