@@ -36,10 +36,12 @@ FIELDS = [("U", 0), ("V", 1), ("W", 2), ("P", 3), ("TE", 5), ("ED", 6), ("TEM", 
 HBM_PEAK_GBS = 8000.0
 
 
-def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None):
+def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=None):
     psolve = mc.PSOLVE_MIXED if "mixed" in form else mc.PSOLVE_FP64
     use_net = form.startswith("learned")
-    ctx = mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5, learned=1 if use_net else 0)
+    opts = dict(mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5, learned=1 if use_net else 0)
+    opts.update(extra or {})
+    ctx = mc.Context(m, **opts)
     if use_net:
         from minicombust_app_amd import learned
         ctx.mpnn_load(learned.load_weights())
@@ -190,6 +192,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-class", default=None, help="kernel class for the roofline (default: dominant)")
     ap.add_argument("--copy-probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
+    ap.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
+                    help="override an mc_options field for every form (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -209,7 +213,12 @@ def main():
             skipped[form] = "no trained weights (%s)" % os.path.relpath(learned.WEIGHTS, ROOT)
             print("# %s skipped: %s" % (form, skipped[form]), file=sys.stderr, flush=True)
             continue
-        forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps, args.roofline_class)
+        extra = {}
+        for kv in args.opt:
+            k, v = kv.split("=", 1)
+            extra[k] = float(v) if "." in v or "e" in v else int(v)
+        forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps, args.roofline_class,
+                               extra)
         print("# %s: %.2f SIMPLE iter/s, pressure %.2f ms/iter (%.1f PCG iters), rel.res %.2e" % (
             form, forms[form]["iter_per_s"], forms[form]["pressure_ms_per_step"],
             forms[form]["pressure_iters_per_step"], forms[form]["last_rel_residual"]), file=sys.stderr, flush=True)
@@ -245,7 +254,7 @@ def main():
         "data": "synthetic (reference box mesh and default initial/boundary fields)",
         "config": {"workload": "box mesh modifier %d (%d hex cells), simple mode, pressure rel. residual 1e-8, "
                                "momentum/scalar BiCGStab rtol 1e-5" % (args.m, N),
-                   "modifier": args.m, "cells": N, "form": head},
+                   "modifier": args.m, "cells": N, "form": head, "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
                       "learned_guesses_per_step": v["learned_guesses_per_step"],

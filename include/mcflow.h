@@ -78,6 +78,8 @@ typedef struct mc_options {
     int    check_every;    /* Krylov iterations between host convergence checks    */
     double inlet_scale;    /* multiplies the reference gas velocity (Mesh.hpp:114,
                               inlet and initial field); 1 = the reference case    */
+    double amg_cheb_ratio; /* > 1: the nu smoothing sweeps use Chebyshev weights on
+                              [2/ratio, 2]; <= 1: every sweep uses amg_omega      */
 } mc_options;
 
 typedef struct mc_solve_status {

@@ -938,23 +938,43 @@ static Mat<T> level_mat(AmgLevel &L)
 }
 
 // ------------------------------------------------------ V-cycle kernels --
-// first two pre-smoothing sweeps from x = 0: x1 = w D^-1 b, x2 = x1 + w D^-1 (b - A x1)
+// first two pre-smoothing sweeps from x = 0: x1 = w1 D^-1 b, x2 = x1 + w2 D^-1 (b - A x1)
 template <int ND, typename T>
-__global__ void __launch_bounds__(BLOCK) k_jac_first2(Mat<T> A, const T *dinv, const T *b, T *xout, T omega,
+__global__ void __launch_bounds__(BLOCK) k_jac_first2(Mat<T> A, const T *dinv, const T *b, T *xout, T w1, T w2,
                                                       int chunk, int two, const KState *stop)
 {
     if (stop && stop->all_done) return;
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
     if (!two) {
-        for (int i = r0 + threadIdx.x; i < r1; i += BLOCK) xout[i] = omega * dinv[i] * b[i];
+        for (int i = r0 + threadIdx.x; i < r1; i += BLOCK) xout[i] = w1 * dinv[i] * b[i];
         return;
     }
-    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)(omega * dinv[j] * b[j]); },
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)(w1 * dinv[j] * b[j]); },
                     [&](int r, const double (&ax)[1]) {
-                        const T x1 = omega * dinv[r] * b[r];
-                        xout[r] = (T)(x1 + omega * dinv[r] * (b[r] - (T)ax[0]));
+                        const T x1 = w1 * dinv[r] * b[r];
+                        xout[r] = (T)(x1 + w2 * dinv[r] * (b[r] - (T)ax[0]));
                     });
+}
+
+// Jacobi weights of the nu smoothing sweeps: all amg_omega, or, with
+// amg_cheb_ratio > 0, the reciprocals of the Chebyshev nodes of degree nu on
+// [2 / ratio, 2] -- a Chebyshev smoother written as nu Jacobi sweeps (the
+// spectrum of D^-1 A lies in [0, 2] for the zero-row-sum M-matrices of every
+// level).  Pre-smoothing uses them in this order, post-smoothing in reverse,
+// so the V-cycle stays symmetric.
+static void smoother_weights(const Ctx *c, int nu, double *w)
+{
+    const double ratio = c->opt.amg_cheb_ratio;
+    if (!(ratio > 1.0)) {
+        for (int k = 0; k < nu; k++) w[k] = c->opt.amg_omega;
+        return;
+    }
+    const double lo = 2.0 / ratio, hi = 2.0;
+    for (int k = 0; k < nu; k++) {
+        const double node = 0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos(M_PI * (2.0 * k + 1.0) / (2.0 * nu));
+        w[k] = 1.0 / node;
+    }
 }
 
 // x_out = y + w D^-1 (b - A y), y_j = x_j + cs * xc[agg_j] (agg == nullptr: y = x).
@@ -1076,7 +1096,8 @@ template <typename T>
 static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *stop)
 {
     AmgLevel &L = c->amg[l];
-    const T omega = (T)c->opt.amg_omega;
+    double wts[8];
+    smoother_weights(c, std::max(1, std::min(c->opt.amg_nu, 8)), wts);
     const int nu = std::max(1, c->opt.amg_nu);
     if (l + 1 == c->amg.size()) {
         hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x, stop);
@@ -1091,11 +1112,12 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     const Mat<T> A = level_mat<T>(L);
     T *dinv = lv_dinv<T>(L), *tmp = lv_t<T>(L);
     T *cur = x, *oth = tmp;
-    MC_DISPATCH_ND(A.nd, (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, omega,
+    MC_DISPATCH_ND(A.nd, (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, (T)wts[0],
+                   (T)wts[nu >= 2 ? 1 : 0],
                    g.chunk, nu >= 2 ? 1 : 0, stop);
     for (int s = 2; s < nu; s++) {
         MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
-                       (const int *)nullptr, (const T *)nullptr, (T)0, oth, omega, g.chunk, (const T *)nullptr,
+                       (const int *)nullptr, (const T *)nullptr, (T)0, oth, (T)wts[s], g.chunk, (const T *)nullptr,
                        (double *)nullptr, stop);
         std::swap(cur, oth);
     }
@@ -1121,7 +1143,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
             KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
             MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
                            first ? L.agg : (const int *)nullptr, first ? xc : (const T *)nullptr, first ? cs : (T)0,
-                           dst, omega, g.chunk, dotv, c->red, stop);
+                           dst, (T)wts[nu - 1 - s], g.chunk, dotv, c->red, stop);
         }
         if (dotv) launch_finish<1>(c->stream, c->red, g.blocks, EpiRz{ks});
         oth = cur;
