@@ -36,10 +36,11 @@ FIELDS = [("U", 0), ("V", 1), ("W", 2), ("P", 3), ("TE", 5), ("ED", 6), ("TEM", 
 HBM_PEAK_GBS = 8000.0
 
 
-def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=None):
+def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=None, mode="simple"):
     psolve = mc.PSOLVE_MIXED if "mixed" in form else mc.PSOLVE_FP64
     use_net = form.startswith("learned")
-    opts = dict(mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-8, mom_rtol=1e-5, learned=1 if use_net else 0)
+    opts = dict(mode=mc.MODE_SIMPLE if mode == "simple" else mc.MODE_COMPAT, psolve=psolve, p_rtol=1e-8,
+                mom_rtol=1e-5, learned=1 if use_net else 0)
     opts.update(extra or {})
     ctx = mc.Context(m, **opts)
     if use_net:
@@ -192,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-class", default=None, help="kernel class for the roofline (default: dominant)")
     ap.add_argument("--copy-probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
+    ap.add_argument("--mode", choices=["simple", "compat"], default="simple",
+                    help="pressure-correction semantics (compat: the reference CPU path as written)")
     ap.add_argument("--opt", action="append", default=[], metavar="KEY=VALUE",
                     help="override an mc_options field for every form (repeatable)")
     args = ap.parse_args()
@@ -218,7 +221,7 @@ def main():
             k, v = kv.split("=", 1)
             extra[k] = float(v) if "." in v or "e" in v else int(v)
         forms[form] = run_form(mc, args.m, form, args.steps, args.warmup, args.profile_steps, args.roofline_class,
-                               extra)
+                               extra, args.mode)
         print("# %s: %.2f SIMPLE iter/s, pressure %.2f ms/iter (%.1f PCG iters), rel.res %.2e" % (
             form, forms[form]["iter_per_s"], forms[form]["pressure_ms_per_step"],
             forms[form]["pressure_iters_per_step"], forms[form]["last_rel_residual"]), file=sys.stderr, flush=True)
@@ -252,8 +255,8 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic (reference box mesh and default initial/boundary fields)",
-        "config": {"workload": "box mesh modifier %d (%d hex cells), simple mode, pressure rel. residual 1e-8, "
-                               "momentum/scalar BiCGStab rtol 1e-5" % (args.m, N),
+        "config": {"workload": "box mesh modifier %d (%d hex cells), %s mode, pressure rel. residual 1e-8, "
+                               "momentum/scalar BiCGStab rtol 1e-5" % (args.m, N, args.mode),
                    "modifier": args.m, "cells": N, "form": head, "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],

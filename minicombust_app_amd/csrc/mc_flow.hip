@@ -965,8 +965,16 @@ struct ScalBatchArgs {
     double *ff0, *ff1, *AV, *vals;
     int type;
     double ev, inlet_ev, rdelta, urf;
+    // the temperature equation in the same pass (its own Visac and matrix)
+    double *tphi;
+    const double *tgx, *tgy, *tgz;
+    double *tS, *tvals;
+    double tinlet;
 };
 
+// TEM: with its own diffusion coefficient, every expression as in k_scalar
+// for TEMP, so its matrix and right-hand side are the ones k_scalar forms.
+template <bool TEM>
 __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -980,6 +988,7 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
     double own_ff[6];
     int own_nb[6];
     int nint = 0;
+    double St = 0.0, At = 0.0, own_fft[6];
     for (int k = 0; k < 6; k++) {
         const int f = g.forder[6 * c + k];
         const int c0 = g.fc0[f], c1 = g.fc1[f];
@@ -994,6 +1003,24 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
             const double VisFace = Visac * g.frlc[f];
             const double ff0 = -VisFace - dmax(mf, 0.0);
             const double ff1 = -VisFace + dmin(mf, 0.0);
+            if constexpr (TEM) {
+                double Vt = ev * lambda0 + ev * lambda1;
+                Vt -= ev;
+                Vt = scalar_visac(TEMP, ev, Vt);
+                const dv3 dPhiXac = ld3(a.tgx, a.tgy, a.tgz, c0) * lambda0 + ld3(a.tgx, a.tgy, a.tgz, c1) * lambda1;
+                const double VisFaceT = Vt * g.frlc[f];
+                const double p0 = a.tphi[c0], p1 = a.tphi[c1];
+                const double PhiFace = (mf >= 0.0) ? p0 : p1;
+                const double fce = mf * PhiFace;
+                const double fde1 = Vt * dot(dPhiXac, nrm);
+                const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
+                const double fdi = VisFaceT * dot(dPhiXac, Xpn);
+                const double t0 = -VisFaceT - dmax(mf, 0.0);
+                const double t1 = -VisFaceT + dmin(mf, 0.0);
+                const double blend = GammaBlend * (fce - fci);
+                if (c == c0) { St = St - blend + fde1 - fdi; own_fft[nint] = t1; }
+                else { St = St + blend - fde1 + fdi; own_fft[nint] = t0; }
+            }
 #pragma unroll
             for (int s = 0; s < SB; s++) {
                 const dv3 dPhiXac = ld3(a.gx[s], a.gy[s], a.gz[s], c0) * lambda0 + ld3(a.gx[s], a.gy[s], a.gz[s], c1) * lambda1;
@@ -1021,6 +1048,19 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
             const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
             const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
             if (bt == INLET) {
+                if constexpr (TEM) {
+                    double Vt = a.inlet_ev;
+                    Vt -= ev;
+                    Vt = scalar_visac(TEMP, ev, Vt);
+                    const double VisFaceT = Vt * g.frlc[f];
+                    const dv3 gP = ld3(a.tgx, a.tgy, a.tgz, c0);
+                    const double fde = Vt * dot(gP, nrm);
+                    const double fdi = VisFaceT * dot(gP, Xpn);
+                    const double fb = -VisFaceT + dmin(mf, 0.0);
+                    At = At - fb;
+                    St = St - fb * a.tinlet + fde - fdi;
+                    if (last) a.tphi[N + side] = a.tinlet;
+                }
                 double Visac = a.inlet_ev;
                 Visac -= ev;
                 Visac = scalar_visac(a.type, ev, Visac);
@@ -1036,6 +1076,18 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                     if (last) a.phi[s][N + side] = a.inlet_phi[s];
                 }
             } else if (bt == OUTLET) {
+                if constexpr (TEM) {
+                    double Vt = ev;
+                    Vt -= ev;
+                    Vt = scalar_visac(TEMP, ev, Vt);
+                    const dv3 gP = ld3(a.tgx, a.tgy, a.tgz, c0);
+                    const double PhiFace = a.tphi[c0] + dot(gP, Xpn);
+                    const double VisFaceT = Vt * g.frlc[f];
+                    const double fde = Vt * dot(gP, nrm);
+                    const double fdi = VisFaceT * dot(gP, Xpn);
+                    St = St + fde - fdi;
+                    if (last) a.tphi[N + side] = PhiFace;
+                }
                 double Visac = ev;
                 Visac -= ev;
                 Visac = scalar_visac(a.type, ev, Visac);
@@ -1051,6 +1103,7 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 }
             } else if (bt == WALL) {
                 if (last) {
+                    if constexpr (TEM) a.tphi[N + side] = a.tphi[c0];
 #pragma unroll
                     for (int s = 0; s < SB; s++) a.phi[s][N + side] = a.phi[s][c0];
                 }
@@ -1058,6 +1111,15 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
         }
     }
     const double fv = g.rho[c] * g.vol[c] * a.rdelta;
+    if constexpr (TEM) {
+        const double pt = a.tphi[c];
+        St += fv * pt;
+        At += fv;
+        for (int k = 0; k < nint; k++) At -= own_fft[k];
+        At *= (1. / a.urf);
+        St = St + (1.0 - a.urf) * At * pt;
+        a.tS[c] = St;
+    }
     double phic[SB];
 #pragma unroll
     for (int s = 0; s < SB; s++) {
@@ -1078,10 +1140,18 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
     const int rp = g.row_ptr[c];
     a.vals[rp + dpos] = A;
     for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+    if constexpr (TEM) {
+        a.tvals[rp + dpos] = At;
+        for (int k = 0; k < nint; k++) a.tvals[rp + pos[k]] = own_fft[k];
+    }
 }
 
-// FUL, PRO, VARF, VARP (FlowSolver.inl:3243-3260) as one 4-RHS solve.
-int scalar_batch_solve(Ctx *c)
+// FUL, PRO, VARF, VARP (FlowSolver.inl:3243-3260) as one 4-RHS solve; with
+// tem, the temperature equation (:3237-3242) is assembled in the same pass
+// into its own matrix and solved first, as the reference orders them (none of
+// the four reads the temperature, and TEM's assembly reads nothing the TE /
+// ED solves change).
+static int batch_solve(Ctx *c, bool tem)
 {
     static const int types[SB] = {FUEL, PROG, VARFU, VARPR};
     static const int fields[SB] = {FUL, PRO, VARF, VARP};
@@ -1095,10 +1165,23 @@ int scalar_batch_solve(Ctx *c)
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.vals = c->A_vals;
     a.type = FUEL;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
+    a.tphi = c->phi[TEM]; a.tgx = c->gx[TEM]; a.tgy = c->gy[TEM]; a.tgz = c->gz[TEM];
+    a.tS = c->SU; a.tvals = c->A_vals2; a.tinlet = c->k.gas_tem;
     {
-        KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * SB * c->N + 56.0 * c->F + 8.0 * c->nnz);
-        hipLaunchKernelGGL(k_scalar_batch, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+        const int nsc = SB + (tem ? 1 : 0);
+        KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * nsc * c->N + 56.0 * c->F + 8.0 * c->nnz * (tem ? 2 : 1));
+        if (tem)
+            hipLaunchKernelGGL(k_scalar_batch<true>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+        else
+            hipLaunchKernelGGL(k_scalar_batch<false>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
         MC_LAUNCH_CHECK(c);
+    }
+    if (tem) {
+        double *bt[1] = {c->SU}, *xt[1] = {c->phi[TEM]};
+        mc_solve_status stt{};
+        MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt));
+        c->st.scalar_iters += stt.iterations;
+        if (!stt.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     }
     double *b[SB], *x[SB];
     for (int s = 0; s < SB; s++) { b[s] = c->srhs[s]; x[s] = c->phi[fields[s]]; }
@@ -1110,6 +1193,8 @@ int scalar_batch_solve(Ctx *c)
     MC_HIP(c, hipMemcpyAsync(c->SU, c->srhs[SB - 1], (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
     return MC_OK;
 }
+
+int scalar_batch_solve(Ctx *c) { return batch_solve(c, false); }
 
 static int scalar_field(int type)
 {
@@ -1185,6 +1270,10 @@ __global__ void __launch_bounds__(256) k_clip_list(Geo g, const int *list, int c
 
 int flow_workspace(Ctx *c)
 {
+    {
+        MemScope ms(c, MC_MEM_MATRIX);
+        MC_TRY(dalloc(c, &c->A_vals2, c->nnz));
+    }
     for (int s = 0; s < 4; s++) MC_TRY(dalloc(c, &c->srhs[s], c->N));
     MC_TRY(dalloc(c, &c->neg_list, c->N));
     size_t tb = 0;
@@ -1257,8 +1346,7 @@ int simple_iteration(Ctx *c, int inner)
     if (c->opt.solve_scalars) {
         MC_TRY(scalar_solve(c, TERBTE));
         MC_TRY(scalar_solve(c, TERBED));
-        MC_TRY(scalar_solve(c, TEMP));
-        MC_TRY(scalar_batch_solve(c));
+        MC_TRY(batch_solve(c, true));   // TEM, then FUL, PRO, VARF, VARP
     }
     c->st.simple_iterations++;
     c->global_iter++;

@@ -114,6 +114,7 @@ struct Ctx {
     int nnz = 0;
     int *row_ptr = nullptr, *col = nullptr, *diag_idx = nullptr;
     double *A_vals = nullptr;      // momentum / scalar matrix
+    double *A_vals2 = nullptr;     // temperature matrix, assembled beside the passive scalars'
     double *PA_vals = nullptr;     // pressure-correction matrix
     float *PA_vals32 = nullptr;
     double *Pb = nullptr;          // pressure RHS (compat: frozen copy)
