@@ -210,9 +210,10 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
-// Block-wide sum of NV values (blockDim.x == 256), result valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*4 */)
+// Block-wide sum of NV values over W waves (blockDim.x == 64 W), result valid
+// in thread 0; the wave sums are added in wave order.
+template <int NV, int W = 4>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*W */)
 {
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -224,12 +225,16 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *lds /* NV*4 *
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) lds[k * 4 + wid] = v[k];
+        for (int k = 0; k < NV; k++) lds[k * W + wid] = v[k];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) v[k] = ((lds[k * 4 + 0] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+        for (int k = 0; k < NV; k++) {
+            double s = lds[k * W];
+            for (int w = 1; w < W; w++) s += lds[k * W + w];
+            v[k] = s;
+        }
     }
 }
 
@@ -251,12 +256,13 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partials
     }
 }
 
-// Fixed-order sum of the partials of `nblocks` blocks (one block of 256
-// threads); the totals are valid in thread 0.
+// Fixed-order sum of the partials of `nblocks` blocks (one block of
+// FINISH_THREADS threads); the totals are valid in thread 0.
+constexpr int FINISH_THREADS = 1024;
 template <int NV>
 __device__ __forceinline__ void sum_partials(const double *partials, int nblocks, double (&v)[NV])
 {
-    __shared__ double lds[NV * 4];
+    __shared__ double lds[NV * (FINISH_THREADS / 64)];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         const double *pk = partials + (size_t)k * nblocks;
@@ -272,12 +278,12 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
         for (; i < nblocks; i += step) s0 += pk[i];
         v[k] = (s0 + s1) + (s2 + s3);
     }
-    block_sum<NV>(v, lds);
+    block_sum<NV, FINISH_THREADS / 64>(v, lds);
 }
 
-// Finishing kernel: one block of 256 threads; thread 0 runs epi(totals).
+// Finishing kernel: one block of FINISH_THREADS threads; thread 0 runs epi(totals).
 template <int NV, class Epi>
-__global__ void __launch_bounds__(256) k_finish(const double *partials, int nblocks, Epi epi)
+__global__ void __launch_bounds__(FINISH_THREADS) k_finish(const double *partials, int nblocks, Epi epi)
 {
     double v[NV];
     sum_partials<NV>(partials, nblocks, v);
@@ -287,7 +293,7 @@ __global__ void __launch_bounds__(256) k_finish(const double *partials, int nblo
 template <int NV, class Epi>
 inline void launch_finish(hipStream_t s, const double *partials, int nblocks, const Epi &epi)
 {
-    hipLaunchKernelGGL((k_finish<NV, Epi>), dim3(1), dim3(256), 0, s, partials, nblocks, epi);
+    hipLaunchKernelGGL((k_finish<NV, Epi>), dim3(1), dim3(FINISH_THREADS), 0, s, partials, nblocks, epi);
 }
 
 // ------------------------------------------------------------ host utils --

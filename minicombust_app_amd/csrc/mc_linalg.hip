@@ -790,7 +790,8 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
         }
         __syncthreads();
     }
-    for (int e = tid; e < n * n; e += blockDim.x) inv[e] = M[e];
+    // stored transposed: k_dense_apply's threads (rows) then read consecutive words
+    for (int e = tid; e < n * n; e += blockDim.x) inv[(e % n) * n + e / n] = M[e];
 }
 
 int amg_setup(Ctx *c, const double *vals)
@@ -1049,6 +1050,31 @@ __global__ void __launch_bounds__(BLOCK) k_resid_restrict(Mat<T> A, int nc, cons
     if (lane8 == 0 && I < nc) bc[I] = (T)s;
 }
 
+// r = b - A x in row order (diagonal storage: coalesced), then the
+// restriction sums each aggregate's members: two passes that replace the
+// member-ordered fused kernel, whose row gathers follow the aggregates.
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_resid(Mat<T> A, const T *b, const T *x, T *r, int chunk, const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)x[j]; },
+                    [&](int i, const double (&ax)[1]) { r[i] = (T)((double)b[i] - ax[0]); });
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_restrict(int nc, const int *aptr, const int *amem, const T *r, T *bc,
+                                                    const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int I = blockIdx.x * BLOCK + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (int t = aptr[I]; t < aptr[I + 1]; t++) s += (double)r[amem[t]];
+    bc[I] = (T)s;
+}
+
 // coarsest level: x = Ainv b in FP64
 template <typename T>
 __global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, const T *b, T *x, const KState *stop)
@@ -1060,7 +1086,7 @@ __global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, c
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int j = 0; j < n; j++) s += inv[(size_t)i * n + j] * bs[j];
+    for (int j = 0; j < n; j++) s += inv[(size_t)j * n + i] * bs[j];   // inverse stored transposed
     x[i] = (T)s;
 }
 
@@ -1123,8 +1149,16 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     }
     AmgLevel &Lc = c->amg[l + 1];
     T *bc = lv_b<T>(Lc), *xc = lv_x<T>(Lc);
-    MC_DISPATCH_ND(A.nd, (k_resid_restrict<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, A, L.nc,
-                   L.agg_ptr, L.agg_members, b, cur, bc, stop);
+    if (A.nd) {
+        T *r = (cur == x) ? tmp : x;   // free until post-smoothing
+        MC_DISPATCH_ND(A.nd, (k_resid<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, (const T *)cur, r,
+                       g.chunk, stop);
+        hipLaunchKernelGGL(k_restrict<T>, dim3(div_up(L.nc, BLOCK)), dim3(BLOCK), 0, c->stream, L.nc, L.agg_ptr,
+                           L.agg_members, (const T *)r, bc, stop);
+    } else {
+        MC_DISPATCH_ND(A.nd, (k_resid_restrict<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, A,
+                       L.nc, L.agg_ptr, L.agg_members, b, cur, bc, stop);
+    }
     MC_TRY(vcycle<T>(c, l + 1, bc, xc, nullptr, stop));
     const T cs = (T)c->opt.amg_cscale;
     // post-smoothing: the first sweep applies the prolongation on the fly; the
