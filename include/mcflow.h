@@ -80,6 +80,10 @@ typedef struct mc_options {
                               inlet and initial field); 1 = the reference case    */
     double amg_cheb_ratio; /* > 1: the nu smoothing sweeps use Chebyshev weights on
                               [2/ratio, 2]; <= 1: every sweep uses amg_omega      */
+    int    cell_tile;      /* generated box mesh: x extent of the 256-cell brick
+                              (cell_tile x 1 x 256/cell_tile) one workgroup of the
+                              cell-centric assembly kernels visits; a power of two
+                              in [1, 256], 0: the default (64)                   */
 } mc_options;
 
 typedef struct mc_solve_status {

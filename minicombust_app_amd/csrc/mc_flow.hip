@@ -29,6 +29,7 @@ struct Geo {
     const uint8_t *is_wall;
     int btype[6];
     int last_face[6];
+    CellWalk w;
 };
 
 static Geo geo(Ctx *c)
@@ -42,6 +43,7 @@ static Geo geo(Ctx *c)
     g.fnx = c->fnx; g.fny = c->fny; g.fnz = c->fnz; g.farea = c->farea; g.flam = c->flam; g.frlc = c->frlc;
     g.vol = c->vol; g.rho = c->rho; g.is_wall = c->is_wall_cell;
     for (int s = 0; s < 6; s++) { g.btype[s] = c->btype[s]; g.last_face[s] = c->last_face[s]; }
+    g.w = cell_walk(c);
     return g;
 }
 
@@ -82,9 +84,8 @@ struct GradArgs {
 template <int NF>
 __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
 {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int c = b * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
     double ph[NF];
 #pragma unroll
@@ -149,7 +150,7 @@ int grad_multi(Ctx *c)
         a.zero_boundary[k] = fields[k] == P;
     }
     KTimer kt(c, MC_KC_GRAD, grad_bytes(c, 11));
-    hipLaunchKernelGGL(k_grad<11>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    hipLaunchKernelGGL(k_grad<11>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -162,7 +163,7 @@ int grad_single(Ctx *c, int field, bool pressure)
     a.gx[0] = c->gx[field]; a.gy[0] = c->gy[field]; a.gz[0] = c->gz[field];
     a.zero_boundary[0] = pressure ? 1 : 0;
     KTimer kt(c, MC_KC_GRAD1, grad_bytes(c, 1));
-    hipLaunchKernelGGL(k_grad<1>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    hipLaunchKernelGGL(k_grad<1>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -171,8 +172,8 @@ int grad_single(Ctx *c, int field, bool pressure)
 // precomp_AU (FlowSolver.inl:1065-1104)
 __global__ void __launch_bounds__(256) k_precomp_AU(Geo g, const double *mflux, double *AU, double ev, double rdelta)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     double A = 0.0;
     for (int k = 0; k < 6; k++) {
         const int f = g.forder[6 * c + k];
@@ -192,7 +193,7 @@ __global__ void __launch_bounds__(256) k_precomp_AU(Geo g, const double *mflux, 
 
 int precomp_AU(Ctx *c)
 {
-    hipLaunchKernelGGL(k_precomp_AU, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), c->mflux, c->AU, c->k.ev,
+    hipLaunchKernelGGL(k_precomp_AU, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), c->mflux, c->AU, c->k.ev,
                        c->k.rdelta);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -212,27 +213,39 @@ struct MomArgs {
 // three momentum systems.
 __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     const int N = g.N;
     const double GammaBlend = 0.0;
     double SU = 0.0, SV = 0.0, SW = 0.0, A = 0.0;
     double own_ff[6];
     int own_nb[6];
     int nint = 0;
+    // the cell's own values, read once (each face reads the other cell's)
+    const dv3 gUc = ld3(a.gUx, a.gUy, a.gUz, c), gVc = ld3(a.gVx, a.gVy, a.gVz, c), gWc = ld3(a.gWx, a.gWy, a.gWz, c);
+    const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
+    const double Uc = a.U[c], Vc = a.V[c], Wc = a.W[c];
     for (int k = 0; k < 6; k++) {
         const int f = g.forder[6 * c + k];
-        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const int o = g.fother[6 * c + k];
+        const bool own0 = o >= 0;          // c is the face's cell0
+        const int nb = o & 0x7fffffff;     // cell across the face, or N + side
         const double mf = a.mflux[f];
-        if (c1 < N) {
+        if (nb < N) {
             const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
-            const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0) * lambda0 + ld3(a.gUx, a.gUy, a.gUz, c1) * lambda1;
-            const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0) * lambda0 + ld3(a.gVx, a.gVy, a.gVz, c1) * lambda1;
-            const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0) * lambda0 + ld3(a.gWx, a.gWy, a.gWz, c1) * lambda1;
+            const dv3 gUn = ld3(a.gUx, a.gUy, a.gUz, nb), gVn = ld3(a.gVx, a.gVy, a.gVz, nb),
+                      gWn = ld3(a.gWx, a.gWy, a.gWz, nb);
+            const dv3 dUdXac = (own0 ? gUc : gUn) * lambda0 + (own0 ? gUn : gUc) * lambda1;
+            const dv3 dVdXac = (own0 ? gVc : gVn) * lambda0 + (own0 ? gVn : gVc) * lambda1;
+            const dv3 dWdXac = (own0 ? gWc : gWn) * lambda0 + (own0 ? gWn : gWc) * lambda1;
             const double Visac = a.ev * lambda0 + a.ev * lambda1;
             const double VisFace = Visac * g.frlc[f];
-            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
-            const double U0 = a.U[c0], U1 = a.U[c1], V0 = a.V[c0], V1 = a.V[c1], W0 = a.W[c0], W1 = a.W[c1];
+            const dv3 cn = ld3(g.cx, g.cy, g.cz, nb);
+            const dv3 Xpn = own0 ? (cn - cc) : (cc - cn);
+            const double Un = a.U[nb], Vn = a.V[nb], Wn = a.W[nb];
+            const double U0 = own0 ? Uc : Un, U1 = own0 ? Un : Uc;
+            const double V0 = own0 ? Vc : Vn, V1 = own0 ? Vn : Vc;
+            const double W0 = own0 ? Wc : Wn, W1 = own0 ? Wn : Wc;
             double UFace, VFace, WFace;
             if (mf >= 0.0) { UFace = U0; VFace = V0; WFace = W0; }
             else { UFace = U1; VFace = V1; WFace = W1; }
@@ -253,34 +266,32 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
             const double blend_u = GammaBlend * (fuce - fuci);
             const double blend_v = GammaBlend * (fvce - fvci);
             const double blend_w = GammaBlend * (fwce - fwci);
-            if (c == c0) {
+            if (own0) {
                 SU = SU - blend_u + fude - fudi;
                 SV = SV - blend_v + fvde - fvdi;
                 SW = SW - blend_w + fwde - fwdi;
-                own_ff[nint] = ff1; own_nb[nint] = c1;
+                own_ff[nint] = ff1;
                 a.ff0[f] = ff0; a.ff1[f] = ff1;
             } else {
                 SU = SU + blend_u - fude + fudi;
                 SV = SV + blend_v - fvde + fvdi;
                 SW = SW + blend_w - fwde + fwdi;
-                own_ff[nint] = ff0; own_nb[nint] = c0;
+                own_ff[nint] = ff0;
             }
+            own_nb[nint] = nb;
             nint++;
         } else {
-            const int side = c1 - N;
+            const int side = nb - N;
             const int bt = g.btype[side];
-            const dv3 cc = ld3(g.cx, g.cy, g.cz, c0);
             const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
             const bool last = (f == g.last_face[side]);
             if (bt == INLET || bt == OUTLET) {
-                const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0);
-                const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0);
-                const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0);
+                const dv3 dUdXac = gUc, dVdXac = gVc, dWdXac = gWc;
                 double UFace, VFace, WFace, Visac;
                 if (bt == INLET) {
                     UFace = a.vel_x; VFace = a.vel_y; WFace = a.vel_z; Visac = a.inlet_ev;
                 } else {
-                    Visac = a.ev; UFace = a.U[c0]; VFace = a.V[c0]; WFace = a.W[c0];
+                    Visac = a.ev; UFace = Uc; VFace = Vc; WFace = Wc;
                 }
                 const double VisFace = Visac * g.frlc[f];
                 const double sx = g.fnx[f], sy = g.fny[f], sz = g.fnz[f];
@@ -301,7 +312,7 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
             } else if (bt == WALL) {
                 const double Visac = a.ev;
                 const double coef = Visac * g.frlc[f];
-                const double U0 = a.U[c0], V0 = a.V[c0], W0 = a.W[c0];
+                const double U0 = Uc, V0 = Vc, W0 = Wc;
                 const dv3 Up = v3(U0 - 0., V0 - 0., W0 - 0.);
                 const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
                 const dv3 nh = n / mag(n);
@@ -331,7 +342,6 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     SV -= a.gPy[c] * vol;
     SW -= a.gPz[c] * vol;
     const double fv = rho * vol * a.rdelta;
-    const double Uc = a.U[c], Vc = a.V[c], Wc = a.W[c];
     SU += fv * Uc; SV += fv * Vc; SW += fv * Wc;
     A += fv;
     // setup_sparse_matrix / update_sparse_matrix
@@ -368,7 +378,7 @@ static MomArgs mom_args(Ctx *c)
 int momentum_assemble(Ctx *c)
 {
     KTimer kt(c, MC_KC_MOM_ASM, 216.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
-    hipLaunchKernelGGL(k_momentum, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), mom_args(c));
+    hipLaunchKernelGGL(k_momentum, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), mom_args(c));
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -465,21 +475,38 @@ struct MassArgs {
     int into_pressure; // write the pressure matrix and RHS
 };
 
-// Rhie-Chow interior mass flux (FlowSolver.inl:1893-1950).
-__device__ __forceinline__ double rhie_chow(const Geo &g, const MassArgs &a, int f, int c0, int c1, double &ApV_out)
+// The per-cell values the Rhie-Chow flux reads on either side of a face.
+struct RCCell {
+    dv3 gU, gV, gW, gP, cc;
+    double U, V, W, P, rho, AU, vol;
+};
+
+__device__ __forceinline__ RCCell rc_cell(const Geo &g, const MassArgs &a, int i)
+{
+    RCCell q;
+    q.gU = ld3(a.gUx, a.gUy, a.gUz, i); q.gV = ld3(a.gVx, a.gVy, a.gVz, i); q.gW = ld3(a.gWx, a.gWy, a.gWz, i);
+    q.gP = ld3(a.gPx, a.gPy, a.gPz, i); q.cc = ld3(g.cx, g.cy, g.cz, i);
+    q.U = a.U[i]; q.V = a.V[i]; q.W = a.W[i]; q.P = a.P[i];
+    q.rho = g.rho[i]; q.AU = a.AU[i]; q.vol = g.vol[i];
+    return q;
+}
+
+// Rhie-Chow interior mass flux (FlowSolver.inl:1893-1950); q0, q1: the
+// face's cell0 and cell1.
+__device__ __forceinline__ double rhie_chow(const Geo &g, int f, const RCCell &q0, const RCCell &q1, double &ApV_out)
 {
     const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
-    const dv3 dUdXac = ld3(a.gUx, a.gUy, a.gUz, c0) * lambda0 + ld3(a.gUx, a.gUy, a.gUz, c1) * lambda1;
-    const dv3 dVdXac = ld3(a.gVx, a.gVy, a.gVz, c0) * lambda0 + ld3(a.gVx, a.gVy, a.gVz, c1) * lambda1;
-    const dv3 dWdXac = ld3(a.gWx, a.gWy, a.gWz, c0) * lambda0 + ld3(a.gWx, a.gWy, a.gWz, c1) * lambda1;
-    const dv3 cc0 = ld3(g.cx, g.cy, g.cz, c0), cc1 = ld3(g.cx, g.cy, g.cz, c1);
+    const dv3 dUdXac = q0.gU * lambda0 + q1.gU * lambda1;
+    const dv3 dVdXac = q0.gV * lambda0 + q1.gV * lambda1;
+    const dv3 dWdXac = q0.gW * lambda0 + q1.gW * lambda1;
+    const dv3 cc0 = q0.cc, cc1 = q1.cc;
     const dv3 Xac = cc1 * lambda1 + cc0 * lambda0;
     const dv3 fc = ld3(g.fcx, g.fcy, g.fcz, f);
     const dv3 delta = fc - Xac;
-    const double UFace = a.U[c1] * lambda1 + a.U[c0] * lambda0 + dot(dUdXac, delta);
-    const double VFace = a.V[c1] * lambda1 + a.V[c0] * lambda0 + dot(dVdXac, delta);
-    const double WFace = a.W[c1] * lambda1 + a.W[c0] * lambda0 + dot(dWdXac, delta);
-    const double rho0 = g.rho[c0], rho1 = g.rho[c1];
+    const double UFace = q1.U * lambda1 + q0.U * lambda0 + dot(dUdXac, delta);
+    const double VFace = q1.V * lambda1 + q0.V * lambda0 + dot(dVdXac, delta);
+    const double WFace = q1.W * lambda1 + q0.W * lambda0 + dot(dWdXac, delta);
+    const double rho0 = q0.rho, rho1 = q1.rho;
     const double densityf = rho0 * lambda0 + rho1 * lambda1;
     const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
     double m = densityf * (UFace * n.x + VFace * n.y + WFace * n.z);
@@ -487,16 +514,16 @@ __device__ __forceinline__ double rhie_chow(const Geo &g, const MassArgs &a, int
     const dv3 Xpac = fc - nh * dot(fc - cc0, nh);
     const dv3 Xnac = fc - nh * dot(fc - cc1, nh);
     const dv3 delp = Xpac - cc0, deln = Xnac - cc1;
-    const dv3 gP0 = ld3(a.gPx, a.gPy, a.gPz, c0), gP1 = ld3(a.gPx, a.gPy, a.gPz, c1);
-    const double cell0_P = a.P[c0] + dot(gP0, delp);
-    const double cell1_P = a.P[c1] + dot(gP1, deln);
+    const dv3 gP0 = q0.gP, gP1 = q1.gP;
+    const double cell0_P = q0.P + dot(gP0, delp);
+    const double cell1_P = q1.P + dot(gP1, deln);
     const dv3 Xpn = Xnac - Xpac;
     const dv3 Xpn2 = cc1 - cc0;
-    const double A0 = a.AU[c0], A1 = a.AU[c1];
+    const double A0 = q0.AU, A1 = q1.AU;
     const double ApV0 = (A0 != 0.0) ? 1.0 / A0 : 0.0;
     const double ApV1 = (A1 != 0.0) ? 1.0 / A1 : 0.0;
     double ApV = rho0 * ApV0 * lambda0 + rho1 * ApV1 * lambda1;
-    const double volume_avg = g.vol[c0] * lambda0 + g.vol[c1] * lambda1;
+    const double volume_avg = q0.vol * lambda0 + q1.vol * lambda1;
     ApV = ApV * g.farea[f] * volume_avg / dot(Xpn2, nh);
     const double dpx = (gP1.x * lambda1 + gP0.x * lambda0) * Xpn.x;
     const double dpy = (gP1.y * lambda1 + gP0.y * lambda0) * Xpn.y;
@@ -512,37 +539,41 @@ __device__ __forceinline__ double rhie_chow(const Geo &g, const MassArgs &a, int
 // then the interior divergence of setup_pressure_matrix.
 __global__ void __launch_bounds__(256) k_mass_pressure(Geo g, MassArgs a)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     const int N = g.N;
     double S = 0.0, AV = 0.0;
-    int fo[6];
-    for (int k = 0; k < 6; k++) fo[k] = g.forder[6 * c + k];
+    int fo[6], ot[6];
+    for (int k = 0; k < 6; k++) { fo[k] = g.forder[6 * c + k]; ot[k] = g.fother[6 * c + k]; }
     for (int k = 0; k < 6; k++) {
-        const int f = fo[k];
-        if (g.fc1[f] >= N && g.btype[g.fc1[f] - N] == INLET) S = S - a.mflux[f];
+        const int nb = ot[k] & 0x7fffffff;
+        if (nb >= N && g.btype[nb - N] == INLET) S = S - a.mflux[fo[k]];
     }
     for (int k = 0; k < 6; k++) {
-        const int f = fo[k];
-        if (g.fc1[f] >= N && g.btype[g.fc1[f] - N] == OUTLET) S -= a.mflux[f];
+        const int nb = ot[k] & 0x7fffffff;
+        if (nb >= N && g.btype[nb - N] == OUTLET) S -= a.mflux[fo[k]];
     }
-    double own[6];
+    const RCCell own = rc_cell(g, a, c);   // read once; each face reads the other cell
+    double own_v[6];
     int own_nb[6];
     int nint = 0;
     for (int k = 0; k < 6; k++) {
         const int f = fo[k];
-        const int c0 = g.fc0[f], c1 = g.fc1[f];
-        if (c1 >= N) continue;
+        const bool own0 = ot[k] >= 0;
+        const int nb = ot[k] & 0x7fffffff;
+        if (nb >= N) continue;
+        const RCCell other = rc_cell(g, a, nb);
         double ApV;
-        const double m = rhie_chow(g, a, f, c0, c1, ApV);
-        if (c == c0) {
+        const double m = own0 ? rhie_chow(g, f, own, other, ApV) : rhie_chow(g, f, other, own, ApV);
+        if (own0) {
             a.mflux[f] = m;
             a.ff0[f] = -ApV; a.ff1[f] = -ApV;
         }
         if (a.assemble) {
-            if (c == c0) { S -= m; AV -= -ApV; own_nb[nint] = c1; }
-            else { S += m; AV -= -ApV; own_nb[nint] = c0; }
-            own[nint] = -ApV;
+            if (own0) { S -= m; AV -= -ApV; }
+            else { S += m; AV -= -ApV; }
+            own_nb[nint] = nb;
+            own_v[nint] = -ApV;
             nint++;
         }
     }
@@ -555,7 +586,7 @@ __global__ void __launch_bounds__(256) k_mass_pressure(Geo g, MassArgs a)
         row_positions(c, nint, own_nb, pos, dpos);
         const int rp = g.row_ptr[c];
         a.PAvals[rp + dpos] = AV;
-        for (int k = 0; k < nint; k++) a.PAvals[rp + pos[k]] = own[k];
+        for (int k = 0; k < nint; k++) a.PAvals[rp + pos[k]] = own_v[k];
     }
 }
 
@@ -578,7 +609,7 @@ int set_up_field(Ctx *c)
     // precomp_AU reads the mass fluxes left by the constructor (all zero)
     MC_TRY(precomp_AU(c));
     MC_TRY(boundary_mass_flux(c));
-    hipLaunchKernelGGL(k_mass_pressure, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), mass_args(c, 0, 0));
+    hipLaunchKernelGGL(k_mass_pressure, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), mass_args(c, 0, 0));
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -588,7 +619,7 @@ int calculate_mass_flux_assemble(Ctx *c)
     const bool into = c->first_pressure || c->opt.mode == MC_MODE_SIMPLE;
     MC_TRY(boundary_mass_flux(c));
     KTimer kt(c, MC_KC_P_ASM, 184.0 * c->N + 88.0 * c->F + 8.0 * c->nnz);
-    hipLaunchKernelGGL(k_mass_pressure, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c),
+    hipLaunchKernelGGL(k_mass_pressure, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c),
                        mass_args(c, 1, into ? 1 : 0));
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -637,15 +668,15 @@ __global__ void __launch_bounds__(256) k_cell_correct(Geo g, const double *fcorr
                                                       const double *gPPy, const double *gPPz, const double *AU,
                                                       int which)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     if (c < 6) PP[g.N + c] = 0.0;
-    if (c >= g.N) return;
     double S = 0.0;
     for (int k = 0; k < 6; k++) {
         const int f = g.forder[6 * c + k];
-        const int c0 = g.fc0[f], c1 = g.fc1[f];
-        if (c1 >= g.N) continue;
-        if (c == c0) S -= fcorr[f];
+        const int o = g.fother[6 * c + k];
+        if ((o & 0x7fffffff) >= g.N) continue;
+        if (o >= 0) S -= fcorr[f];   // c is the face's cell0
         else S += fcorr[f];
     }
     SU[c] = S;
@@ -752,7 +783,7 @@ int calculate_pressure(Ctx *c)
                                    c->phi[PP], c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
             }
             const int which = simple ? 2 : (c->first_pressure ? 1 : 0);
-            hipLaunchKernelGGL(k_cell_correct, dim3(div_up(N, 256)), dim3(256), 0, c->stream, geo(c), c->fcorr, c->SU,
+            hipLaunchKernelGGL(k_cell_correct, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), c->fcorr, c->SU,
                                c->phi[P], c->phi[PP], c->phi[U], c->phi[V], c->phi[W], c->gx[PP], c->gy[PP],
                                c->gz[PP], c->AU, which);
             MC_LAUNCH_CHECK(c);
@@ -793,8 +824,8 @@ __device__ __forceinline__ double scalar_visac(int type, double ev, double Visac
 // solveTurbulenceModels (:2746-2883) and setup_sparse_matrix (:1186-1297).
 __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     const int N = g.N;
     const double ev = a.ev;
     const double GammaBlend = 0.0;
@@ -804,21 +835,28 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     double own_ff[6];
     int own_nb[6];
     int nint = 0;
-    int fo[6];
-    for (int k = 0; k < 6; k++) fo[k] = g.forder[6 * c + k];
+    int fo[6], ot[6];
+    for (int k = 0; k < 6; k++) { fo[k] = g.forder[6 * c + k]; ot[k] = g.fother[6 * c + k]; }
+    // the cell's own values, read once: each face takes the other cell's
+    // from memory and these for its own side
+    const dv3 gc = ld3(a.gx, a.gy, a.gz, c), cc = ld3(g.cx, g.cy, g.cz, c);
+    const double pc = a.phi[c];
     for (int k = 0; k < 6; k++) {
         const int f = fo[k];
-        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const bool own0 = ot[k] >= 0;            // c is the face's cell0
+        const int nb = ot[k] & 0x7fffffff;       // cell across the face, or N + side
         const double mf = a.mflux[f];
-        if (c1 < N) {
+        if (nb < N) {
             const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
             double Visac = ev * lambda0 + ev * lambda1;
             Visac -= ev;
             Visac = scalar_visac(a.type, ev, Visac);
-            const dv3 dPhiXac = ld3(a.gx, a.gy, a.gz, c0) * lambda0 + ld3(a.gx, a.gy, a.gz, c1) * lambda1;
-            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 gn = ld3(a.gx, a.gy, a.gz, nb), cn = ld3(g.cx, g.cy, g.cz, nb);
+            const dv3 dPhiXac = (own0 ? gc : gn) * lambda0 + (own0 ? gn : gc) * lambda1;
+            const dv3 Xpn = own0 ? (cn - cc) : (cc - cn);
             const double VisFace = Visac * g.frlc[f];
-            const double p0 = a.phi[c0], p1 = a.phi[c1];
+            const double pn = a.phi[nb];
+            const double p0 = own0 ? pc : pn, p1 = own0 ? pn : pc;
             const double PhiFace = (mf >= 0.0) ? p0 : p1;
             const double fce = mf * PhiFace;
             const double fde1 = Visac * dot(dPhiXac, ld3(g.fnx, g.fny, g.fnz, f));
@@ -828,24 +866,26 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
             double ff1 = -VisFace + dmin(mf, 0.0);
             const double blend = GammaBlend * (fce - fci);
             if (ed) {  // FlowSolver.inl:2855-2875: rows of wall cells lose their off-diagonals
-                if (g.is_wall[c0]) ff1 = 0.0;
-                if (g.is_wall[c1]) ff0 = 0.0;
+                const bool wn = g.is_wall[nb] != 0;
+                if (own0 ? wallcell : wn) ff1 = 0.0;
+                if (own0 ? wn : wallcell) ff0 = 0.0;
             }
-            if (c == c0) {
+            if (own0) {
                 S = S - blend + fde1 - fdi;
-                own_ff[nint] = ff1; own_nb[nint] = c1;
+                own_ff[nint] = ff1;
                 a.ff0[f] = ff0; a.ff1[f] = ff1;
             } else {
                 S = S + blend - fde1 + fdi;
-                own_ff[nint] = ff0; own_nb[nint] = c0;
+                own_ff[nint] = ff0;
             }
+            own_nb[nint] = nb;
             nint++;
         } else {
-            const int side = c1 - N;
+            const int side = nb - N;
             const int bt = g.btype[side];
             const bool last = (f == g.last_face[side]);
-            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
-            const dv3 gP = ld3(a.gx, a.gy, a.gz, c0);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+            const dv3 gP = gc;
             if (bt == INLET) {
                 const double PhiFace = a.inlet_phi;
                 double Visac = a.inlet_ev;
@@ -862,19 +902,19 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
                 double Visac = ev;
                 Visac -= ev;
                 Visac = scalar_visac(a.type, ev, Visac);
-                const double PhiFace = a.phi[c0] + dot(gP, Xpn);
+                const double PhiFace = pc + dot(gP, Xpn);
                 const double VisFace = Visac * g.frlc[f];
                 const double fde = Visac * dot(gP, ld3(g.fnx, g.fny, g.fnz, f));
                 const double fdi = VisFace * dot(gP, Xpn);
                 S = S + fde - fdi;
                 if (last) a.phi[N + side] = PhiFace;
             } else if (bt == WALL) {
-                if (last && !ed) a.phi[N + side] = a.phi[c0];
+                if (last && !ed) a.phi[N + side] = pc;
             }
         }
     }
     const double vol = g.vol[c], rho = g.rho[c];
-    double phic = a.phi[c];
+    double phic = pc;
     const double fv = rho * vol * a.rdelta;
     S += fv * phic;
     A += fv;
@@ -892,10 +932,10 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
         if (wallcell) {
             for (int k = 0; k < 6; k++) {
                 const int f = fo[k];
-                const int c1 = g.fc1[f];
+                const int c1 = ot[k] & 0x7fffffff;
                 if (c1 < N || g.btype[c1 - N] != WALL) continue;
                 S = S - TP * vol;
-                const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c);
+                const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
                 const dv3 Up = v3(a.U[c] - 0., a.V[c] - 0., a.W[c] - 0.);
                 const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
                 const dv3 nh = n / mag(n);
@@ -922,9 +962,9 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
         if (wallcell) {
             for (int k = 0; k < 6; k++) {
                 const int f = fo[k];
-                const int c1 = g.fc1[f];
+                const int c1 = ot[k] & 0x7fffffff;
                 if (c1 < N || g.btype[c1 - N] != WALL) continue;
-                const double distance = mag(ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c));
+                const double distance = mag(ld3(g.fcx, g.fcy, g.fcz, f) - cc);
                 const double Dis = a.cmu75 * pow(TEc, 1.5) / (distance * 0.419);
                 phic = Dis;
                 S = Dis;
@@ -977,8 +1017,8 @@ struct ScalBatchArgs {
 template <bool TEM>
 __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
 {
-    const int c = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (c >= g.N) return;
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
     const int N = g.N;
     const double ev = a.ev;
     const double GammaBlend = 0.0;
@@ -989,16 +1029,28 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
     int own_nb[6];
     int nint = 0;
     double St = 0.0, At = 0.0, own_fft[6];
+    // the cell's own values, read once (each face reads the other cell's)
+    const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
+    dv3 gs[SB];
+    double ps[SB];
+#pragma unroll
+    for (int s = 0; s < SB; s++) { gs[s] = ld3(a.gx[s], a.gy[s], a.gz[s], c); ps[s] = a.phi[s][c]; }
+    dv3 gt = v3(0.0, 0.0, 0.0);
+    double pt = 0.0;
+    if constexpr (TEM) { gt = ld3(a.tgx, a.tgy, a.tgz, c); pt = a.tphi[c]; }
     for (int k = 0; k < 6; k++) {
         const int f = g.forder[6 * c + k];
-        const int c0 = g.fc0[f], c1 = g.fc1[f];
+        const int o = g.fother[6 * c + k];
+        const bool own0 = o >= 0;          // c is the face's cell0
+        const int nb = o & 0x7fffffff;     // cell across the face, or N + side
         const double mf = a.mflux[f];
-        if (c1 < N) {
+        if (nb < N) {
             const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
             double Visac = ev * lambda0 + ev * lambda1;
             Visac -= ev;
             Visac = scalar_visac(a.type, ev, Visac);
-            const dv3 Xpn = ld3(g.cx, g.cy, g.cz, c1) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 cn = ld3(g.cx, g.cy, g.cz, nb);
+            const dv3 Xpn = own0 ? (cn - cc) : (cc - cn);
             const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
             const double VisFace = Visac * g.frlc[f];
             const double ff0 = -VisFace - dmax(mf, 0.0);
@@ -1007,9 +1059,11 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 double Vt = ev * lambda0 + ev * lambda1;
                 Vt -= ev;
                 Vt = scalar_visac(TEMP, ev, Vt);
-                const dv3 dPhiXac = ld3(a.tgx, a.tgy, a.tgz, c0) * lambda0 + ld3(a.tgx, a.tgy, a.tgz, c1) * lambda1;
+                const dv3 gn = ld3(a.tgx, a.tgy, a.tgz, nb);
+                const dv3 dPhiXac = (own0 ? gt : gn) * lambda0 + (own0 ? gn : gt) * lambda1;
                 const double VisFaceT = Vt * g.frlc[f];
-                const double p0 = a.tphi[c0], p1 = a.tphi[c1];
+                const double pn = a.tphi[nb];
+                const double p0 = own0 ? pt : pn, p1 = own0 ? pn : pt;
                 const double PhiFace = (mf >= 0.0) ? p0 : p1;
                 const double fce = mf * PhiFace;
                 const double fde1 = Vt * dot(dPhiXac, nrm);
@@ -1018,34 +1072,37 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 const double t0 = -VisFaceT - dmax(mf, 0.0);
                 const double t1 = -VisFaceT + dmin(mf, 0.0);
                 const double blend = GammaBlend * (fce - fci);
-                if (c == c0) { St = St - blend + fde1 - fdi; own_fft[nint] = t1; }
+                if (own0) { St = St - blend + fde1 - fdi; own_fft[nint] = t1; }
                 else { St = St + blend - fde1 + fdi; own_fft[nint] = t0; }
             }
 #pragma unroll
             for (int s = 0; s < SB; s++) {
-                const dv3 dPhiXac = ld3(a.gx[s], a.gy[s], a.gz[s], c0) * lambda0 + ld3(a.gx[s], a.gy[s], a.gz[s], c1) * lambda1;
-                const double p0 = a.phi[s][c0], p1 = a.phi[s][c1];
+                const dv3 gn = ld3(a.gx[s], a.gy[s], a.gz[s], nb);
+                const dv3 dPhiXac = (own0 ? gs[s] : gn) * lambda0 + (own0 ? gn : gs[s]) * lambda1;
+                const double pn = a.phi[s][nb];
+                const double p0 = own0 ? ps[s] : pn, p1 = own0 ? pn : ps[s];
                 const double PhiFace = (mf >= 0.0) ? p0 : p1;
                 const double fce = mf * PhiFace;
                 const double fde1 = Visac * dot(dPhiXac, nrm);
                 const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
                 const double fdi = VisFace * dot(dPhiXac, Xpn);
                 const double blend = GammaBlend * (fce - fci);
-                if (c == c0) S[s] = S[s] - blend + fde1 - fdi;
+                if (own0) S[s] = S[s] - blend + fde1 - fdi;
                 else S[s] = S[s] + blend - fde1 + fdi;
             }
-            if (c == c0) {
-                own_ff[nint] = ff1; own_nb[nint] = c1;
+            if (own0) {
+                own_ff[nint] = ff1;
                 a.ff0[f] = ff0; a.ff1[f] = ff1;
             } else {
-                own_ff[nint] = ff0; own_nb[nint] = c0;
+                own_ff[nint] = ff0;
             }
+            own_nb[nint] = nb;
             nint++;
         } else {
-            const int side = c1 - N;
+            const int side = nb - N;
             const int bt = g.btype[side];
             const bool last = (f == g.last_face[side]);
-            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - ld3(g.cx, g.cy, g.cz, c0);
+            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
             const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
             if (bt == INLET) {
                 if constexpr (TEM) {
@@ -1053,7 +1110,7 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                     Vt -= ev;
                     Vt = scalar_visac(TEMP, ev, Vt);
                     const double VisFaceT = Vt * g.frlc[f];
-                    const dv3 gP = ld3(a.tgx, a.tgy, a.tgz, c0);
+                    const dv3 gP = gt;
                     const double fde = Vt * dot(gP, nrm);
                     const double fdi = VisFaceT * dot(gP, Xpn);
                     const double fb = -VisFaceT + dmin(mf, 0.0);
@@ -1069,7 +1126,7 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 A = A - fb;
 #pragma unroll
                 for (int s = 0; s < SB; s++) {
-                    const dv3 gP = ld3(a.gx[s], a.gy[s], a.gz[s], c0);
+                    const dv3 gP = gs[s];
                     const double fde = Visac * dot(gP, nrm);
                     const double fdi = VisFace * dot(gP, Xpn);
                     S[s] = S[s] - fb * a.inlet_phi[s] + fde - fdi;
@@ -1080,8 +1137,8 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                     double Vt = ev;
                     Vt -= ev;
                     Vt = scalar_visac(TEMP, ev, Vt);
-                    const dv3 gP = ld3(a.tgx, a.tgy, a.tgz, c0);
-                    const double PhiFace = a.tphi[c0] + dot(gP, Xpn);
+                    const dv3 gP = gt;
+                    const double PhiFace = pt + dot(gP, Xpn);
                     const double VisFaceT = Vt * g.frlc[f];
                     const double fde = Vt * dot(gP, nrm);
                     const double fdi = VisFaceT * dot(gP, Xpn);
@@ -1094,8 +1151,8 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 const double VisFace = Visac * g.frlc[f];
 #pragma unroll
                 for (int s = 0; s < SB; s++) {
-                    const dv3 gP = ld3(a.gx[s], a.gy[s], a.gz[s], c0);
-                    const double PhiFace = a.phi[s][c0] + dot(gP, Xpn);
+                    const dv3 gP = gs[s];
+                    const double PhiFace = ps[s] + dot(gP, Xpn);
                     const double fde = Visac * dot(gP, nrm);
                     const double fdi = VisFace * dot(gP, Xpn);
                     S[s] = S[s] + fde - fdi;
@@ -1103,16 +1160,15 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
                 }
             } else if (bt == WALL) {
                 if (last) {
-                    if constexpr (TEM) a.tphi[N + side] = a.tphi[c0];
+                    if constexpr (TEM) a.tphi[N + side] = pt;
 #pragma unroll
-                    for (int s = 0; s < SB; s++) a.phi[s][N + side] = a.phi[s][c0];
+                    for (int s = 0; s < SB; s++) a.phi[s][N + side] = ps[s];
                 }
             }
         }
     }
     const double fv = g.rho[c] * g.vol[c] * a.rdelta;
     if constexpr (TEM) {
-        const double pt = a.tphi[c];
         St += fv * pt;
         At += fv;
         for (int k = 0; k < nint; k++) At -= own_fft[k];
@@ -1123,7 +1179,7 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
     double phic[SB];
 #pragma unroll
     for (int s = 0; s < SB; s++) {
-        phic[s] = a.phi[s][c];
+        phic[s] = ps[s];
         S[s] += fv * phic[s];
     }
     A += fv;
@@ -1171,9 +1227,9 @@ static int batch_solve(Ctx *c, bool tem)
         const int nsc = SB + (tem ? 1 : 0);
         KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * nsc * c->N + 56.0 * c->F + 8.0 * c->nnz * (tem ? 2 : 1));
         if (tem)
-            hipLaunchKernelGGL(k_scalar_batch<true>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+            hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
         else
-            hipLaunchKernelGGL(k_scalar_batch<false>, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+            hipLaunchKernelGGL(k_scalar_batch<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
         MC_LAUNCH_CHECK(c);
     }
     if (tem) {
@@ -1226,7 +1282,7 @@ int scalar_assemble(Ctx *c, int type)
     default: a.inlet_phi = 0.0; break;
     }
     KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
-    hipLaunchKernelGGL(k_scalar, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, geo(c), a);
+    hipLaunchKernelGGL(k_scalar, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -1246,8 +1302,9 @@ __global__ void __launch_bounds__(256) k_flag_neg(int n, const double *v, uint8_
 }
 
 __global__ void __launch_bounds__(256) k_clip_list(Geo g, const int *list, int count, const double *orig, double *phi,
-                                                   int *changed)
+                                                   const int *prev, int *changed)
 {
+    if (prev && *prev == 0) return;   // the previous pass changed nothing: fixed point reached
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const int c = list[i];
@@ -1284,12 +1341,19 @@ int flow_workspace(Ctx *c)
     return MC_OK;
 }
 
-static int clip_nonneg(Ctx *c, double *phi)
+// Passes are launched in batches with one host check per batch; a pass
+// whose predecessor changed nothing returns at once, so a batch longer than
+// needed costs only launches.  The batch length follows the passes the same
+// field needed last time.
+constexpr int CLIP_BATCH_MAX = 64;
+
+static int clip_nonneg(Ctx *c, double *phi, int slot)
 {
     const int N = c->N;
     int *dcount = (int *)(c->dscal + 700);
     int *hcount = (int *)(c->hscal + 700);
-    int *changed = (int *)(c->dscal + 704);   // 4 pass counters
+    int *changed = (int *)(c->dscal + 704);   // CLIP_BATCH_MAX pass counters
+    int *hchanged = (int *)(c->hscal + 704);
     uint8_t *flag = (uint8_t *)c->wk[13];
     hipLaunchKernelGGL(k_flag_neg, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, phi, flag);
     MC_LAUNCH_CHECK(c);
@@ -1303,17 +1367,25 @@ static int clip_nonneg(Ctx *c, double *phi)
     if (K == 0) return MC_OK;
     double *orig = c->wk[14];
     MC_HIP(c, hipMemcpyAsync(orig, phi, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+    int B = std::min(std::max(c->clip_batch[slot], 4), CLIP_BATCH_MAX);
+    int used = 0;
     for (int batch = 0; batch < 100000; batch++) {
-        MC_HIP(c, hipMemsetAsync(changed, 0, 4 * sizeof(int), c->stream));
-        for (int p = 0; p < 4; p++)
+        MC_HIP(c, hipMemsetAsync(changed, 0, B * sizeof(int), c->stream));
+        for (int p = 0; p < B; p++)
             hipLaunchKernelGGL(k_clip_list, dim3(div_up(K, 256)), dim3(256), 0, c->stream, geo(c), c->neg_list, K,
-                               orig, phi, changed + p);
+                               orig, phi, p ? changed + p - 1 : (const int *)nullptr, changed + p);
         MC_LAUNCH_CHECK(c);
-        c->st.clip_passes += 4;
-        MC_HIP(c, hipMemcpyAsync(hcount, changed + 3, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        MC_HIP(c, hipMemcpyAsync(hchanged, changed, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         MC_HIP(c, hipStreamSynchronize(c->stream));
-        if (*hcount == 0) break;
+        int ran = B;   // passes that did work: up to and including the first that changed nothing
+        for (int p = 0; p < B; p++)
+            if (hchanged[p] == 0) { ran = p + 1; break; }
+        used += ran;
+        if (hchanged[B - 1] == 0) break;
+        B = CLIP_BATCH_MAX;
     }
+    c->st.clip_passes += used;
+    c->clip_batch[slot] = used + 1;
     return MC_OK;
 }
 
@@ -1327,7 +1399,7 @@ int scalar_solve(Ctx *c, int type)
     MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
-    if (type == TERBTE || type == TERBED) MC_TRY(clip_nonneg(c, c->phi[field]));
+    if (type == TERBTE || type == TERBED) MC_TRY(clip_nonneg(c, c->phi[field], type == TERBTE ? 0 : 1));
     return MC_OK;
 }
 

@@ -138,6 +138,7 @@ struct Ctx {
     double *red = nullptr;         // reduction partials
     double *dscal = nullptr;       // device scalars for the solvers
     double *hscal = nullptr;       // pinned host mirror
+    int clip_batch[2] = {8, 8};    // TE / ED repair: passes to launch before the first check
     int red_blocks = 0;
 
     // ---- AMG ----
@@ -204,11 +205,61 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
     return x * per + (x < rem ? x : rem) + local;
 }
 
+// Cell order of the cell-centric kernels.  On the generated box mesh (cells
+// numbered x fastest, then y, then z) workgroup b visits a brick of
+// tx x 1 x tz cells (tx tz = 256), bricks ordered y fastest after the XCD
+// remap: a cell's neighbours at z +- 1 and the z faces it shares with them
+// then mostly lie in its own workgroup, and those at y +- 1 in the
+// workgroups running beside it on the same XCD, so a neighbour's values come
+// from L2 rather than from HBM a second time.  Other meshes: 256 consecutive
+// cells per workgroup.  Every cell is visited exactly once either way.
+struct CellWalk {
+    int N = 0;
+    int nx = 0, ny = 0, nz = 0;   // nx == 0: consecutive cells
+    int lgtx = 0, ntx = 0, ntz = 0;
+};
+
+__device__ __forceinline__ int walk_cell(const CellWalk &w)
+{
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int t = threadIdx.x;
+    if (w.nx == 0) {
+        const int c = b * 256 + t;
+        return c < w.N ? c : -1;
+    }
+    const int y = b % w.ny, r = b / w.ny;
+    const int xt = r % w.ntx, zt = r / w.ntx;
+    const int x = (xt << w.lgtx) + (t & ((1 << w.lgtx) - 1));
+    const int z = zt * (256 >> w.lgtx) + (t >> w.lgtx);
+    if (x >= w.nx || z >= w.nz) return -1;
+    return x + w.nx * (y + w.ny * z);
+}
+
 // std::min / std::max semantics (FlowSolver.inl uses them on doubles).
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// Workgroups (of 256 threads) that cover every cell in CellWalk order.
+inline int walk_blocks(const CellWalk &w)
+{
+    return w.nx ? w.ny * w.ntx * w.ntz : div_up(w.N, 256);
+}
+
+inline CellWalk cell_walk(const Ctx *c)
+{
+    CellWalk w;
+    w.N = c->N;
+    if (c->m > 0) {   // generated box: 2m x m x m cells
+        const int tx = c->opt.cell_tile > 0 ? c->opt.cell_tile : 64;
+        int lg = 0;
+        while ((1 << lg) < tx) lg++;
+        w.nx = (int)(2 * c->m); w.ny = (int)c->m; w.nz = (int)c->m;
+        w.lgtx = lg; w.ntx = div_up(w.nx, tx); w.ntz = div_up(w.nz, 256 / tx);
+    }
+    return w;
+}
 
 // Block-wide sum of NV values over W waves (blockDim.x == 64 W), result valid
 // in thread 0; the wave sums are added in wave order.

@@ -753,45 +753,79 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
 // Dense regularised coarsest operator A + alpha 11^T (alpha = mean diag / n):
 // for a mean-zero right-hand side its inverse returns the mean-zero solution
 // of the singular Neumann system.  In-place Gauss-Jordan inversion (SPD, no
-// pivoting) held entirely in LDS: n <= 128 -> at most 128 KiB of gfx950's
-// 160 KiB, one workgroup, no global-memory traffic inside the n steps.
+// pivoting) of n <= 128 rows in one workgroup of 1024 threads, the matrix
+// held in registers: thread t owns row t / 8 and the 16 columns
+// j = t % 8 + 8 q.  A step k passes the pivot column and the scaled pivot
+// row through LDS (double-buffered by the parity of k, so two barriers per
+// step suffice) and updates every other element in place.
 __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, const int *col, const double *v,
                                                         double *inv)
 {
-    __shared__ double M[DENSE_MAX * DENSE_MAX];
-    __shared__ double colk[DENSE_MAX];
+    constexpr int QN = DENSE_MAX / 8;
+    __shared__ double colk[2][DENSE_MAX];
+    __shared__ double prow[2][DENSE_MAX];
     __shared__ double red[1024 / 64];
     const int tid = threadIdx.x;
+    const int i = tid >> 3, jb = tid & 7;
     double s = 0.0;
-    for (int i = tid; i < n; i += blockDim.x)
-        for (int k = rp[i]; k < rp[i + 1]; k++) if (col[k] == i) s += v[k];
+    for (int r = tid; r < n; r += blockDim.x)
+        for (int k = rp[r]; k < rp[r + 1]; k++) if (col[k] == r) s += v[k];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
     double tot = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
     const double alpha = (tot / n) / n;
-    for (int e = tid; e < n * n; e += blockDim.x) M[e] = alpha;
-    __syncthreads();
-    for (int i = tid; i < n; i += blockDim.x)
-        for (int k = rp[i]; k < rp[i + 1]; k++) M[i * n + col[k]] += v[k];
-    __syncthreads();
+    double m[QN];
+#pragma unroll
+    for (int q = 0; q < QN; q++) m[q] = alpha;
+    if (i < n)
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            const int j = col[k];
+            if ((j & 7) != jb) continue;
+#pragma unroll
+            for (int q = 0; q < QN; q++) if (q == (j >> 3)) m[q] += v[k];
+        }
     for (int k = 0; k < n; k++) {
-        const double akk = M[k * n + k];
-        if (tid < n) colk[tid] = M[tid * n + k];
-        __syncthreads();
-        if (tid < n) M[k * n + tid] = ((tid == k) ? 1.0 : M[k * n + tid]) / akk;
-        __syncthreads();
-        for (int e = tid; e < n * n; e += blockDim.x) {
-            const int i = e / n, j = e - (e / n) * n;
-            if (i == k) continue;
-            const double mij = (j == k) ? 0.0 : M[e];
-            M[e] = mij - colk[i] * M[k * n + j];
+        const int buf = k & 1, kq = k >> 3;
+        if (i < n && jb == (k & 7)) {
+            double mk = 0.0;
+#pragma unroll
+            for (int q = 0; q < QN; q++) if (q == kq) mk = m[q];
+            colk[buf][i] = mk;
         }
         __syncthreads();
+        const double akk = colk[buf][k];
+        if (i == k) {
+#pragma unroll
+            for (int q = 0; q < QN; q++) {
+                const int j = jb + 8 * q;
+                if (j < n) {
+                    m[q] = ((j == k) ? 1.0 : m[q]) / akk;
+                    prow[buf][j] = m[q];
+                }
+            }
+        }
+        __syncthreads();
+        if (i < n && i != k) {
+            const double cik = colk[buf][i];
+#pragma unroll
+            for (int q = 0; q < QN; q++) {
+                const int j = jb + 8 * q;
+                if (j < n) {
+                    const double mij = (j == k) ? 0.0 : m[q];
+                    m[q] = mij - cik * prow[buf][j];
+                }
+            }
+        }
     }
     // stored transposed: k_dense_apply's threads (rows) then read consecutive words
-    for (int e = tid; e < n * n; e += blockDim.x) inv[(e % n) * n + e / n] = M[e];
+    if (i < n)
+#pragma unroll
+        for (int q = 0; q < QN; q++) {
+            const int j = jb + 8 * q;
+            if (j < n) inv[(size_t)j * n + i] = m[q];
+        }
 }
 
 int amg_setup(Ctx *c, const double *vals)
@@ -1075,19 +1109,38 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(int nc, const int *aptr, con
     bc[I] = (T)s;
 }
 
-// coarsest level: x = Ainv b in FP64
+// coarsest level: x = Ainv b in FP64.  One workgroup of 1024 threads: thread
+// t sums row t % 128 over the 16 columns of part t / 128 (consecutive lanes
+// read consecutive words of the transposed inverse), then the 8 part sums of
+// a row are added in part order.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dense_apply(int n, const double *inv, const T *b, T *x, const KState *stop)
+__global__ void __launch_bounds__(1024) k_dense_apply(int n, const double *inv, const T *b, T *x, const KState *stop)
 {
     if (stop && stop->all_done) return;
+    constexpr int PARTS = 1024 / DENSE_MAX, PLEN = DENSE_MAX / PARTS;
     __shared__ double bs[DENSE_MAX];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) bs[i] = (double)b[i];
+    __shared__ double part[PARTS][DENSE_MAX];
+    const int t = threadIdx.x;
+    if (t < n) bs[t] = (double)b[t];
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int i = t % DENSE_MAX, p = t / DENSE_MAX;
     double s = 0.0;
-    for (int j = 0; j < n; j++) s += inv[(size_t)j * n + i] * bs[j];   // inverse stored transposed
-    x[i] = (T)s;
+    if (i < n) {
+        const int j0 = p * PLEN;
+#pragma unroll
+        for (int jj = 0; jj < PLEN; jj++) {
+            const int j = j0 + jj;
+            if (j < n) s += inv[(size_t)j * n + i] * bs[j];
+        }
+    }
+    part[p][i] = s;
+    __syncthreads();
+    if (t < n) {
+        double y = 0.0;
+#pragma unroll
+        for (int q = 0; q < PARTS; q++) y += part[q][t];
+        x[t] = (T)y;
+    }
 }
 
 // partials of (r, z) when no smoothing sweep can carry them
@@ -1126,7 +1179,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     smoother_weights(c, std::max(1, std::min(c->opt.amg_nu, 8)), wts);
     const int nu = std::max(1, c->opt.amg_nu);
     if (l + 1 == c->amg.size()) {
-        hipLaunchKernelGGL(k_dense_apply<T>, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, c->amg_dense, b, x, stop);
+        hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(1024), 0, c->stream, L.n, c->amg_dense, b, x, stop);
         if (ks) {  // one-level hierarchy: no smoothing sweep carries the (r, z) reduction
             const Grid g = grid_for(L.n);
             hipLaunchKernelGGL(k_rz<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, L.n, b, x, ks, c->red, g.chunk);

@@ -576,6 +576,7 @@ void mco_default_opts(mco_opts *o)
     o->p_maxit = 100000;
     o->solve_scalars = 1;
     o->p_solver = 0;
+    o->limit_grad = 0;
 }
 
 static void setup_sparse_matrix(mco_state *s, double URFactor, double *A_c, double *phi_c, double *S_c);
@@ -825,6 +826,53 @@ void mco_get_phi_gradients(mco_state *s)
     static const int fields[11] = {MCO_U, MCO_V, MCO_W, MCO_P, MCO_TE, MCO_ED, MCO_TEM, MCO_FUL, MCO_PRO, MCO_VARF, MCO_VARP};
     static const int zb[11] = {0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0};
     lsq_gradients(s, 11, fields, zb);
+}
+
+/* limit_phi_gradient (FlowSolver.inl:804-873), Venkatakrishnan's limiter:
+   phi's min and max over the cell and its six face neighbours (boundary
+   slots at the boundary), then the gradient scaled by the least
+   (r^2 + 2r) / (r^2 + r + 2) over the faces, r = (max or min - phi_c) over
+   grad . (x_face - x_c), r = 1000 where |grad . (x_face - x_c)| < 6e-6.
+   The reference's pow(r, 2) is evaluated as r * r, the correctly rounded
+   square (glibc's pow is within 0.52 ulp of it). */
+static void limit_phi_gradient(mco_state *s, int field)
+{
+    const mco_mesh *M = &s->mesh;
+    const int64_t N = s->N;
+    const double *phi = s->phi[field];
+    mco_vec *g = s->grad[field];
+    for (int64_t c = 0; c < N; c++) {
+        double pmax = phi[c], pmin = phi[c];
+        for (int f = 0; f < 6; f++) {
+            const uint64_t face = M->cell_faces[c * 6 + f];
+            const uint64_t c0 = M->faces[2 * face], c1 = M->faces[2 * face + 1];
+            double v;
+            if (c1 < (uint64_t)N) v = (c0 == (uint64_t)c) ? phi[c1] : phi[c0];
+            else v = phi[N + (c1 - N)];
+            pmax = dmax(pmax, v);
+            pmin = dmin(pmin, v);
+        }
+        const double deltamax = pmax - phi[c], deltamin = pmin - phi[c];
+        double alpha = 1.0;
+        for (int f = 0; f < 6; f++) {
+            const uint64_t face = M->cell_faces[c * 6 + f];
+            const mco_vec ds = vsub(s->face_centers[face], M->cell_centers[c]);
+            const double delta_face = g[c].x * ds.x + g[c].y * ds.y + g[c].z * ds.z;
+            double r;
+            if (fabs(delta_face) < 0.000006) r = 1000.0;
+            else if (delta_face > 0.0) r = deltamax / delta_face;
+            else r = deltamin / delta_face;
+            alpha = dmin(alpha, (r * r + 2.0 * r) / (r * r + r + 2.0));
+        }
+        g[c].x = g[c].x * alpha; g[c].y = g[c].y * alpha; g[c].z = g[c].z * alpha;
+    }
+}
+
+void mco_limit_phi_gradients(mco_state *s)
+{
+    /* FlowSolver.inl:875-893, the same 11 fields in the same order */
+    static const int fields[11] = {MCO_U, MCO_V, MCO_W, MCO_P, MCO_TE, MCO_ED, MCO_TEM, MCO_FUL, MCO_PRO, MCO_VARF, MCO_VARP};
+    for (int k = 0; k < 11; k++) limit_phi_gradient(s, fields[k]);
 }
 
 void mco_get_phi_gradient(mco_state *s, int field, int pressure)
@@ -1569,6 +1617,7 @@ void mco_scalar_solve(mco_state *s, int type, int field)
 void mco_simple_iteration(mco_state *s, int inner)
 {
     mco_get_phi_gradients(s);
+    if (s->opts.limit_grad) mco_limit_phi_gradients(s);   /* FlowSolver.inl:3197-3198 */
     if (s->timestep_count == 0 && inner == 0) mco_set_up_field(s);
     mco_calculate_UVW(s);
     mco_calculate_pressure(s);

@@ -71,6 +71,8 @@ typedef struct mco_opts {
     int    p_maxit;
     int    solve_scalars;    /* 1: run the 7 scalar transport solves          */
     int    p_solver;         /* 0: Jacobi-PCG (checker), 1: aggregation-AMG PCG */
+    int    limit_grad;       /* 1: Venkatakrishnan limiter after the gradients
+                                (FLOW_SOLVER_LIMIT_GRAD, utils.hpp:26; off there) */
 } mco_opts;
 
 typedef struct mco_stats {
@@ -128,6 +130,7 @@ void mco_destroy(mco_state *s);
 
 /* ---- stages (names follow FlowSolver.inl) ---- */
 void mco_get_phi_gradients(mco_state *s);
+void mco_limit_phi_gradients(mco_state *s);
 void mco_get_phi_gradient(mco_state *s, int field, int pressure);
 void mco_precomp_AU(mco_state *s);
 void mco_set_up_field(mco_state *s);

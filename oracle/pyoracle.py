@@ -41,6 +41,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("delta", C.c_double), ("mom_rtol", C.c_double), ("mom_maxit", C.c_int),
         ("p_rtol", C.c_double), ("p_maxit", C.c_int), ("solve_scalars", C.c_int), ("p_solver", C.c_int),
+        ("limit_grad", C.c_int),
     ]
 
 
@@ -98,7 +99,7 @@ def lib():
         L.mco_create.argtypes = [C.c_int64, C.POINTER(Opts)]
         L.mco_default_opts.argtypes = [C.POINTER(Opts)]
         L.mco_destroy.argtypes = [SP]
-        for name in ["mco_get_phi_gradients", "mco_precomp_AU", "mco_set_up_field", "mco_calculate_flux_UVW",
+        for name in ["mco_get_phi_gradients", "mco_limit_phi_gradients", "mco_precomp_AU", "mco_set_up_field", "mco_calculate_flux_UVW",
                      "mco_momentum_assemble", "mco_calculate_UVW", "mco_calculate_mass_flux",
                      "mco_setup_pressure_matrix", "mco_solve_pressure_matrix", "mco_update_mass_flux",
                      "mco_calculate_pressure", "mco_timestep"]:
