@@ -84,6 +84,9 @@ typedef struct mc_options {
                               (cell_tile x 1 x 256/cell_tile) one workgroup of the
                               cell-centric assembly kernels visits; a power of two
                               in [1, 256], 0: the default (64)                   */
+    int    limit_grad;     /* 1: Venkatakrishnan limiter on the 11 gradients of each
+                              SIMPLE iteration (FLOW_SOLVER_LIMIT_GRAD,
+                              utils.hpp:26, FlowSolver.inl:3197; 0 as there)     */
 } mc_options;
 
 typedef struct mc_solve_status {
@@ -200,6 +203,11 @@ int  mc_get_phi_gradients(mc_ctx *ctx);
 /* C_kernel_get_phi_gradient (gpu_kernels.cuh:48) / get_phi_gradient
    (FlowSolver.inl:738-803); pressure != 0 selects dphi = 0 on boundaries.     */
 int  mc_get_phi_gradient(mc_ctx *ctx, int field, int pressure);
+/* limit_phi_gradients (FlowSolver.inl:875-893, limit_phi_gradient :804-873):
+   Venkatakrishnan's limiter applied in place to the 11 gradients (the GPU
+   reference has no counterpart; its call is commented out,
+   gpu/FlowSolver.inl:1333).                                                   */
+int  mc_limit_phi_gradients(mc_ctx *ctx);
 /* C_kernel_precomp_AU (gpu_kernels.cuh:38) + calculate_mass_flux:
    set_up_field (FlowSolver.inl:1065-1113).                                   */
 int  mc_set_up_field(mc_ctx *ctx);

@@ -164,6 +164,7 @@ int mc_options_default(mc_options *o)
     o->inlet_scale = 1.0;
     o->amg_cheb_ratio = 4.0;   // Chebyshev weights on [0.5, 2]: 12.8 -> 9.8 PCG iterations at m = 128
     o->cell_tile = 0;
+    o->limit_grad = 0;
     return MC_OK;
 }
 
@@ -182,6 +183,7 @@ static int validate(const mc_options *o, std::string &err)
         err = "cell_tile must be 0 or a power of two in [1, 256]";
         return MC_ERR_ARG;
     }
+    if (o->limit_grad != 0 && o->limit_grad != 1) { err = "limit_grad must be 0 or 1"; return MC_ERR_ARG; }
     return MC_OK;
 }
 
@@ -303,7 +305,8 @@ int mc_simple_iteration(mc_ctx *x, int inner)
     return MC_OK;
 }
 
-int mc_get_phi_gradients(mc_ctx *x) { CTX_OR_FAIL return grad_multi(c); }
+int mc_get_phi_gradients(mc_ctx *x) { CTX_OR_FAIL return grad_multi(c, false); }
+int mc_limit_phi_gradients(mc_ctx *x) { CTX_OR_FAIL return limit_gradients(c); }
 
 int mc_get_phi_gradient(mc_ctx *x, int field, int pressure)
 {

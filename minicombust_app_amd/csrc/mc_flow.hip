@@ -76,12 +76,35 @@ struct GradArgs {
     int zero_boundary[11];
 };
 
+// limit_phi_gradient (FlowSolver.inl:804-873), Venkatakrishnan's limiter
+// for one cell: pmax / pmin are phi's extremes over the cell and its six face
+// neighbours (boundary slots at the boundary), ds[s] = x_face - x_c for the
+// faces in side order; the gradient is scaled by the least
+// (r^2 + 2r) / (r^2 + r + 2) over the faces.  pow(r, 2) is evaluated as r * r.
+__device__ __forceinline__ void venkat(double phic, double pmax, double pmin, const dv3 (&ds)[6], double &gx,
+                                       double &gy, double &gz)
+{
+    const double deltamax = pmax - phic, deltamin = pmin - phic;
+    double alpha = 1.0;
+    for (int s = 0; s < 6; s++) {
+        const double delta_face = gx * ds[s].x + gy * ds[s].y + gz * ds[s].z;
+        double r;
+        if (fabs(delta_face) < 0.000006) r = 1000.0;
+        else if (delta_face > 0.0) r = deltamax / delta_face;
+        else r = deltamin / delta_face;
+        alpha = dmin(alpha, (r * r + 2.0 * r) / (r * r + r + 2.0));
+    }
+    gx = gx * alpha; gy = gy * alpha; gz = gz * alpha;
+}
+
 // get_phi_gradients / get_phi_gradient (FlowSolver.inl:738-803, 917-1063):
 // A = sum dX dX^T, b_k = sum dX dphi_k over the six faces in side order,
 // then the 3x3 Cramer solve of FlowSolver.inl:895-915.  For an interior face
 // mask*(cc1-cc0) and mask*(phi1-phi0) equal cc[nb]-cc[c] and phi[nb]-phi[c]
-// exactly (IEEE subtraction is antisymmetric).
-template <int NF>
+// exactly (IEEE subtraction is antisymmetric).  LIMIT: limit_phi_gradients
+// (FlowSolver.inl:875-893) follows in the same pass, from the gradient just
+// formed and the neighbour values this cell has already read.
+template <int NF, bool LIMIT = false>
 __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
 {
     const int c = walk_cell(g.w);
@@ -91,9 +114,13 @@ __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
 #pragma unroll
     for (int k = 0; k < NF; k++) ph[k] = a.phi[k][c];
     double a00 = 0.0, a10 = 0.0, a20 = 0.0, a11 = 0.0, a21 = 0.0, a22 = 0.0;
-    double bx[NF], by[NF], bz[NF];
+    double bx[NF], by[NF], bz[NF], pmax[LIMIT ? NF : 1], pmin[LIMIT ? NF : 1];
 #pragma unroll
     for (int k = 0; k < NF; k++) { bx[k] = 0.0; by[k] = 0.0; bz[k] = 0.0; }
+    if constexpr (LIMIT) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) { pmax[k] = ph[k]; pmin[k] = ph[k]; }
+    }
     for (int s = 0; s < 6; s++) {
         const int nb = g.nbr[6 * c + s];
         dv3 dX;
@@ -101,13 +128,23 @@ __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
         if (nb >= 0) {
             dX = ld3(g.cx, g.cy, g.cz, nb) - cc;
 #pragma unroll
-            for (int k = 0; k < NF; k++) d[k] = a.phi[k][nb] - ph[k];
+            for (int k = 0; k < NF; k++) {
+                const double v = a.phi[k][nb];
+                d[k] = v - ph[k];
+                if constexpr (LIMIT) { pmax[k] = dmax(pmax[k], v); pmin[k] = dmin(pmin[k], v); }
+            }
         } else {
             const int f = g.cell_faces[6 * c + s];
             const int slot = g.fc1[f];
             dX = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
 #pragma unroll
-            for (int k = 0; k < NF; k++) d[k] = a.zero_boundary[k] ? 0.0 : a.phi[k][slot] - ph[k];
+            for (int k = 0; k < NF; k++) {
+                if constexpr (LIMIT) {
+                    const double v = a.phi[k][slot];
+                    pmax[k] = dmax(pmax[k], v); pmin[k] = dmin(pmin[k], v);
+                }
+                d[k] = a.zero_boundary[k] ? 0.0 : a.phi[k][slot] - ph[k];
+            }
         }
         a00 += (dX.x * dX.x); a10 += (dX.x * dX.y); a20 += (dX.x * dX.z);
         a11 += (dX.y * dX.y); a21 += (dX.y * dX.z); a22 += (dX.z * dX.z);
@@ -125,11 +162,48 @@ __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
     const double c00 = (A11 * A22 - A21 * A12), c01 = (A21 * A02 - A01 * A22), c02 = (A01 * A12 - A11 * A02);
     const double c10 = (A12 * A20 - A10 * A22), c11 = (A00 * A22 - A20 * A02), c12 = (A10 * A02 - A00 * A12);
     const double c20 = (A10 * A21 - A20 * A11), c21 = (A20 * A01 - A00 * A21), c22 = (A00 * A11 - A10 * A01);
+    dv3 ds[6];
+    if constexpr (LIMIT) {
+        for (int s = 0; s < 6; s++) {
+            const int f = g.cell_faces[6 * c + s];
+            ds[s] = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < NF; k++) {
-        a.gx[k][c] = (bx[k] * invdet * c00) + (by[k] * invdet * c01) + (bz[k] * invdet * c02);
-        a.gy[k][c] = (bx[k] * invdet * c10) + (by[k] * invdet * c11) + (bz[k] * invdet * c12);
-        a.gz[k][c] = (bx[k] * invdet * c20) + (by[k] * invdet * c21) + (bz[k] * invdet * c22);
+        double gx = (bx[k] * invdet * c00) + (by[k] * invdet * c01) + (bz[k] * invdet * c02);
+        double gy = (bx[k] * invdet * c10) + (by[k] * invdet * c11) + (bz[k] * invdet * c12);
+        double gz = (bx[k] * invdet * c20) + (by[k] * invdet * c21) + (bz[k] * invdet * c22);
+        if constexpr (LIMIT) venkat(ph[k], pmax[k], pmin[k], ds, gx, gy, gz);
+        a.gx[k][c] = gx; a.gy[k][c] = gy; a.gz[k][c] = gz;
+    }
+}
+
+// limit_phi_gradients on its own (FlowSolver.inl:875-893): the stored
+// gradients of the NF fields are limited in place.
+template <int NF>
+__global__ void __launch_bounds__(256) k_limit(Geo g, GradArgs a)
+{
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
+    const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
+    dv3 ds[6];
+    for (int s = 0; s < 6; s++) {
+        const int f = g.cell_faces[6 * c + s];
+        ds[s] = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
+    }
+    for (int k = 0; k < NF; k++) {
+        const double phic = a.phi[k][c];
+        double pmax = phic, pmin = phic;
+        for (int s = 0; s < 6; s++) {
+            const int nb = g.nbr[6 * c + s];
+            const double v = (nb >= 0) ? a.phi[k][nb] : a.phi[k][g.fc1[g.cell_faces[6 * c + s]]];
+            pmax = dmax(pmax, v);
+            pmin = dmin(pmin, v);
+        }
+        double gx = a.gx[k][c], gy = a.gy[k][c], gz = a.gz[k][c];
+        venkat(phic, pmax, pmin, ds, gx, gy, gz);
+        a.gx[k][c] = gx; a.gy[k][c] = gy; a.gz[k][c] = gz;
     }
 }
 
@@ -140,7 +214,7 @@ static double grad_bytes(Ctx *c, int nf)
     return 32.0 * nf * c->N + 48.0 * c->N + 8.0 * c->F + 24.0 * c->Fb;
 }
 
-int grad_multi(Ctx *c)
+static GradArgs grad11_args(Ctx *c)
 {
     static const int fields[11] = {U, V, W, P, TE, ED, TEM, FUL, PRO, VARF, VARP};
     GradArgs a;
@@ -149,8 +223,24 @@ int grad_multi(Ctx *c)
         a.gx[k] = c->gx[fields[k]]; a.gy[k] = c->gy[fields[k]]; a.gz[k] = c->gz[fields[k]];
         a.zero_boundary[k] = fields[k] == P;
     }
+    return a;
+}
+
+int grad_multi(Ctx *c, bool limit)
+{
+    const GradArgs a = grad11_args(c);
     KTimer kt(c, MC_KC_GRAD, grad_bytes(c, 11));
-    hipLaunchKernelGGL(k_grad<11>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
+    if (limit)
+        hipLaunchKernelGGL((k_grad<11, true>), dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
+    else
+        hipLaunchKernelGGL((k_grad<11, false>), dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
+    MC_LAUNCH_CHECK(c);
+    return MC_OK;
+}
+
+int limit_gradients(Ctx *c)
+{
+    hipLaunchKernelGGL(k_limit<11>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), grad11_args(c));
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -1406,7 +1496,7 @@ int scalar_solve(Ctx *c, int type)
 // ============================================================= A17 ==
 int simple_iteration(Ctx *c, int inner)
 {
-    MC_TRY(grad_multi(c));
+    MC_TRY(grad_multi(c, c->opt.limit_grad != 0));   // + limit_phi_gradients, FlowSolver.inl:3197-3198
     // FlowSolver.inl:3204-3208: the first pass of the first timestep
     if (c->timestep_count == 0 && inner == 0 && !c->field_set_up) {
         MC_TRY(set_up_field(c));

@@ -383,7 +383,8 @@ int mesh_finish(Ctx *c);
 int init_fields(Ctx *c);   // constructor fields and boundary slots   // geometry, init, CSR pattern, face orders
 
 // stages (mc_flow.hip)
-int grad_multi(Ctx *c);                                  // 11 fields
+int grad_multi(Ctx *c, bool limit);                      // 11 fields, optionally limited
+int limit_gradients(Ctx *c);                             // limiter alone, 11 fields
 int grad_single(Ctx *c, int field, bool pressure);
 int set_up_field(Ctx *c);
 int momentum_assemble(Ctx *c);

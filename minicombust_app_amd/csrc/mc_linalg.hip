@@ -1653,6 +1653,9 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     int rc;
     bool used = false, tried = false;
     double bn0 = 0.0;
+    // a state set through the C-ABI can reach a solve before any assembly
+    // built the hierarchy: build it from the stored pressure matrix
+    if (!c->amg_ready) MC_TRY(amg_setup(c, c->PA_vals));
     // learned = 2: every loop-1 solve starts from the guess; learned = 1: only
     // while its measured saving (iterations x time per iteration) exceeds its
     // cost, re-checking the other choice every 16th solve

@@ -47,7 +47,7 @@ class Options(C.Structure):
         ("p_rtol", C.c_double), ("p_maxit", C.c_int),
         ("amg_nu", C.c_int), ("amg_omega", C.c_double), ("amg_cscale", C.c_double), ("amg_coarse_max", C.c_int),
         ("mixed_inner_rtol", C.c_double), ("check_every", C.c_int), ("inlet_scale", C.c_double),
-        ("amg_cheb_ratio", C.c_double), ("cell_tile", C.c_int),
+        ("amg_cheb_ratio", C.c_double), ("cell_tile", C.c_int), ("limit_grad", C.c_int),
     ]
 
 
@@ -100,7 +100,7 @@ def lib():
         L.mc_last_error.restype = C.c_char_p
         L.mc_ctx_sizes.argtypes = [P_] + [C.POINTER(C.c_int64)] * 4
         L.mc_set_options.argtypes = [P_, C.POINTER(Options)]
-        for n in ["mc_timestep", "mc_get_phi_gradients", "mc_set_up_field", "mc_momentum_assemble",
+        for n in ["mc_timestep", "mc_get_phi_gradients", "mc_limit_phi_gradients", "mc_set_up_field", "mc_momentum_assemble",
                   "mc_pressure_assemble", "mc_calculate_pressure", "mc_update_P", "mc_sync", "mc_profile_reset",
                   "mc_scalar_batch_solve", "mc_reset_fields"]:
             getattr(L, n).argtypes = [P_]
@@ -212,6 +212,9 @@ class Context:
 
     def get_phi_gradients(self):
         self._check(self.L.mc_get_phi_gradients(self.h), "mc_get_phi_gradients")
+
+    def limit_phi_gradients(self):
+        self._check(self.L.mc_limit_phi_gradients(self.h), "mc_limit_phi_gradients")
 
     def get_phi_gradient(self, field, pressure):
         self._check(self.L.mc_get_phi_gradient(self.h, field, int(pressure)), "mc_get_phi_gradient")

@@ -234,3 +234,76 @@ def test_scalar_batch_equals_single():
     g1.close()
     g2.close()
     o.close()
+
+
+LIMITED = [po.U, po.V, po.W, po.P, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF, po.VARP]
+
+
+def _rough(o, seed, rel=0.1):
+    """Seeded noise on every limited field (cells and slots): its local extrema
+    make the gradient limiter act in many cells."""
+    rng = np.random.default_rng(seed)
+    for f in LIMITED:
+        ph = o.phi(f)
+        ph[:] = ph + rng.standard_normal(ph.size) * (rel * np.abs(ph).max() + 1e-6)
+
+
+def test_gradient_limiter_bits():
+    """limit_phi_gradients (FlowSolver.inl:804-893) bit for bit: the C-ABI
+    entry after the gradients, and the limiter fused into the gradient pass of
+    a SIMPLE iteration with limit_grad = 1."""
+    m = 5
+    print("gradient limiter m=%d" % m, flush=True)
+    o, g = make_pair(m)
+    _rough(o, 11)
+    push_state(o, g)
+    o.get_phi_gradients()
+    g.get_phi_gradients()
+    unl = {f: o.grad(f).copy() for f in LIMITED}
+    o.limit_phi_gradients()
+    g.limit_phi_gradients()
+    acted = 0
+    for f in LIMITED:
+        assert_bits(g.grad(f), o.grad(f), "limited grad %s" % po.FIELD_NAMES[f])
+        acted += int(np.sum(np.any(o.grad(f)[:o.N] != unl[f][:o.N], axis=1)))
+    assert acted > o.N
+    g.close()
+    o.close()
+    # fused form, from a state one timestep in with 1% multiplicative noise
+    # (the iteration's solves need positive TE / ED)
+    o = po.Oracle(m)
+    o.timestep()
+    rng = np.random.default_rng(12)
+    for f in LIMITED:
+        ph = o.phi(f)
+        ph[:] = ph * (1.0 + 1e-2 * rng.standard_normal(ph.size))
+    g2 = mc.Context(m, limit_grad=1)
+    push_state(o, g2)
+    o.get_phi_gradients()
+    unl = {f: o.grad(f).copy() for f in LIMITED}
+    o.limit_phi_gradients()
+    assert sum(int(np.sum(np.any(o.grad(f)[:o.N] != unl[f][:o.N], axis=1))) for f in LIMITED) > o.N
+    g2.simple_iteration(1)
+    # P and PP are re-differentiated later in the iteration; the other ten
+    # keep the gradients of its first pass
+    for f in [po.U, po.V, po.W, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF, po.VARP]:
+        assert_bits(g2.grad(f), o.grad(f), "fused limited grad %s" % po.FIELD_NAMES[f])
+    o.close()
+    g2.close()
+
+
+@pytest.mark.parametrize("mode", [po.MODE_COMPAT, po.MODE_SIMPLE])
+def test_timesteps_limited(mode):
+    """Whole timesteps with the limiter on both sides (FLOW_SOLVER_LIMIT_GRAD = 1)."""
+    m = 8
+    print("limited timesteps m=%d mode=%d" % (m, mode), flush=True)
+    o = po.Oracle(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=100000, limit_grad=1)
+    g = mc.Context(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, limit_grad=1)
+    for ts in range(2):
+        o.timestep()
+        g.timestep()
+        for f in LIMITED:
+            assert_close(g.phi(f), o.phi(f), "limited ts %d field %s" % (ts, po.FIELD_NAMES[f]), 1e-8)
+        assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "limited ts %d mass flux" % ts, 1e-8)
+    o.close()
+    g.close()

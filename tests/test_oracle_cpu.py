@@ -313,3 +313,81 @@ def test_timestep_fields_finite_and_bounded():
     for f in [po.U, po.V, po.W, po.P, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF, po.VARP]:
         assert np.all(np.isfinite(o.phi(f)[:o.N])), po.FIELD_NAMES[f]
     assert o.phi(po.TE)[:o.N].min() >= 0 and o.phi(po.ED)[:o.N].min() >= 0
+
+
+LIMITED = [po.U, po.V, po.W, po.P, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF, po.VARP]
+
+
+def _face_values(o, M, phi):
+    """phi across each of the six faces of every cell (boundary: side slot)."""
+    N = o.N
+    faces = M["faces"].astype(np.int64)
+    cf = M["cell_faces"].astype(np.int64)
+    c0, c1 = faces[cf, 0], faces[cf, 1]
+    cells = np.arange(N)[:, None]
+    other = np.where(c0 == cells, c1, c0)      # boundary faces: c1 = N + side
+    return phi[other]
+
+
+def test_limiter_keeps_face_values_within_neighbour_bounds():
+    """Venkatakrishnan's limiter (FlowSolver.inl:804-873): every limited
+    gradient is alpha times the unlimited one with 0 <= alpha <= 1, and the
+    limited reconstruction phi_c + g . (x_f - x_c) stays within the min / max of
+    the cell and its neighbours at every face, up to the 6e-6 threshold below
+    which a face does not limit (f(r) <= r for r >= 0 gives the bound)."""
+    m = 4
+    o = po.Oracle(m)
+    M = o.mesh_arrays()
+    N = o.N
+    rng = np.random.default_rng(3)
+    for f in LIMITED:
+        ph = o.phi(f)
+        ph[:] = ph + rng.standard_normal(ph.size) * (0.1 * np.abs(ph).max() + 1e-3)
+    o.get_phi_gradients()
+    unl = {f: o.grad(f)[:N].copy() for f in LIMITED}
+    o.limit_phi_gradients()
+    fc = o.facevec("face_centers")
+    ds = fc[M["cell_faces"].astype(np.int64)] - M["cell_centers"][:, None, :]   # (N, 6, 3)
+    limited = 0
+    for f in LIMITED:
+        ph = o.phi(f)
+        g, g0 = o.grad(f)[:N], unl[f]
+        nbv = _face_values(o, M, ph)
+        pmax = np.maximum(ph[:N], nbv.max(axis=1))
+        pmin = np.minimum(ph[:N], nbv.min(axis=1))
+        # g = alpha g0, one alpha per cell
+        j = np.argmax(np.abs(g0), axis=1)
+        big = np.abs(g0[np.arange(N), j]) > 0
+        alpha = np.where(big, g[np.arange(N), j] / np.where(big, g0[np.arange(N), j], 1.0), 1.0)
+        assert np.all(alpha >= -1e-15) and np.all(alpha <= 1.0 + 1e-15)
+        assert np.allclose(g, alpha[:, None] * g0, rtol=1e-14, atol=1e-300)
+        rec = ph[:N, None] + np.einsum("nk,nsk->ns", g, ds)
+        tol = 6e-6 + 1e-12 * np.abs(ph).max()
+        assert np.all(rec <= pmax[:, None] + tol) and np.all(rec >= pmin[:, None] - tol)
+        limited += int(np.sum(alpha < 1.0 - 1e-12))
+    assert limited > N   # the rough fields are limited in many cells
+
+
+def test_limiter_leaves_linear_fields_and_flattens_extrema():
+    """On a linear field the interior gradient is kept (r = 2 on every face,
+    f(2) = 1); at a cell raised above all its neighbours deltamax = 0, so any
+    face with g . ds > 0 gives r = 0 and the gradient is set to zero."""
+    o = po.Oracle(5)
+    M = o.mesh_arrays()
+    N = o.N
+    cc = M["cell_centers"]
+    a = np.array([3.25, -1.5, 0.75])
+    inner = _interior_cells(M, N)
+    spike = int(np.nonzero(inner)[0][len(np.nonzero(inner)[0]) // 2])
+    ph = o.phi(po.U)
+    ph[:N] = cc @ a + 2.0
+    o.get_phi_gradients()
+    g0 = o.grad(po.U)[:N].copy()
+    o.limit_phi_gradients()
+    g = o.grad(po.U)[:N]
+    np.testing.assert_allclose(g[inner], g0[inner], rtol=1e-12, atol=0)
+    ph[spike] += 10.0
+    o.get_phi_gradients()
+    assert np.abs(o.grad(po.U)[spike]).max() > 1.0
+    o.limit_phi_gradients()
+    assert np.all(o.grad(po.U)[spike] == 0.0)
