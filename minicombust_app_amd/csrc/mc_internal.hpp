@@ -116,7 +116,6 @@ struct Ctx {
     double *A_vals = nullptr;      // momentum / scalar matrix
     double *A_vals2 = nullptr;     // temperature matrix, assembled beside the passive scalars'
     double *PA_vals = nullptr;     // pressure-correction matrix
-    float *PA_vals32 = nullptr;
     double *Pb = nullptr;          // pressure RHS (compat: frozen copy)
     double *u = nullptr;           // solution scratch
     int fine_nd = 0;               // diagonal copy of the fine pattern (0: CSR kernels)

@@ -750,6 +750,26 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
     for (int k = i; k < nnz; k += gridDim.x * blockDim.x) v32[k] = (float)v[k];
 }
 
+// A level with diagonal storage: one pass over its CSR rows fills the FP64 and
+// FP32 diagonal copies and the inverse diagonal.
+__global__ void k_level_dia(int n, const int *rp, const int *col, const int *map, const double *v, double *dia,
+                            float *dia32, double *dinv, float *dinv32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; k++) {
+        const double x = v[k];
+        const int m = map[k];
+        dia[m] = x;
+        dia32[m] = (float)x;
+        if (col[k] == i) d = x;
+    }
+    const double di = (d != 0.0) ? 1.0 / d : 0.0;
+    dinv[i] = di;
+    dinv32[i] = (float)di;
+}
+
 // Dense regularised coarsest operator A + alpha 11^T (alpha = mean diag / n):
 // for a mean-zero right-hand side its inverse returns the mean-zero solution
 // of the singular Neumann system.  In-place Gauss-Jordan inversion (SPD, no
@@ -797,11 +817,12 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
         __syncthreads();
         const double akk = colk[buf][k];
         if (i == k) {
+            const double rk = 1.0 / akk;
 #pragma unroll
             for (int q = 0; q < QN; q++) {
                 const int j = jb + 8 * q;
                 if (j < n) {
-                    m[q] = ((j == k) ? 1.0 : m[q]) / akk;
+                    m[q] = ((j == k) ? 1.0 : m[q]) * rk;
                     prow[buf][j] = m[q];
                 }
             }
@@ -856,8 +877,8 @@ int amg_setup(Ctx *c, const double *vals)
             MC_HIP(c, hipMemcpy(L.row_ptr, cur.rp.data(), (size_t)(L.n + 1) * 4, hipMemcpyHostToDevice));
             MC_HIP(c, hipMemcpy(L.col, cur.col.data(), (size_t)L.nnz * 4, hipMemcpyHostToDevice));
         }
-        MC_TRY(dalloc(c, &L.vals, L.nnz));
-        MC_TRY(dalloc(c, &L.vals32, L.nnz));
+        if (first) L.vals = c->PA_vals;   // the fine level reads the pressure matrix in place
+        else MC_TRY(dalloc(c, &L.vals, L.nnz));
         MC_TRY(dalloc(c, &L.dinv, L.n));
         MC_TRY(dalloc(c, &L.dinv32, L.n));
         MC_TRY(dalloc(c, &L.x, L.n)); MC_TRY(dalloc(c, &L.b, L.n)); MC_TRY(dalloc(c, &L.t, L.n));
@@ -876,6 +897,7 @@ int amg_setup(Ctx *c, const double *vals)
                 MC_HIP(c, hipMemset(L.dia, 0, (size_t)L.nd * L.n * 8));
                 MC_HIP(c, hipMemset(L.dia32, 0, (size_t)L.nd * L.n * 4));
             }
+            if (!L.nd) MC_TRY(dalloc(c, &L.vals32, L.nnz));   // FP32 CSR only where no diagonal copy
         }
         if (cur.n <= coarse_max) {
             c->amg.push_back(L);
@@ -937,12 +959,14 @@ int amg_update_values(Ctx *c, const double *vals)
     const double *vf = vals;
     for (size_t l = 0; l < c->amg.size(); l++) {
         AmgLevel &L = c->amg[l];
-        MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
-        hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col, L.vals,
-                           L.dinv, L.dinv32, L.vals32, L.nnz);
+        if (L.vals != vf)
+            MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
         if (L.nd)
-            hipLaunchKernelGGL(k_csr_to_dia, dim3(div_up(L.nnz, 256)), dim3(256), 0, c->stream, L.nnz, L.dia_map,
-                               L.vals, L.dia, L.dia32);
+            hipLaunchKernelGGL(k_level_dia, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
+                               L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32);
+        else
+            hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
+                               L.vals, L.dinv, L.dinv32, L.vals32, L.nnz);
         MC_LAUNCH_CHECK(c);
         if (l + 1 < c->amg.size()) {
             AmgLevel &Lc = c->amg[l + 1];

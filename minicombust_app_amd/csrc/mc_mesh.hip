@@ -488,7 +488,6 @@ int mesh_finish(Ctx *c)
     MC_TRY(dalloc(c, &c->col, nnz));
     MC_TRY(dalloc(c, &c->A_vals, nnz));
     MC_TRY(dalloc(c, &c->PA_vals, nnz));
-    MC_TRY(dalloc(c, &c->PA_vals32, nnz));
     MC_TRY(dalloc(c, &c->Pb, N));
     MC_TRY(dalloc(c, &c->u, N));
     MC_HIP(c, hipMemsetAsync(c->A_vals, 0, (size_t)nnz * 8, c->stream));
