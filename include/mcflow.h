@@ -259,6 +259,17 @@ int  mc_state_flags(mc_ctx *ctx, int *first_pressure, int64_t *timestep_count);
 int  mc_set_state_flags(mc_ctx *ctx, int first_pressure, int64_t timestep_count);
 int  mc_sync(mc_ctx *ctx);
 
+/* ---- output ---------------------------------------------------------------
+   FlowSolver<T>::output_data (FlowSolver.inl:18-23) ->
+   VisitWriter<T>::write_flow (include/visit/VisitWriter.hpp:127-234): one
+   legacy-ASCII VTK POLYDATA file <prefix>_flow<rank>_timestep<timestep>.vtk
+   with the cell centres as points, the velocity and nine scalars as point
+   data, byte for byte as the reference writes it.                            */
+int  mc_write_vtk(mc_ctx *ctx, const char *prefix, int64_t timestep, int rank);
+/* The same file from host arrays (no device needed): centres [3 n] x,y,z
+   interleaved, fields[11] = U V W P TE ED TEM FUL PRO VARF VARP, n each.     */
+int  mc_vtk_write_arrays(const char *path, int64_t n, const double *centres, const double *const *fields);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Profiling: each launch of a class whose bit (1 << MC_KC_x) is set in `mask`
    is bracketed by HIP events on the context stream, with no synchronisation

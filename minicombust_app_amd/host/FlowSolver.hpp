@@ -18,6 +18,7 @@
 #pragma once
 
 #include <chrono>
+#include <filesystem>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
@@ -69,19 +70,13 @@ class FlowSolver {
         timesteps_++;
     }
 
-    // FlowSolver<T>::output_data (FlowSolver.inl:18): the reference writes
-    // VTK files; this build reports per-field range and mean (VTK output is
-    // outside the accelerated path).
+    // FlowSolver<T>::output_data (FlowSolver.inl:18-23): the flow fields as the
+    // reference's VTK file, out/flow/minicombust_flow0_timestep<t>.vtk
     void output_data(uint64_t timestep)
     {
-        static const char *names[] = {"U", "V", "W", "P", "PP", "TE", "ED", "TP", "TEM", "FUL", "PRO", "VARF", "VARP"};
-        fprintf(out_, "Flow fields at timestep %" PRIu64 ":\n", timestep);
-        for (int f : {MC_U, MC_V, MC_W, MC_P, MC_TE, MC_ED, MC_TEM, MC_FUL, MC_PRO, MC_VARF, MC_VARP}) {
-            const std::vector<double> v = field(f);
-            double lo = INFINITY, hi = -INFINITY, sum = 0.0;
-            for (size_t i = 0; i < n_cells_; i++) { lo = std::fmin(lo, v[i]); hi = std::fmax(hi, v[i]); sum += v[i]; }
-            fprintf(out_, "\t%-5s min % .6e  max % .6e  mean % .6e\n", names[f], lo, hi, sum / (double)n_cells_);
-        }
+        std::error_code ec;
+        std::filesystem::create_directories("out/flow", ec);
+        check(mc_write_vtk(ctx_, "out/flow/minicombust", (int64_t)timestep, 0), "mc_write_vtk");
     }
 
     // FlowSolver<T>::print_logger_stats (FlowSolver.inl:3079): solver work and time

@@ -105,6 +105,7 @@ def lib():
                   "mc_scalar_batch_solve", "mc_reset_fields"]:
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
+        L.mc_write_vtk.argtypes = [P_, C.c_char_p, C.c_int64, C.c_int]
         L.mc_get_phi_gradient.argtypes = [P_, C.c_int, C.c_int]
         L.mc_momentum_solve.argtypes = [P_, C.POINTER(SolveStatus)]
         L.mc_pressure_solve.argtypes = [P_, C.POINTER(SolveStatus)]
@@ -212,6 +213,11 @@ class Context:
 
     def get_phi_gradients(self):
         self._check(self.L.mc_get_phi_gradients(self.h), "mc_get_phi_gradients")
+
+    def write_vtk(self, prefix, timestep, rank=0):
+        """The reference's flow VTK file: <prefix>_flow<rank>_timestep<timestep>.vtk."""
+        self._check(self.L.mc_write_vtk(self.h, prefix.encode(), timestep, rank), "mc_write_vtk")
+        return "%s_flow%d_timestep%d.vtk" % (prefix, rank, timestep)
 
     def limit_phi_gradients(self):
         self._check(self.L.mc_limit_phi_gradients(self.h), "mc_limit_phi_gradients")

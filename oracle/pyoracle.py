@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libmc_oracle.so")
 REF_MESH_DUMP = os.path.join(HERE, "_ref", "ref_mesh_dump")
+REF_VTK_DUMP = os.path.join(HERE, "_ref", "ref_vtk_dump")
 
 U, V, W, P, PP, TE, ED, TP, TEM, FUL, PRO, VARF, VARP = range(13)
 NFIELDS = 13
@@ -243,6 +244,34 @@ def read_ref_mesh(path):
         out["cell_centers"] = take(np.float64, 3 * N).reshape(N, 3)
         out["boundary_types"] = take(np.uint64, 6)
     return out
+
+
+def vtk_test_fields(n, seed):
+    """Deterministic field values for the VTK writer tests: 11 x n values
+    spanning many magnitudes, with signs, zeros and integral values."""
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((11, n)) * 10.0 ** rng.integers(-9, 9, size=(11, n))
+    f[:, ::7] = 0.0
+    f[:, 3::11] = np.round(f[:, 3::11])
+    f[:, 5::13] = 50.0
+    return f
+
+
+def ref_vtk(m, fields, tmpdir, timestep=7):
+    """The reference's own VisitWriter::write_flow output (oracle/_ref) for
+    the box mesh m and fields (11 x N: U V W P TE ED TEM FUL PRO VARF VARP)."""
+    if not os.path.exists(REF_VTK_DUMP):
+        build_ref()
+    fields = np.ascontiguousarray(fields, dtype=np.float64)
+    inp = os.path.join(tmpdir, "vtk_fields.bin")
+    with open(inp, "wb") as f:
+        np.array([fields.shape[1]], dtype=np.int64).tofile(f)
+        fields.tofile(f)
+    prefix = os.path.join(tmpdir, "ref")
+    subprocess.run([REF_VTK_DUMP, str(m), inp, prefix, str(timestep)], check=True, timeout=600,
+                   stdout=subprocess.DEVNULL)
+    with open(prefix + "_flow0_timestep%d.vtk" % timestep, "rb") as f:
+        return f.read()
 
 
 def ref_mesh(m, tmpdir):

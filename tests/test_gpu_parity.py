@@ -307,3 +307,22 @@ def test_timesteps_limited(mode):
         assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "limited ts %d mass flux" % ts, 1e-8)
     o.close()
     g.close()
+
+
+def test_write_vtk_from_device(tmp_path):
+    """output_data: mc_write_vtk downloads the fields and writes the same bytes
+    as the host writer on those arrays (whose layout the reference's own
+    writer pins in tests/test_io_cpu.py); file name as VisitWriter.hpp:131."""
+    from tests.test_io_cpu import write_vtk
+    m = 4
+    print("vtk m=%d" % m, flush=True)
+    with mc.Context(m) as g:
+        g.timestep()
+        path = g.write_vtk(str(tmp_path / "minicombust"), 3)
+        assert path.endswith("minicombust_flow0_timestep3.vtk")
+        ours = open(path, "rb").read()
+        n = g.N
+        fields = [g.phi(f)[:n] for f in [po.U, po.V, po.W, po.P, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF,
+                                           po.VARP]]
+        cc = g.get(mc.ARR_CELL_CENTER).reshape(-1, 3)
+    assert ours == write_vtk(str(tmp_path / "host.vtk"), cc, fields)
