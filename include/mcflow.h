@@ -140,7 +140,9 @@ enum {
     MC_ARR_ROW_PTR = 64, MC_ARR_COL,                                 /* i32        */
     MC_ARR_A_VALS, MC_ARR_PA_VALS, MC_ARR_PB,                        /* f64        */
     MC_ARR_FACES = 72, MC_ARR_CELL_FACES, MC_ARR_CELL_NEIGHBOURS,    /* u64        */
-    MC_ARR_CELLS, MC_ARR_POINTS, MC_ARR_CELLS_PER_POINT              /* u64/f64/u8 */
+    MC_ARR_CELLS, MC_ARR_POINTS, MC_ARR_CELLS_PER_POINT,             /* u64/f64/u8 */
+    MC_ARR_PARTICLE_TERMS = 80                                       /* N x 5 f64: particle_aos
+                                                                        (momentum xyz, energy, fuel) */
 };
 
 /* Kernel classes timed with HIP events when profiling is on.  Every class but
@@ -258,6 +260,24 @@ int  mc_get_stats(mc_ctx *ctx, mc_stats *out);
 int  mc_state_flags(mc_ctx *ctx, int *first_pressure, int64_t *timestep_count);
 int  mc_set_state_flags(mc_ctx *ctx, int first_pressure, int64_t timestep_count);
 int  mc_sync(mc_ctx *ctx);
+
+/* ---- particle / flow coupling ----------------------------------------------
+   The flow side of update_flow_field (FlowSolver.inl:496-700) without its MPI
+   transport; the caller passes the arrays the reference receives and sends.
+   GPU reference: gpu/FlowSolver.inl:361-601, kernel_process_particle_fields
+   and kernel_interpolate_phi_to_nodes (gpu_kernels.cu:131-148, 2132-2191).  */
+/* memset of mesh->particle_terms (FlowSolver.inl:508).                       */
+int  mc_particle_terms_reset(mc_ctx *ctx);
+/* FlowSolver.inl:675-683: terms[5 i .. 5 i + 4] (particle_aos: momentum xyz,
+   energy, fuel) added to cell cells[i], in receive order (host arrays).
+   Result: mc_array_get(MC_ARR_PARTICLE_TERMS).                               */
+int  mc_particle_terms_add(mc_ctx *ctx, const uint64_t *cells, const double *terms, int64_t n);
+/* FlowSolver.inl:297-335 + interpolate_to_nodes (:428-493): values at the
+   requested nodes from the interpolation cell set `cells` (duplicates count
+   once; each node sums its cells in ascending order).  node_fields[5 k ..]:
+   flow_aos (vel xyz, pressure, temp) of nodes[k] (host arrays).              */
+int  mc_interpolate_to_nodes(mc_ctx *ctx, const uint64_t *cells, int64_t ncells, const uint64_t *nodes,
+                             int64_t nnodes, double *node_fields);
 
 /* ---- output ---------------------------------------------------------------
    FlowSolver<T>::output_data (FlowSolver.inl:18-23) ->

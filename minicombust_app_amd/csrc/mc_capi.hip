@@ -441,6 +441,9 @@ static int describe(Ctx *c, int id, ArrDesc &d)
     case MC_ARR_CELL_NEIGHBOURS: d.kind = 4; d.count = (int64_t)c->N * 6; d.a = c->nbr; return MC_OK;
     case MC_ARR_CELLS: d.kind = 5; d.ptr = c->cells; d.count = (int64_t)c->N * 8; return MC_OK;
     case MC_ARR_POINTS: d.kind = 7; d.count = (int64_t)c->NP * 3; d.sx = c->px; d.sy = c->py; d.sz = c->pz; return MC_OK;
+    case MC_ARR_PARTICLE_TERMS:
+        if (!c->pterms) { c->err = "no particle terms yet (mc_particle_terms_reset / _add)"; return MC_ERR_STATE; }
+        d.ptr = c->pterms; d.count = (int64_t)c->N * 5; return MC_OK;
     case MC_ARR_CELLS_PER_POINT: d.kind = 6; d.ptr = c->cells_per_point; d.count = c->NP; return MC_OK;
     default: c->err = "unknown array id"; return MC_ERR_ARG;
     }

@@ -138,6 +138,12 @@ struct Ctx {
     double *dscal = nullptr;       // device scalars for the solvers
     double *hscal = nullptr;       // pinned host mirror
     int clip_batch[2] = {8, 8};    // TE / ED repair: passes to launch before the first check
+    // particle/flow coupling (allocated on first use)
+    double *pterms = nullptr;      // [5N] particle_aos per cell: momentum xyz, energy, fuel
+    int *p2c_ptr = nullptr, *p2c_cell = nullptr;   // point -> cells, ascending
+    uint8_t *cell_flag = nullptr;  // [N] interpolation cell set
+    void *cpl_buf = nullptr;       // grow-only scratch of the coupling calls
+    size_t cpl_bytes = 0;
     int red_blocks = 0;
 
     // ---- AMG ----

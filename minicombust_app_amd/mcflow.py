@@ -28,11 +28,12 @@ ARR_FF0, ARR_FF1, ARR_MFLUX = 40, 41, 42
  ARR_CELL_CENTER, ARR_CELL_VOLUME, ARR_CELL_DENSITY) = range(48, 56)
 ARR_ROW_PTR, ARR_COL, ARR_A_VALS, ARR_PA_VALS, ARR_PB = range(64, 69)
 ARR_FACES, ARR_CELL_FACES, ARR_CELL_NEIGHBOURS, ARR_CELLS, ARR_POINTS, ARR_CELLS_PER_POINT = range(72, 78)
+ARR_PARTICLE_TERMS = 80
 
 KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "pcg_spmv64", "pcg_spmv32", "vcycle",
             "flux_correct", "mpnn_fwd", "jac64", "jac32", "bicg_spmv", "scalar_batch", "ir_resid", "mpnn_bwd"]
 # kernel symbol (as a profiler lists it) of each single-kernel class
-KC_KERNELS = {"grad11": "mc::k_grad<11>", "grad1": "mc::k_grad<1>", "mom_asm": "mc::k_momentum",
+KC_KERNELS = {"grad11": "mc::k_grad<11, false>", "grad1": "mc::k_grad<1, false>", "mom_asm": "mc::k_momentum",
               "p_asm": "mc::k_mass_pressure", "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<7, double>",
               "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
               "scalar_batch": "mc::k_scalar_batch", "ir_resid": "mc::k_ir_resid<7>"}
@@ -106,6 +107,10 @@ def lib():
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
         L.mc_write_vtk.argtypes = [P_, C.c_char_p, C.c_int64, C.c_int]
+        L.mc_particle_terms_reset.argtypes = [P_]
+        L.mc_particle_terms_add.argtypes = [P_, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int64]
+        L.mc_interpolate_to_nodes.argtypes = [P_, C.POINTER(C.c_uint64), C.c_int64, C.POINTER(C.c_uint64), C.c_int64,
+                                              C.POINTER(C.c_double)]
         L.mc_get_phi_gradient.argtypes = [P_, C.c_int, C.c_int]
         L.mc_momentum_solve.argtypes = [P_, C.POINTER(SolveStatus)]
         L.mc_pressure_solve.argtypes = [P_, C.POINTER(SolveStatus)]
@@ -219,6 +224,29 @@ class Context:
         self._check(self.L.mc_write_vtk(self.h, prefix.encode(), timestep, rank), "mc_write_vtk")
         return "%s_flow%d_timestep%d.vtk" % (prefix, rank, timestep)
 
+    def particle_terms_reset(self):
+        self._check(self.L.mc_particle_terms_reset(self.h), "mc_particle_terms_reset")
+
+    def particle_terms_add(self, cells, terms):
+        """terms: (n, 5) particle_aos rows (momentum xyz, energy, fuel) for cells (n,)."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint64)
+        terms = np.ascontiguousarray(terms, dtype=np.float64).reshape(-1, 5)
+        assert terms.shape[0] == cells.shape[0]
+        self._check(self.L.mc_particle_terms_add(self.h, cells.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                 terms.ctypes.data_as(C.POINTER(C.c_double)), cells.shape[0]),
+                    "mc_particle_terms_add")
+
+    def interpolate_to_nodes(self, cells, nodes):
+        """(len(nodes), 5) flow_aos rows (vel xyz, pressure, temp)."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint64)
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint64)
+        out = np.zeros((nodes.shape[0], 5))
+        self._check(self.L.mc_interpolate_to_nodes(self.h, cells.ctypes.data_as(C.POINTER(C.c_uint64)), cells.shape[0],
+                                                   nodes.ctypes.data_as(C.POINTER(C.c_uint64)), nodes.shape[0],
+                                                   out.ctypes.data_as(C.POINTER(C.c_double))),
+                    "mc_interpolate_to_nodes")
+        return out
+
     def limit_phi_gradients(self):
         self._check(self.L.mc_limit_phi_gradients(self.h), "mc_limit_phi_gradients")
 
@@ -305,6 +333,7 @@ class Context:
             ARR_FACES: (2 * self.F, np.uint64), ARR_CELL_FACES: (6 * self.N, np.uint64),
             ARR_CELL_NEIGHBOURS: (6 * self.N, np.uint64), ARR_CELLS: (8 * self.N, np.uint64),
             ARR_POINTS: (3 * self.NP, np.float64), ARR_CELLS_PER_POINT: (self.NP, np.uint8),
+            ARR_PARTICLE_TERMS: (5 * self.N, np.float64),
         }
         return table[aid]
 

@@ -1612,6 +1612,79 @@ void mco_scalar_solve(mco_state *s, int type, int field)
 }
 
 /* ======================================================================
+ * Particle -> flow coupling, the flow side of update_flow_field without its
+ * MPI transport (FlowSolver.inl:496-700)
+ * ====================================================================== */
+/* FlowSolver.inl:675-683: each received (cell, particle_aos) entry is added
+   to the cell's particle terms in receive order.  pt: [5 N] momentum x y z,
+   energy, fuel per cell (particle_aos<double>). */
+void mco_particle_terms_add(mco_state *s, int64_t n, const uint64_t *cells, const double *terms, double *pt)
+{
+    (void)s;
+    for (int64_t i = 0; i < n; i++) {
+        double *d = pt + 5 * cells[i];
+        const double *t = terms + 5 * i;
+        d[0] = d[0] + t[0]; d[1] = d[1] + t[1]; d[2] = d[2] + t[2];
+        d[3] += t[3];
+        d[4] += t[4];
+    }
+}
+
+/* Node values for the particle side: each node of the interpolation cells
+   starts from the mesh's dummy gas state weighted by its missing cells
+   (FlowSolver.inl:297-335, Mesh.hpp:114-116), then every cell of the set adds
+   (phi_c + grad phi_c . (x_node - x_c)) / 8 to each of its eight nodes
+   (interpolate_to_nodes, :428-493), cells in the order given (the reference
+   iterates an unordered_set; duplicates count once).  out: [5 nnodes]
+   flow_aos<double> (vel x y z, pressure, temp) of the requested nodes. */
+void mco_interpolate_to_nodes(mco_state *s, int64_t ncells, const uint64_t *cells, int64_t nnodes,
+                              const uint64_t *nodes, double *out)
+{
+    const mco_mesh *M = &s->mesh;
+    const double node_neighbours = 8;
+    uint8_t *want = calloc((size_t)M->n_points, 1);
+    uint8_t *seen = calloc((size_t)s->N, 1);
+    double *val = malloc(sizeof(double) * 5 * (size_t)M->n_points);
+    for (int64_t k = 0; k < nnodes; k++) {
+        const uint64_t nd = nodes[k];
+        want[nd] = 1;
+        const double b = node_neighbours - M->cells_per_point[nd];
+        double *o = val + 5 * nd;
+        o[0] = 50. * (b / node_neighbours);
+        o[1] = 0.00 * (b / node_neighbours);
+        o[2] = 0.00 * (b / node_neighbours);
+        o[3] = 4000.0 * (b / node_neighbours);
+        o[4] = 2000.0 * (b / node_neighbours);
+    }
+    for (int64_t i = 0; i < ncells; i++) {
+        const uint64_t c = cells[i];
+        if (seen[c]) continue;
+        seen[c] = 1;
+        const double T = s->phi[MCO_TEM][c], P = s->phi[MCO_P][c];
+        const double U = s->phi[MCO_U][c], V = s->phi[MCO_V][c], W = s->phi[MCO_W][c];
+        const mco_vec gT = s->grad[MCO_TEM][c], gP = s->grad[MCO_P][c];
+        const mco_vec gU = s->grad[MCO_U][c], gV = s->grad[MCO_V][c], gW = s->grad[MCO_W][c];
+        const mco_vec cc = M->cell_centers[c];
+        for (int n = 0; n < 8; n++) {
+            const uint64_t nd = M->cells[8 * c + n];
+            if (!want[nd]) continue;
+            const mco_vec d = vsub(M->points[nd], cc);
+            double *o = val + 5 * nd;
+            o[4] += (T + (gT.x * d.x + gT.y * d.y + gT.z * d.z)) / node_neighbours;
+            o[3] += (P + (gP.x * d.x + gP.y * d.y + gP.z * d.z)) / node_neighbours;
+            o[0] += (U + (gU.x * d.x + gU.y * d.y + gU.z * d.z)) / node_neighbours;
+            o[1] += (V + (gV.x * d.x + gV.y * d.y + gV.z * d.z)) / node_neighbours;
+            o[2] += (W + (gW.x * d.x + gW.y * d.y + gW.z * d.z)) / node_neighbours;
+        }
+    }
+    for (int64_t k = 0; k < nnodes; k++)
+        for (int q = 0; q < 5; q++) out[5 * k + q] = val[5 * nodes[k] + q];
+    free(want);
+    free(seen);
+    free(val);
+}
+
+/* ======================================================================
  * A17: FlowSolver::timestep (FlowSolver.inl:3148-3356)
  * ====================================================================== */
 void mco_simple_iteration(mco_state *s, int inner)

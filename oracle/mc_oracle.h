@@ -131,6 +131,9 @@ void mco_destroy(mco_state *s);
 /* ---- stages (names follow FlowSolver.inl) ---- */
 void mco_get_phi_gradients(mco_state *s);
 void mco_limit_phi_gradients(mco_state *s);
+void mco_particle_terms_add(mco_state *s, int64_t n, const uint64_t *cells, const double *terms, double *pt);
+void mco_interpolate_to_nodes(mco_state *s, int64_t ncells, const uint64_t *cells, int64_t nnodes,
+                              const uint64_t *nodes, double *out);
 void mco_get_phi_gradient(mco_state *s, int field, int pressure);
 void mco_precomp_AU(mco_state *s);
 void mco_set_up_field(mco_state *s);

@@ -326,3 +326,43 @@ def test_write_vtk_from_device(tmp_path):
                                            po.VARP]]
         cc = g.get(mc.ARR_CELL_CENTER).reshape(-1, 3)
     assert ours == write_vtk(str(tmp_path / "host.vtk"), cc, fields)
+
+
+def test_particle_coupling_bits():
+    """Flow side of update_flow_field (FlowSolver.inl:496-700): particle terms
+    added in receive order over two calls, and node values interpolated from a
+    cell set, bit for bit against the oracle (which takes the set in ascending
+    order, the order each node sums its cells in)."""
+    import ctypes as C
+    m = 4
+    print("particle coupling m=%d" % m, flush=True)
+    o, g = make_pair(m)
+    o.timestep()
+    push_state(o, g)
+    rng = np.random.default_rng(21)
+    up = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64))   # noqa: E731
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))   # noqa: E731
+    pt = np.zeros((o.N, 5))
+    g.particle_terms_reset()
+    for call in range(2):
+        n = 3000
+        cells = rng.integers(0, o.N, size=n).astype(np.uint64)
+        terms = rng.standard_normal((n, 5)) * 10.0 ** rng.integers(-6, 6, size=(n, 5))
+        o.L.mco_particle_terms_add(o.sp, n, up(cells), dp(terms), dp(pt))
+        g.particle_terms_add(cells, terms)
+        assert_bits(g.get(mc.ARR_PARTICLE_TERMS).reshape(-1, 5), pt, "particle terms after call %d" % call)
+    g.particle_terms_reset()
+    assert not np.any(g.get(mc.ARR_PARTICLE_TERMS))
+    # interpolation set: a random cluster of cells and their face neighbours
+    M = o.mesh_arrays()
+    seed_cells = rng.choice(o.N, size=12, replace=False)
+    nb = M["cell_neighbours"][seed_cells].reshape(-1)
+    cset = np.unique(np.concatenate([seed_cells, nb[nb < o.N]])).astype(np.uint64)
+    nodes = np.unique(M["cells"][cset.astype(np.int64)].reshape(-1)).astype(np.uint64)
+    nodes = np.concatenate([nodes, np.array([0, M["points"].shape[0] - 1], dtype=np.uint64)])
+    out = np.zeros((nodes.size, 5))
+    o.L.mco_interpolate_to_nodes(o.sp, cset.size, up(cset), nodes.size, up(nodes), dp(out))
+    shuffled = rng.permutation(np.concatenate([cset, cset[:5]]))
+    assert_bits(g.interpolate_to_nodes(shuffled, nodes), out, "node values")
+    o.close()
+    g.close()

@@ -4,6 +4,7 @@ properties of the least-squares gradient, invariants of the assembled
 systems, an independent sparse solver, and the mass conservation the pressure
 correction must deliver.  None of these needs a GPU.
 """
+import ctypes as C
 import os
 import tempfile
 
@@ -391,3 +392,68 @@ def test_limiter_leaves_linear_fields_and_flattens_extrema():
     assert np.abs(o.grad(po.U)[spike]).max() > 1.0
     o.limit_phi_gradients()
     assert np.all(o.grad(po.U)[spike] == 0.0)
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def test_particle_terms_add_in_receive_order():
+    """FlowSolver.inl:675-683: entries add to their cell in receive order,
+    which numpy's unbuffered np.add.at reproduces addition for addition."""
+    o = po.Oracle(3)
+    rng = np.random.default_rng(4)
+    n = 500
+    cells = rng.integers(0, o.N, size=n).astype(np.uint64)
+    terms = rng.standard_normal((n, 5)) * 10.0 ** rng.integers(-6, 6, size=(n, 5))
+    pt = np.zeros((o.N, 5))
+    o.L.mco_particle_terms_add(o.sp, n, _u64p(cells), _dp(terms), _dp(pt))
+    ref = np.zeros((o.N, 5))
+    np.add.at(ref, cells.astype(np.int64), terms)
+    assert np.array_equal(pt.view(np.uint64), ref.view(np.uint64))
+
+
+def test_node_interpolation_closed_forms():
+    """interpolate_to_nodes (FlowSolver.inl:297-335, 428-493): with exact
+    gradients of linear fields every cell contributes phi(x_node) / 8, so a node
+    whose cells are all in the set gets phi(x_node) plus the dummy gas state
+    (Mesh.hpp:114-116) times its missing cells / 8; a node outside the set
+    keeps only that dummy term."""
+    o = po.Oracle(3)
+    M = o.mesh_arrays()
+    N = o.N
+    cc, pts = M["cell_centers"], M["points"]
+    cpp = M["cells_per_point"].astype(np.float64)
+    rng = np.random.default_rng(6)
+    coef = {}
+    for f in [po.U, po.V, po.W, po.P, po.TEM]:
+        a = rng.standard_normal(3)
+        b = rng.standard_normal()
+        coef[f] = (a, b)
+        o.phi(f)[:N] = cc @ a + b
+        o.grad(f)[:N] = a
+    cells = np.arange(N, dtype=np.uint64)[::-1].copy()       # every cell, reversed order
+    nodes = np.arange(pts.shape[0], dtype=np.uint64)
+    out = np.zeros((nodes.size, 5))
+    o.L.mco_interpolate_to_nodes(o.sp, cells.size, _u64p(cells), nodes.size, _u64p(nodes), _dp(out))
+    dummy = np.array([50.0, 0.0, 0.0, 4000.0, 2000.0])
+    for q, f in enumerate([po.U, po.V, po.W, po.P, po.TEM]):
+        a, b = coef[f]
+        expect = dummy[q] * ((8.0 - cpp) / 8.0) + cpp / 8.0 * (pts @ a + b)
+        np.testing.assert_allclose(out[:, q], expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max())
+    # only the first cell in the set: its 8 nodes get 1/8 of phi, all others the dummy term
+    one = np.array([5], dtype=np.uint64)
+    out1 = np.zeros((nodes.size, 5))
+    o.L.mco_interpolate_to_nodes(o.sp, 1, _u64p(one), nodes.size, _u64p(nodes), _dp(out1))
+    mine = set(int(v) for v in M["cells"][5])
+    for k in range(nodes.size):
+        base = dummy * ((8.0 - cpp[k]) / 8.0)
+        if k in mine:
+            phi_n = np.array([pts[k] @ coef[f][0] + coef[f][1] for f in [po.U, po.V, po.W, po.P, po.TEM]])
+            np.testing.assert_allclose(out1[k], base + phi_n / 8.0, rtol=1e-12, atol=1e-12)
+        else:
+            assert np.array_equal(out1[k], base)
