@@ -401,7 +401,20 @@ struct ArrDesc {
                      // 5 u64 plain, 6 u8, 7 points x3 SoA, 8 u64 cell_faces from i32
     const double *sx, *sy, *sz;
     const int *a, *b;
+    bool face;       // one entry per face, stored in the context's face order
 };
+
+// reference face id <-> storage position, host copies made on first use
+static int face_maps(Ctx *c)
+{
+    if ((int64_t)c->h_face_pos.size() == c->F) return MC_OK;
+    c->h_face_pos.resize(c->F);
+    c->h_face_ref.resize(c->F);
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    MC_HIP(c, hipMemcpy(c->h_face_pos.data(), c->face_pos, (size_t)c->F * 4, hipMemcpyDeviceToHost));
+    MC_HIP(c, hipMemcpy(c->h_face_ref.data(), c->face_ref, (size_t)c->F * 4, hipMemcpyDeviceToHost));
+    return MC_OK;
+}
 
 static int describe(Ctx *c, int id, ArrDesc &d)
 {
@@ -420,14 +433,14 @@ static int describe(Ctx *c, int id, ArrDesc &d)
     case MC_ARR_SU: d.ptr = c->SU; d.count = NS; return MC_OK;
     case MC_ARR_SV: d.ptr = c->SV; d.count = NS; return MC_OK;
     case MC_ARR_SW: d.ptr = c->SW; d.count = NS; return MC_OK;
-    case MC_ARR_FF0: d.ptr = c->ff0; d.count = c->F; return MC_OK;
-    case MC_ARR_FF1: d.ptr = c->ff1; d.count = c->F; return MC_OK;
-    case MC_ARR_MFLUX: d.ptr = c->mflux; d.count = c->F; return MC_OK;
-    case MC_ARR_FACE_AREA: d.ptr = c->farea; d.count = c->F; return MC_OK;
-    case MC_ARR_FACE_LAMBDA: d.ptr = c->flam; d.count = c->F; return MC_OK;
-    case MC_ARR_FACE_RLENCOS: d.ptr = c->frlc; d.count = c->F; return MC_OK;
-    case MC_ARR_FACE_NORMAL: d.kind = 1; d.count = (int64_t)c->F * 3; d.sx = c->fnx; d.sy = c->fny; d.sz = c->fnz; return MC_OK;
-    case MC_ARR_FACE_CENTER: d.kind = 1; d.count = (int64_t)c->F * 3; d.sx = c->fcx; d.sy = c->fcy; d.sz = c->fcz; return MC_OK;
+    case MC_ARR_FF0: d.ptr = c->ff0; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_FF1: d.ptr = c->ff1; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_MFLUX: d.ptr = c->mflux; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_FACE_AREA: d.ptr = c->farea; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_FACE_LAMBDA: d.ptr = c->flam; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_FACE_RLENCOS: d.ptr = c->frlc; d.count = c->F; d.face = true; return MC_OK;
+    case MC_ARR_FACE_NORMAL: d.kind = 1; d.count = (int64_t)c->F * 3; d.sx = c->fnx; d.sy = c->fny; d.sz = c->fnz; d.face = true; return MC_OK;
+    case MC_ARR_FACE_CENTER: d.kind = 1; d.count = (int64_t)c->F * 3; d.sx = c->fcx; d.sy = c->fcy; d.sz = c->fcz; d.face = true; return MC_OK;
     case MC_ARR_CELL_CENTER: d.kind = 1; d.count = (int64_t)c->N * 3; d.sx = c->cx; d.sy = c->cy; d.sz = c->cz; return MC_OK;
     case MC_ARR_CELL_VOLUME: d.ptr = c->vol; d.count = c->N; return MC_OK;
     case MC_ARR_CELL_DENSITY: d.ptr = c->rho; d.count = c->N; return MC_OK;
@@ -436,7 +449,7 @@ static int describe(Ctx *c, int id, ArrDesc &d)
     case MC_ARR_A_VALS: d.ptr = c->A_vals; d.count = c->nnz; return MC_OK;
     case MC_ARR_PA_VALS: d.ptr = c->PA_vals; d.count = c->nnz; return MC_OK;
     case MC_ARR_PB: d.ptr = c->Pb; d.count = c->N; return MC_OK;
-    case MC_ARR_FACES: d.kind = 3; d.count = (int64_t)c->F * 2; d.a = c->fc0; d.b = c->fc1; return MC_OK;
+    case MC_ARR_FACES: d.kind = 3; d.count = (int64_t)c->F * 2; d.a = c->fc0; d.b = c->fc1; d.face = true; return MC_OK;
     case MC_ARR_CELL_FACES: d.kind = 8; d.count = (int64_t)c->N * 6; d.a = c->cell_faces; return MC_OK;
     case MC_ARR_CELL_NEIGHBOURS: d.kind = 4; d.count = (int64_t)c->N * 6; d.a = c->nbr; return MC_OK;
     case MC_ARR_CELLS: d.kind = 5; d.ptr = c->cells; d.count = (int64_t)c->N * 8; return MC_OK;
@@ -456,8 +469,15 @@ int mc_array_get(mc_ctx *x, int id, void *host, int64_t count)
     MC_TRY(describe(c, id, d));
     if (!host || count != d.count) { c->err = "array size mismatch (expected " + std::to_string(d.count) + ")"; return MC_ERR_ARG; }
     MC_HIP(c, hipStreamSynchronize(c->stream));
-    if (d.kind == 0) {
+    if (d.face || d.kind == 8) MC_TRY(face_maps(c));
+    const int *pos = d.face ? c->h_face_pos.data() : nullptr;   // reference id -> stored entry
+    if (d.kind == 0 && !d.face) {
         MC_HIP(c, hipMemcpy(host, d.ptr, (size_t)count * 8, hipMemcpyDeviceToHost));
+    } else if (d.kind == 0) {
+        std::vector<double> t(count);
+        MC_HIP(c, hipMemcpy(t.data(), d.ptr, (size_t)count * 8, hipMemcpyDeviceToHost));
+        double *o = (double *)host;
+        for (int64_t r = 0; r < count; r++) o[r] = t[pos[r]];
     } else if (d.kind == 1 || d.kind == 7) {
         const int64_t n = count / 3;
         std::vector<double> t(n);
@@ -465,7 +485,7 @@ int mc_array_get(mc_ctx *x, int id, void *host, int64_t count)
         const double *src[3] = {d.sx, d.sy, d.sz};
         for (int k = 0; k < 3; k++) {
             MC_HIP(c, hipMemcpy(t.data(), src[k], (size_t)n * 8, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; i++) o[3 * i + k] = t[i];
+            for (int64_t i = 0; i < n; i++) o[3 * i + k] = t[pos ? pos[i] : i];
         }
     } else if (d.kind == 2) {
         MC_HIP(c, hipMemcpy(host, d.ptr, (size_t)count * 4, hipMemcpyDeviceToHost));
@@ -475,12 +495,19 @@ int mc_array_get(mc_ctx *x, int id, void *host, int64_t count)
         MC_HIP(c, hipMemcpy(a.data(), d.a, (size_t)n * 4, hipMemcpyDeviceToHost));
         MC_HIP(c, hipMemcpy(b.data(), d.b, (size_t)n * 4, hipMemcpyDeviceToHost));
         uint64_t *o = (uint64_t *)host;
-        for (int64_t i = 0; i < n; i++) { o[2 * i] = (uint64_t)a[i]; o[2 * i + 1] = (uint64_t)b[i]; }
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t j = pos ? pos[i] : i;
+            o[2 * i] = (uint64_t)a[j];
+            o[2 * i + 1] = (uint64_t)b[j];
+        }
     } else if (d.kind == 4 || d.kind == 8) {
         std::vector<int> a(count);
         MC_HIP(c, hipMemcpy(a.data(), d.a, (size_t)count * 4, hipMemcpyDeviceToHost));
         uint64_t *o = (uint64_t *)host;
-        for (int64_t i = 0; i < count; i++) o[i] = a[i] < 0 ? UINT64_MAX : (uint64_t)a[i];
+        if (d.kind == 8)   // storage positions -> reference face ids
+            for (int64_t i = 0; i < count; i++) o[i] = (uint64_t)c->h_face_ref[a[i]];
+        else
+            for (int64_t i = 0; i < count; i++) o[i] = a[i] < 0 ? UINT64_MAX : (uint64_t)a[i];
     } else if (d.kind == 5) {
         MC_HIP(c, hipMemcpy(host, d.ptr, (size_t)count * 8, hipMemcpyDeviceToHost));
     } else if (d.kind == 6) {
@@ -496,15 +523,22 @@ int mc_array_set(mc_ctx *x, int id, const void *host, int64_t count)
     MC_TRY(describe(c, id, d));
     if (!host || count != d.count) { c->err = "array size mismatch (expected " + std::to_string(d.count) + ")"; return MC_ERR_ARG; }
     MC_HIP(c, hipStreamSynchronize(c->stream));
-    if (d.kind == 0) {
+    if (d.face) MC_TRY(face_maps(c));
+    const int *pos = d.face ? c->h_face_pos.data() : nullptr;
+    if (d.kind == 0 && !d.face) {
         MC_HIP(c, hipMemcpy(d.ptr, host, (size_t)count * 8, hipMemcpyHostToDevice));
+    } else if (d.kind == 0) {
+        std::vector<double> t(count);
+        const double *in = (const double *)host;
+        for (int64_t r = 0; r < count; r++) t[pos[r]] = in[r];
+        MC_HIP(c, hipMemcpy(d.ptr, t.data(), (size_t)count * 8, hipMemcpyHostToDevice));
     } else if (d.kind == 1) {
         const int64_t n = count / 3;
         std::vector<double> t(n);
         const double *in = (const double *)host;
         const double *dst[3] = {d.sx, d.sy, d.sz};
         for (int k = 0; k < 3; k++) {
-            for (int64_t i = 0; i < n; i++) t[i] = in[3 * i + k];
+            for (int64_t i = 0; i < n; i++) t[pos ? pos[i] : i] = in[3 * i + k];
             MC_HIP(c, hipMemcpy((void *)dst[k], t.data(), (size_t)n * 8, hipMemcpyHostToDevice));
         }
     } else {

@@ -79,6 +79,15 @@ struct Ctx {
     int64_t m = 0;              // box modifier (0 when the mesh came from the host)
     int N = 0, F = 0, NP = 0, Fb = 0;
     int *fc0 = nullptr, *fc1 = nullptr;     // [F] face cells; boundary: fc1 = N + side
+    // Face arrays are stored in an order of their own: on the generated box
+    // side-major (x faces, y faces, z faces by creating cell, then the three
+    // low boundary sides), so the k-th face of 64 consecutive cells is 64
+    // consecutive entries; elsewhere in reference order.  Face indices held in
+    // device arrays (cell_faces, forder, bfaces, last_face) are storage
+    // positions; sums over faces still run in reference face-id order.
+    int *face_ref = nullptr;                 // [F] reference face id of each storage position
+    int *face_pos = nullptr;                 // [F] storage position of each reference face id
+    std::vector<int> h_face_pos, h_face_ref; // host copies for array get / set
     int *cell_faces = nullptr;               // [6N] side order (FRONT..UP)
     int *forder = nullptr;                   // [6N] the cell's faces in ascending face id
     int *fother = nullptr;                   // [6N] other cell of each forder face (boundary: N + side),

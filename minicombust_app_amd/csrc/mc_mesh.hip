@@ -47,8 +47,25 @@ __device__ __forceinline__ int64_t faces_before(int64_t c, int64_t nx, int64_t n
     return 3 * c + z0 + x0 + y0;
 }
 
+// Storage position of a face in the side-major order: the RIGHT, UP and BACK
+// faces of cell c at c, N + c and 2N + c, then the LEFT (x = 0), DOWN (y = 0)
+// and FRONT (z = 0) boundary faces by their cell's position on that side.
+__device__ __forceinline__ int64_t box_face_pos(int side, int64_t c, int64_t x, int64_t y, int64_t z, int64_t nx,
+                                                int64_t ny, int64_t nz, int64_t N)
+{
+    const int64_t nxny = nx * ny;
+    switch (side) {
+    case 3: return c;                                       // RIGHT
+    case 5: return N + c;                                   // UP
+    case 1: return 2 * N + c;                               // BACK
+    case 2: return x > 0 ? c - 1 : 3 * N + y + ny * z;      // LEFT: RIGHT face of x - 1
+    case 4: return y > 0 ? N + c - nx : 3 * N + ny * nz + x + nx * z;                   // DOWN
+    default: return z > 0 ? 2 * N + c - nxny : 3 * N + ny * nz + nx * nz + x + nx * y;  // FRONT
+    }
+}
+
 __global__ void k_box_cells(int64_t N, int64_t nx, int64_t ny, int64_t nz, uint64_t *cells, int *nbr,
-                            int *cell_faces, int *fc0, int *fc1)
+                            int *cell_faces, int *fc0, int *fc1, int *face_ref, int *face_pos)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
@@ -78,12 +95,15 @@ __global__ void k_box_cells(int64_t N, int64_t nx, int64_t ny, int64_t nz, uint6
         id[4] = faces_before(cp, nx, nxny) + (z == 0) + 1 + (x == 0) + 1 + ((y - 1) == 0);
     }
     for (int f = 0; f < 6; f++) {
+        const int64_t pos = box_face_pos(f, c, x, y, z, nx, ny, nz, N);
         nbr[6 * c + f] = (int)nb[f];
-        cell_faces[6 * c + f] = (int)id[f];
+        cell_faces[6 * c + f] = (int)pos;
         if (created[f]) {
             const int64_t other = nb[f] < 0 ? N + f : nb[f];
-            fc0[id[f]] = (int)(c < other ? c : other);
-            fc1[id[f]] = (int)(c < other ? other : c);
+            fc0[pos] = (int)(c < other ? c : other);
+            fc1[pos] = (int)(c < other ? other : c);
+            face_ref[pos] = (int)id[f];
+            face_pos[id[f]] = (int)pos;
         }
     }
 }
@@ -231,8 +251,8 @@ __global__ void k_row_count(int N, const int *nbr, int *cnt)
 // Row = {c} u {interior neighbours}, columns ascending (MatSeqAIJ after
 // MatSetValue of FlowSolver.inl:1221-1266).  forder = the cell's faces in
 // ascending face id: the order in which the reference's face loops visit them.
-__global__ void k_row_fill(int N, const int *nbr, const int *cell_faces, const int *row_ptr, int *col, int *diag_idx,
-                           int *forder)
+__global__ void k_row_fill(int N, const int *nbr, const int *cell_faces, const int *face_ref, const int *row_ptr,
+                           int *col, int *diag_idx, int *forder)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
@@ -251,13 +271,13 @@ __global__ void k_row_fill(int N, const int *nbr, const int *cell_faces, const i
         col[rp + k] = cols[k];
         if (cols[k] == c) diag_idx[c] = rp + k;
     }
-    int fo[6];
-    for (int s = 0; s < 6; s++) fo[s] = cell_faces[6 * c + s];
-    for (int a = 1; a < 6; a++) {
-        const int v = fo[a];
+    int fo[6], id[6];
+    for (int s = 0; s < 6; s++) { fo[s] = cell_faces[6 * c + s]; id[s] = face_ref[fo[s]]; }
+    for (int a = 1; a < 6; a++) {   // by reference face id
+        const int v = fo[a], iv = id[a];
         int b = a - 1;
-        while (b >= 0 && fo[b] > v) { fo[b + 1] = fo[b]; b--; }
-        fo[b + 1] = v;
+        while (b >= 0 && id[b] > iv) { fo[b + 1] = fo[b]; id[b + 1] = id[b]; b--; }
+        fo[b + 1] = v; id[b + 1] = iv;
     }
     for (int s = 0; s < 6; s++) forder[6 * c + s] = fo[s];
 }
@@ -273,13 +293,16 @@ __global__ void k_face_other(int N, const int *forder, const int *fc0, const int
     fother[i] = (c0 == c) ? c1 : (int)((unsigned)c0 | 0x80000000u);
 }
 
-__global__ void k_boundary_flags(int F, int N, const int *fc1, uint8_t *flag, int *last_face)
+// flags and last faces by reference face id r (the selection below then lists
+// boundary faces in reference order, as storage positions)
+__global__ void k_boundary_flags(int F, int N, const int *fc1, const int *face_pos, uint8_t *flag, int *last_ref)
 {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= F) return;
+    const int f = face_pos[r];
     const int b = fc1[f] >= N;
-    flag[f] = (uint8_t)b;
-    if (b) atomicMax(&last_face[fc1[f] - N], f);
+    flag[r] = (uint8_t)b;
+    if (b) atomicMax(&last_ref[fc1[f] - N], r);
 }
 
 __global__ void k_wall_cells(int Fb, int N, const int *bfaces, const int *fc0, const int *fc1, const int *btype,
@@ -318,11 +341,12 @@ int mesh_box_generate(Ctx *c, int64_t m)
     MC_TRY(dalloc(c, &c->nbr, 6 * N));
     MC_TRY(dalloc(c, &c->cell_faces, 6 * N));
     MC_TRY(dalloc(c, &c->fc0, F)); MC_TRY(dalloc(c, &c->fc1, F));
+    MC_TRY(dalloc(c, &c->face_ref, F)); MC_TRY(dalloc(c, &c->face_pos, F));
     hipLaunchKernelGGL(k_box_points, dim3(div_up(NP, 256)), dim3(256), 0, c->stream, NP, ex + 1, ey + 1, ez + 1, edx, edy,
                        edz, c->px, c->py, c->pz, c->cells_per_point);
     MC_LAUNCH_CHECK(c);
     hipLaunchKernelGGL(k_box_cells, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, ex, ey, ez, c->cells, c->nbr,
-                       c->cell_faces, c->fc0, c->fc1);
+                       c->cell_faces, c->fc0, c->fc1, c->face_ref, c->face_pos);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -344,6 +368,9 @@ int mesh_from_host(Ctx *c, const mc_mesh_desc *d)
     MC_TRY(dalloc(c, &c->nbr, 6 * N));
     MC_TRY(dalloc(c, &c->cell_faces, 6 * N));
     MC_TRY(dalloc(c, &c->fc0, F)); MC_TRY(dalloc(c, &c->fc1, F));
+    MC_TRY(dalloc(c, &c->face_ref, F)); MC_TRY(dalloc(c, &c->face_pos, F));
+    hipLaunchKernelGGL(k_iota, dim3(div_up(F, 256)), dim3(256), 0, c->stream, (int)F, c->face_ref);
+    hipLaunchKernelGGL(k_iota, dim3(div_up(F, 256)), dim3(256), 0, c->stream, (int)F, c->face_pos);
     void *tmp = nullptr;
     const size_t tb = std::max<size_t>({(size_t)NP * 24, (size_t)F * 16, (size_t)N * 48});
     MC_HIP(c, hipMalloc(&tmp, tb));
@@ -492,8 +519,8 @@ int mesh_finish(Ctx *c)
     MC_TRY(dalloc(c, &c->u, N));
     MC_HIP(c, hipMemsetAsync(c->A_vals, 0, (size_t)nnz * 8, c->stream));
     MC_HIP(c, hipMemsetAsync(c->PA_vals, 0, (size_t)nnz * 8, c->stream));
-    hipLaunchKernelGGL(k_row_fill, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->nbr, c->cell_faces, c->row_ptr,
-                       c->col, c->diag_idx, c->forder);
+    hipLaunchKernelGGL(k_row_fill, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->nbr, c->cell_faces, c->face_ref,
+                       c->row_ptr, c->col, c->diag_idx, c->forder);
     MC_TRY(dalloc(c, &c->fother, 6 * (int64_t)N));
     hipLaunchKernelGGL(k_face_other, dim3(div_up(6 * (int64_t)N, 256)), dim3(256), 0, c->stream, N, c->forder, c->fc0,
                        c->fc1, c->fother);
@@ -501,28 +528,31 @@ int mesh_finish(Ctx *c)
 
     // boundary faces, last face per side, wall cells
     uint8_t *flag = nullptr;
-    int *d_last = nullptr, *d_idx = nullptr, *d_cnt = nullptr;
+    int *d_last = nullptr, *d_cnt = nullptr;
     MC_HIP(c, hipMalloc(&flag, F));
     MC_HIP(c, hipMalloc(&d_last, 6 * sizeof(int)));
-    MC_HIP(c, hipMalloc(&d_idx, (size_t)F * sizeof(int)));
     MC_HIP(c, hipMalloc(&d_cnt, sizeof(int)));
     int neg[6] = {-1, -1, -1, -1, -1, -1};
     MC_HIP(c, hipMemcpyAsync(d_last, neg, sizeof(neg), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_boundary_flags, dim3(div_up(F, 256)), dim3(256), 0, c->stream, F, N, c->fc1, flag, d_last);
+    hipLaunchKernelGGL(k_boundary_flags, dim3(div_up(F, 256)), dim3(256), 0, c->stream, F, N, c->fc1, c->face_pos, flag,
+                       d_last);
     MC_LAUNCH_CHECK(c);
-    hipLaunchKernelGGL(k_iota, dim3(div_up(F, 256)), dim3(256), 0, c->stream, F, d_idx);
     MC_TRY(dalloc(c, &c->bfaces, F));
     tmpb = 0;
-    MC_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tmpb, d_idx, flag, c->bfaces, d_cnt, F, c->stream));
+    MC_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tmpb, c->face_pos, flag, c->bfaces, d_cnt, F, c->stream));
     MC_HIP(c, hipMalloc(&tmps, tmpb + 16));
-    MC_HIP(c, hipcub::DeviceSelect::Flagged(tmps, tmpb, d_idx, flag, c->bfaces, d_cnt, F, c->stream));
+    MC_HIP(c, hipcub::DeviceSelect::Flagged(tmps, tmpb, c->face_pos, flag, c->bfaces, d_cnt, F, c->stream));
+    int last_ref[6];
     MC_HIP(c, hipMemcpyAsync(&c->Fb, d_cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    MC_HIP(c, hipMemcpyAsync(c->last_face, d_last, sizeof(neg), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipMemcpyAsync(last_ref, d_last, sizeof(neg), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
     for (int s = 0; s < 6; s++) {
+        c->last_face[s] = -1;
         c->last_cell[s] = -1;
-        if (c->last_face[s] >= 0)
+        if (last_ref[s] >= 0) {
+            MC_HIP(c, hipMemcpy(&c->last_face[s], c->face_pos + last_ref[s], sizeof(int), hipMemcpyDeviceToHost));
             MC_HIP(c, hipMemcpy(&c->last_cell[s], c->fc0 + c->last_face[s], sizeof(int), hipMemcpyDeviceToHost));
+        }
     }
     MC_TRY(dalloc(c, &c->is_wall_cell, N));
     MC_HIP(c, hipMemsetAsync(c->is_wall_cell, 0, N, c->stream));
@@ -535,7 +565,6 @@ int mesh_finish(Ctx *c)
     MC_HIP(c, hipFree(tmps));
     MC_HIP(c, hipFree(flag));
     MC_HIP(c, hipFree(d_last));
-    MC_HIP(c, hipFree(d_idx));
     MC_HIP(c, hipFree(d_cnt));
     (void)CFVM;
     return MC_OK;
