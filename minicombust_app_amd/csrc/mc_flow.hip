@@ -722,32 +722,40 @@ int calculate_mass_flux_assemble(Ctx *c)
 __global__ void __launch_bounds__(256) k_flux_correct(Geo g, const double *PP, const double *gPPx, const double *gPPy,
                                                       const double *gPPz, const double *ff0, double *mflux, double *fcorr)
 {
-    const int f = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (f >= g.F) return;
-    const int c0 = g.fc0[f], c1 = g.fc1[f];
-    if (c1 >= g.N) return;
-    const double fact = ff0[f];
-    double m = mflux[f];
-    m += (fact * (PP[c1] - PP[c0]));
-    const dv3 fc = ld3(g.fcx, g.fcy, g.fcz, f), cc0 = ld3(g.cx, g.cy, g.cz, c0), cc1 = ld3(g.cx, g.cy, g.cz, c1);
-    const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
-    const dv3 nh = n / mag(n);
-    const dv3 Xpac = fc - nh * dot(fc - cc0, nh);
-    const dv3 Xnac = fc - nh * dot(fc - cc1, nh);
-    dv3 Xn = Xnac - cc1, Xp = Xpac - cc0;
-    if (fabs(Xn.x) < 0.0000000000000003) Xn.x = 0.0;
-    if (fabs(Xn.y) < 0.0000000000000003) Xn.y = 0.0;
-    if (fabs(Xn.z) < 0.0000000000000003) Xn.z = 0.0;
-    if (fabs(Xp.x) < 0.0000000000000003) Xp.x = 0.0;
-    if (fabs(Xp.y) < 0.0000000000000003) Xp.y = 0.0;
-    if (fabs(Xp.z) < 0.0000000000000003) Xp.z = 0.0;
-    const double dpx = gPPx[c1] * Xn.x - gPPx[c0] * Xp.x;
-    const double dpy = gPPy[c1] * Xn.y - gPPy[c0] * Xp.y;
-    const double dpz = gPPz[c1] * Xn.z - gPPz[c0] * Xp.z;
-    const double fcv = fact * (dpx + dpy + dpz) * 0.8;
-    m += fcv;
-    mflux[f] = m;
-    fcorr[f] = fcv;
+    // one thread per cell, over the interior faces it owns (it is their
+    // cell0): the cell's values are read once for its faces
+    const int c = walk_cell(g.w);
+    if (c < 0) return;
+    const dv3 cc0 = ld3(g.cx, g.cy, g.cz, c), gP0 = ld3(gPPx, gPPy, gPPz, c);
+    const double P0 = PP[c];
+    for (int k = 0; k < 6; k++) {
+        const int c1 = g.fother[6 * c + k];
+        if (c1 < 0 || c1 >= g.N) continue;   // not this cell's face, or a boundary face
+        const int f = g.forder[6 * c + k];
+        const double fact = ff0[f];
+        double m = mflux[f];
+        m += (fact * (PP[c1] - P0));
+        const dv3 fc = ld3(g.fcx, g.fcy, g.fcz, f), cc1 = ld3(g.cx, g.cy, g.cz, c1);
+        const dv3 n = ld3(g.fnx, g.fny, g.fnz, f);
+        const dv3 nh = n / mag(n);
+        const dv3 Xpac = fc - nh * dot(fc - cc0, nh);
+        const dv3 Xnac = fc - nh * dot(fc - cc1, nh);
+        dv3 Xn = Xnac - cc1, Xp = Xpac - cc0;
+        if (fabs(Xn.x) < 0.0000000000000003) Xn.x = 0.0;
+        if (fabs(Xn.y) < 0.0000000000000003) Xn.y = 0.0;
+        if (fabs(Xn.z) < 0.0000000000000003) Xn.z = 0.0;
+        if (fabs(Xp.x) < 0.0000000000000003) Xp.x = 0.0;
+        if (fabs(Xp.y) < 0.0000000000000003) Xp.y = 0.0;
+        if (fabs(Xp.z) < 0.0000000000000003) Xp.z = 0.0;
+        const dv3 gP1 = ld3(gPPx, gPPy, gPPz, c1);
+        const double dpx = gP1.x * Xn.x - gP0.x * Xp.x;
+        const double dpy = gP1.y * Xn.y - gP0.y * Xp.y;
+        const double dpz = gP1.z * Xn.z - gP0.z * Xp.z;
+        const double fcv = fact * (dpx + dpy + dpz) * 0.8;
+        m += fcv;
+        mflux[f] = m;
+        fcorr[f] = fcv;
+    }
 }
 
 // cell correction (FlowSolver.inl:2416-2433), S_phi.U reset + gather of the
@@ -869,7 +877,7 @@ int calculate_pressure(Ctx *c)
         {
             {
                 KTimer kt(c, MC_KC_CORRECT, 56.0 * c->N + 88.0 * c->F);
-                hipLaunchKernelGGL(k_flux_correct, dim3(div_up(c->F, 256)), dim3(256), 0, c->stream, geo(c),
+                hipLaunchKernelGGL(k_flux_correct, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c),
                                    c->phi[PP], c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
             }
             const int which = simple ? 2 : (c->first_pressure ? 1 : 0);
