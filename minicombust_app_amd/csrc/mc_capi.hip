@@ -276,6 +276,10 @@ int mc_reset_fields(mc_ctx *x)
     MC_HIP(c, hipMemsetAsync(c->ff1, 0, (size_t)c->F * 8, c->stream));
     MC_HIP(c, hipMemsetAsync(c->fcorr, 0, (size_t)c->F * 8, c->stream));
     MC_HIP(c, hipMemsetAsync(c->A_vals, 0, (size_t)c->nnz * 8, c->stream));
+    if (c->fine_nd) {
+        MC_HIP(c, hipMemsetAsync(c->A_dia, 0, (size_t)c->fine_nd * c->N * 8, c->stream));
+        MC_HIP(c, hipMemsetAsync(c->A_dia2, 0, (size_t)c->fine_nd * c->N * 8, c->stream));
+    }
     MC_HIP(c, hipMemsetAsync(c->PA_vals, 0, (size_t)c->nnz * 8, c->stream));
     MC_HIP(c, hipMemsetAsync(c->Pb, 0, (size_t)c->N * 8, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
@@ -468,6 +472,8 @@ int mc_array_get(mc_ctx *x, int id, void *host, int64_t count)
     ArrDesc d;
     MC_TRY(describe(c, id, d));
     if (!host || count != d.count) { c->err = "array size mismatch (expected " + std::to_string(d.count) + ")"; return MC_ERR_ARG; }
+    // a matrix held in diagonal storage is handed out in CSR order
+    if (id == MC_ARR_A_VALS && fine_dia_of(c, c->A_vals)) MC_TRY(dia_to_csr(c, fine_dia_of(c, c->A_vals), c->A_vals));
     MC_HIP(c, hipStreamSynchronize(c->stream));
     if (d.face || d.kind == 8) MC_TRY(face_maps(c));
     const int *pos = d.face ? c->h_face_pos.data() : nullptr;   // reference id -> stored entry
@@ -545,6 +551,8 @@ int mc_array_set(mc_ctx *x, int id, const void *host, int64_t count)
         c->err = "array is read-only";
         return MC_ERR_ARG;
     }
+    if (id == MC_ARR_A_VALS && fine_dia_of(c, c->A_vals))
+        MC_TRY(csr_to_dia(c, c->A_vals, (double *)fine_dia_of(c, c->A_vals)));
     return MC_OK;
 }
 

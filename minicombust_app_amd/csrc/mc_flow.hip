@@ -30,6 +30,8 @@ struct Geo {
     int btype[6];
     int last_face[6];
     CellWalk w;
+    int dnd;                 // diagonals of the momentum / scalar matrix storage (0: CSR)
+    int doff[DIA_MAX];       // their column offsets, ascending
 };
 
 static Geo geo(Ctx *c)
@@ -44,6 +46,8 @@ static Geo geo(Ctx *c)
     g.vol = c->vol; g.rho = c->rho; g.is_wall = c->is_wall_cell;
     for (int s = 0; s < 6; s++) { g.btype[s] = c->btype[s]; g.last_face[s] = c->last_face[s]; }
     g.w = cell_walk(c);
+    g.dnd = c->fine_nd;
+    for (int d = 0; d < DIA_MAX; d++) g.doff[d] = d < c->fine_nd ? c->fine_off[d] : 0;
     return g;
 }
 
@@ -66,6 +70,33 @@ __device__ __forceinline__ void row_positions(int c, int n_int, const int *nb, i
         int p = (c < nb[k]);
         for (int t = 0; t < n_int; t++) p += nb[t] < nb[k];
         pos[k] = p;
+    }
+}
+
+// Diagonal of column offset `off` (present in every row's pattern).
+__device__ __forceinline__ int dslot(const Geo &g, int off)
+{
+    int d = 0;
+    while (d < g.dnd - 1 && g.doff[d] != off) d++;
+    return d;
+}
+
+// Row c of a momentum / scalar matrix: with a diagonal pattern each entry
+// goes to the diagonal of its column offset (the solvers read that storage);
+// otherwise to the sorted-column CSR positions.
+__device__ __forceinline__ void store_row(const Geo &g, int c, int nint, const int *nb, const double *v, double diag,
+                                          double *vals, double *dia)
+{
+    if (dia) {
+        const size_t N = (size_t)g.N;
+        dia[(size_t)dslot(g, 0) * N + c] = diag;
+        for (int k = 0; k < nint; k++) dia[(size_t)dslot(g, nb[k] - c) * N + c] = v[k];
+    } else {
+        int pos[6], dpos;
+        row_positions(c, nint, nb, pos, dpos);
+        const int rp = g.row_ptr[c];
+        vals[rp + dpos] = diag;
+        for (int k = 0; k < nint; k++) vals[rp + pos[k]] = v[k];
     }
 }
 
@@ -293,7 +324,7 @@ int precomp_AU(Ctx *c)
 struct MomArgs {
     double *U, *V, *W;
     const double *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz, *gPx, *gPy, *gPz, *TEM;
-    double *mflux, *ff0, *ff1, *AU, *SU, *SV, *SW, *vals;
+    double *mflux, *ff0, *ff1, *AU, *SU, *SV, *SW, *vals, *dia;
     double ev, inlet_ev, rdelta, urf, gas_tem, vel_x, vel_y, vel_z;
 };
 
@@ -442,11 +473,7 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     SW = SW + (1.0 - a.urf) * A * Wc;
     a.AU[c] = A;
     a.SU[c] = SU; a.SV[c] = SV; a.SW[c] = SW;
-    int pos[6], dpos;
-    row_positions(c, nint, own_nb, pos, dpos);
-    const int rp = g.row_ptr[c];
-    a.vals[rp + dpos] = A;
-    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+    store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
 
 static MomArgs mom_args(Ctx *c)
@@ -460,6 +487,7 @@ static MomArgs mom_args(Ctx *c)
     a.TEM = c->phi[TEM];
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1;
     a.AU = c->AU; a.SU = c->SU; a.SV = c->SV; a.SW = c->SW; a.vals = c->A_vals;
+    a.dia = (double *)fine_dia_of(c, c->A_vals);
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.5; a.gas_tem = c->k.gas_tem;
     a.vel_x = c->k.gas_vel_x; a.vel_y = c->k.gas_vel_y; a.vel_z = c->k.gas_vel_z;
     return a;
@@ -905,7 +933,7 @@ struct ScalArgs {
     const double *U, *V, *W, *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz;
     double *TE, *ED, *TP;
     const double *mflux;
-    double *ff0, *ff1, *AV, *SU, *vals;
+    double *ff0, *ff1, *AV, *SU, *vals, *dia;
     int type;
     double ev, inlet_ev, rdelta, urf, inlet_phi, cmu75;
 };
@@ -1080,11 +1108,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     S = S + (1.0 - a.urf) * A * phic;
     a.AV[c] = A;
     a.SU[c] = S;
-    int pos[6], dpos;
-    row_positions(c, nint, own_nb, pos, dpos);
-    const int rp = g.row_ptr[c];
-    a.vals[rp + dpos] = A;
-    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
+    store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
 
 // The passive scalars FUL, PRO, VARF and VARP share one Visac rule
@@ -1100,13 +1124,13 @@ struct ScalBatchArgs {
     double *S[SB];
     double inlet_phi[SB];
     const double *mflux;
-    double *ff0, *ff1, *AV, *vals;
+    double *ff0, *ff1, *AV, *vals, *dia;
     int type;
     double ev, inlet_ev, rdelta, urf;
     // the temperature equation in the same pass (its own Visac and matrix)
     double *tphi;
     const double *tgx, *tgy, *tgz;
-    double *tS, *tvals;
+    double *tS, *tvals, *tdia;
     double tinlet;
 };
 
@@ -1289,15 +1313,8 @@ __global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
         a.S[s][c] = S[s];
     }
     a.AV[c] = A;
-    int pos[6], dpos;
-    row_positions(c, nint, own_nb, pos, dpos);
-    const int rp = g.row_ptr[c];
-    a.vals[rp + dpos] = A;
-    for (int k = 0; k < nint; k++) a.vals[rp + pos[k]] = own_ff[k];
-    if constexpr (TEM) {
-        a.tvals[rp + dpos] = At;
-        for (int k = 0; k < nint; k++) a.tvals[rp + pos[k]] = own_fft[k];
-    }
+    store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
+    if constexpr (TEM) store_row(g, c, nint, own_nb, own_fft, At, a.tvals, a.tdia);
 }
 
 // FUL, PRO, VARF, VARP (FlowSolver.inl:3243-3260) as one 4-RHS solve; with
@@ -1317,10 +1334,11 @@ static int batch_solve(Ctx *c, bool tem)
         a.inlet_phi[s] = (types[s] == FUEL || types[s] == VARFU) ? c->k.gas_ful : 0.0;
     }
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.vals = c->A_vals;
+    a.dia = (double *)fine_dia_of(c, c->A_vals);
     a.type = FUEL;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
     a.tphi = c->phi[TEM]; a.tgx = c->gx[TEM]; a.tgy = c->gy[TEM]; a.tgz = c->gz[TEM];
-    a.tS = c->SU; a.tvals = c->A_vals2; a.tinlet = c->k.gas_tem;
+    a.tS = c->SU; a.tvals = c->A_vals2; a.tdia = (double *)fine_dia_of(c, c->A_vals2); a.tinlet = c->k.gas_tem;
     {
         const int nsc = SB + (tem ? 1 : 0);
         KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * nsc * c->N + 56.0 * c->F + 8.0 * c->nnz * (tem ? 2 : 1));
@@ -1368,6 +1386,7 @@ int scalar_assemble(Ctx *c, int type)
     a.gWx = c->gx[W]; a.gWy = c->gy[W]; a.gWz = c->gz[W];
     a.TE = c->phi[TE]; a.ED = c->phi[ED]; a.TP = c->phi[TP];
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.SU = c->SU; a.vals = c->A_vals;
+    a.dia = (double *)fine_dia_of(c, c->A_vals);
     a.type = type;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta;
     a.urf = (type == TERBTE || type == TERBED) ? 0.5 : 0.95;

@@ -130,7 +130,9 @@ struct Ctx {
     int fine_nd = 0;               // diagonal copy of the fine pattern (0: CSR kernels)
     int fine_off[DIA_MAX] = {};
     int *fine_map = nullptr;
-    double *A_dia = nullptr;       // diagonal copy of the momentum / scalar matrix
+    double *A_dia = nullptr;       // momentum / scalar matrix in diagonal storage; with a diagonal
+                                   // pattern the assembly kernels write it directly (A_vals unused)
+    double *A_dia2 = nullptr;      // the same for the temperature matrix (A_vals2)
     bool first_pressure = true;
     bool field_set_up = false;   // set_up_field has run (first pass of timestep 0)
     int64_t timestep_count = 0;
@@ -416,6 +418,11 @@ int precomp_AU(Ctx *c);
 int boundary_mass_flux(Ctx *c);
 
 // solvers (mc_linalg.hip)
+// diagonal storage holding the matrix assembled as `csr` (A_vals -> A_dia,
+// A_vals2 -> A_dia2); nullptr without a diagonal pattern
+const double *fine_dia_of(const Ctx *c, const double *csr);
+int dia_to_csr(Ctx *c, const double *dia, double *csr);   // array access in CSR order
+int csr_to_dia(Ctx *c, const double *csr, double *dia);
 int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x,
                    double rtol, int maxit, mc_solve_status *st);
 int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol = 0.0);

@@ -199,7 +199,31 @@ int dia_setup_fine(Ctx *c)
     MC_TRY(dalloc(c, &c->fine_map, c->nnz));
     MC_HIP(c, hipMemcpy(c->fine_map, map.data(), (size_t)c->nnz * 4, hipMemcpyHostToDevice));
     MC_TRY(dalloc(c, &c->A_dia, (size_t)c->fine_nd * c->N));
+    MC_TRY(dalloc(c, &c->A_dia2, (size_t)c->fine_nd * c->N));
+    // entries outside the pattern (missing neighbours) stay zero for good
     MC_HIP(c, hipMemset(c->A_dia, 0, (size_t)c->fine_nd * c->N * 8));
+    MC_HIP(c, hipMemset(c->A_dia2, 0, (size_t)c->fine_nd * c->N * 8));
+    return MC_OK;
+}
+
+const double *fine_dia_of(const Ctx *c, const double *csr)
+{
+    if (!c->fine_nd) return nullptr;
+    if (csr == c->A_vals) return c->A_dia;
+    if (csr == c->A_vals2) return c->A_dia2;
+    return nullptr;
+}
+
+__global__ void k_dia_to_csr(int nnz, const int *map, const double *dia, double *csr)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) csr[k] = dia[map[k]];
+}
+
+int dia_to_csr(Ctx *c, const double *dia, double *csr)
+{
+    hipLaunchKernelGGL(k_dia_to_csr, dim3(div_up(c->nnz, 256)), dim3(256), 0, c->stream, c->nnz, c->fine_map, dia, csr);
+    MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
 
@@ -207,14 +231,14 @@ static Mat<double> fine_mat(Ctx *c, const double *csr, const double *dia)
 {
     Mat<double> A;
     A.n = c->N; A.rp = c->row_ptr; A.col = c->col; A.csr = csr;
-    A.dia = c->fine_nd ? dia : nullptr;
-    A.nd = c->fine_nd;
+    A.dia = (c->fine_nd && dia) ? dia : nullptr;
+    A.nd = A.dia ? c->fine_nd : 0;
     for (int d = 0; d < DIA_MAX; d++) A.off[d] = d < c->fine_nd ? c->fine_off[d] : 0;
     return A;
 }
 
-// Momentum / scalar matrix: refresh the diagonal copy of A_vals.
-static int fine_dia_from(Ctx *c, const double *csr, double *dia)
+// CSR values (set through the array access) into the diagonal storage.
+int csr_to_dia(Ctx *c, const double *csr, double *dia)
 {
     if (!c->fine_nd) return MC_OK;
     hipLaunchKernelGGL(k_csr_to_dia, dim3(div_up(c->nnz, 256)), dim3(256), 0, c->stream, c->nnz, c->fine_map, csr, dia,
@@ -235,8 +259,9 @@ __global__ void __launch_bounds__(BLOCK) k_spmv(Mat<double> A, const double *x, 
 int spmv(Ctx *c, const double *vals, const double *x, double *y)
 {
     const Grid g = grid_for(c->N);
-    if (c->fine_nd) MC_TRY(fine_dia_from(c, vals, c->A_dia));
-    const Mat<double> A = fine_mat(c, vals, c->A_dia);
+    // the momentum / scalar matrices through their diagonal storage, any other
+    // (the pressure matrix) through CSR
+    const Mat<double> A = fine_mat(c, vals, fine_dia_of(c, vals));
     MC_DISPATCH_ND(A.nd, (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, x, y, g.chunk);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -506,8 +531,9 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
         w[0].p[k] = pa; w[0].v[k] = va; w[0].po[k] = pb; w[0].vo[k] = vb;
         w[1].p[k] = pb; w[1].v[k] = vb; w[1].po[k] = pa; w[1].vo[k] = va;
     }
-    MC_TRY(fine_dia_from(c, vals, c->A_dia));
-    const Mat<double> A = fine_mat(c, vals, c->A_dia);
+    const double *dia = fine_dia_of(c, vals);
+    if (c->fine_nd && !dia) { c->err = "bicgstab: matrix without its diagonal storage"; return MC_ERR_STATE; }
+    const Mat<double> A = fine_mat(c, vals, dia);
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     const Grid g = grid_for(n);
