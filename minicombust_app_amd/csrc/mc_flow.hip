@@ -12,6 +12,7 @@
 // written in FlowSolver.inl.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -540,37 +541,42 @@ __global__ void k_boundary_mflux(Geo g, const int *bfaces, int Fb, double *mflux
 
 // FlowIn / FlowOut / areaout and the FlowOut < 1e-10 rescale
 // (FlowSolver.inl:2010-2079), one workgroup.
-__global__ void __launch_bounds__(1024) k_outlet_balance(Geo g, const int *bfaces, int Fb, double *mflux, double *U,
-                                                         double *V, double *W, double vx, double vy, double vz)
+// The outlet balance (FlowSolver.inl:1957-2006) as a deterministic
+// two-kernel reduction over the boundary faces (inflow, outflow, outlet
+// area), then the rescaling pass when the outflow vanishes.
+__global__ void __launch_bounds__(256) k_outlet_sums(Geo g, const int *bfaces, int Fb, const double *mflux,
+                                                     double *partials, int chunk)
 {
-    __shared__ double s_in[1024], s_out[1024], s_area[1024];
-    double fin = 0.0, fout = 0.0, aout = 0.0;
-    for (int i = threadIdx.x; i < Fb; i += blockDim.x) {
+    const int b = blockIdx.x;
+    const int i0 = b * chunk, i1 = min(Fb, i0 + chunk);
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
         const int f = bfaces[i];
         const int bt = g.btype[g.fc1[f] - g.N];
-        if (bt == INLET) fin += mflux[f];
-        else if (bt == OUTLET) { fout += mflux[f]; aout += g.farea[f]; }
+        if (bt == INLET) v[0] += mflux[f];
+        else if (bt == OUTLET) { v[1] += mflux[f]; v[2] += g.farea[f]; }
     }
-    s_in[threadIdx.x] = fin; s_out[threadIdx.x] = fout; s_area[threadIdx.x] = aout;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            s_in[threadIdx.x] += s_in[threadIdx.x + s];
-            s_out[threadIdx.x] += s_out[threadIdx.x + s];
-            s_area[threadIdx.x] += s_area[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    const double FlowIn = s_in[0], FlowOut = s_out[0], areaout = s_area[0];
+    block_partials<3>(v, partials, b, gridDim.x);
+}
+
+struct EpiOutlet {
+    double *out;   // FlowIn, FlowOut, areaout
+    __device__ void operator()(const double (&v)[3]) const { out[0] = v[0]; out[1] = v[1]; out[2] = v[2]; }
+};
+
+__global__ void k_outlet_apply(Geo g, const int *bfaces, int Fb, const double *sums, double *mflux, double *U,
+                               double *V, double *W, double vx, double vy, double vz)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Fb) return;
+    const double FlowIn = sums[0], FlowOut = sums[1], areaout = sums[2];
     if (!(FlowOut < 0.0000000001)) return;
     const double ratearea = -FlowIn / areaout;
-    for (int i = threadIdx.x; i < Fb; i += blockDim.x) {
-        const int f = bfaces[i];
-        const int side = g.fc1[f] - g.N;
-        if (g.btype[side] == OUTLET) {
-            mflux[f] = ratearea * g.farea[f];
-            if (f == g.last_face[side]) { U[g.N + side] = vx; V[g.N + side] = vy; W[g.N + side] = vz; }
-        }
+    const int f = bfaces[i];
+    const int side = g.fc1[f] - g.N;
+    if (g.btype[side] == OUTLET) {
+        mflux[f] = ratearea * g.farea[f];
+        if (f == g.last_face[side]) { U[g.N + side] = vx; V[g.N + side] = vy; W[g.N + side] = vz; }
     }
 }
 
@@ -579,8 +585,17 @@ int boundary_mass_flux(Ctx *c)
     if (c->Fb == 0) return MC_OK;
     hipLaunchKernelGGL(k_boundary_mflux, dim3(div_up(c->Fb, 256)), dim3(256), 0, c->stream, geo(c), c->bfaces, c->Fb,
                        c->mflux, c->phi[TE], c->phi[ED], c->phi[TEM], c->k.gas_vel_x, c->k.gas_vel_y, c->k.gas_vel_z);
-    hipLaunchKernelGGL(k_outlet_balance, dim3(1), dim3(1024), 0, c->stream, geo(c), c->bfaces, c->Fb, c->mflux,
-                       c->phi[U], c->phi[V], c->phi[W], c->k.gas_vel_x, c->k.gas_vel_y, c->k.gas_vel_z);
+    if (c->Fb > 0) {
+        const int Fb = c->Fb;
+        const int blocks = std::max(1, std::min(256, div_up(Fb, 2048)));
+        const int chunk = div_up(Fb, blocks);
+        double *sums = c->dscal + 760;
+        hipLaunchKernelGGL(k_outlet_sums, dim3(blocks), dim3(256), 0, c->stream, geo(c), c->bfaces, Fb, c->mflux, c->red,
+                           chunk);
+        launch_finish<3>(c->stream, c->red, blocks, EpiOutlet{sums});
+        hipLaunchKernelGGL(k_outlet_apply, dim3(div_up(Fb, 256)), dim3(256), 0, c->stream, geo(c), c->bfaces, Fb, sums,
+                           c->mflux, c->phi[U], c->phi[V], c->phi[W], c->k.gas_vel_x, c->k.gas_vel_y, c->k.gas_vel_z);
+    }
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }

@@ -49,6 +49,8 @@ struct AmgLevel {
     int *row_ptr = nullptr, *col = nullptr;   // CSR pattern of this level
     int nd = 0;                               // diagonals of the diagonal copy (0: none)
     int off[DIA_MAX] = {};
+    int sym = 0;                              // symmetric pattern: the FP64 copy stores only the
+                                              // diagonals with offset >= 0 (nd / 2 + 1 arrays)
     int *dia_map = nullptr;                   // CSR index -> d*n + i
     double *dia = nullptr;
     float *dia32 = nullptr;

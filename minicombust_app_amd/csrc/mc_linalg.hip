@@ -79,7 +79,28 @@ struct Mat {
     const T *dia;          // diagonal storage, nullptr -> CSR kernels
     int nd;
     int off[DIA_MAX];
+    int sym;               // symmetric: dia holds diagonals nd/2 .. nd-1 (offsets >= 0) only
 };
+
+// Entry (i, i + off[d]) of a diagonal-storage matrix.  Symmetric storage
+// (SYM) keeps the main diagonal and the upper ones; a lower entry is the
+// stored upper entry of the transposed position (row i + off[d], diagonal
+// nd-1-d), zero where that row does not exist.  Reading it there touches the
+// words a later row reads as its upper entry, so each value crosses HBM once.
+// SYM is a compile-time choice (negative ND in the kernels' dispatch): a
+// run-time branch here slows every diagonal-storage kernel.
+template <bool SYM, typename T>
+__device__ __forceinline__ double dia_at(const Mat<T> &A, int d, int i)
+{
+    if constexpr (!SYM) {
+        return (double)A.dia[(size_t)d * A.n + i];
+    } else {
+        const int h = A.nd >> 1;
+        if (d >= h) return (double)A.dia[(size_t)(d - h) * A.n + i];
+        const int j = i + A.off[d];
+        return j >= 0 ? (double)A.dia[(size_t)(A.nd - 1 - d - h) * A.n + j] : 0.0;
+    }
+}
 
 // bytes of one pass over a matrix: DIA values, or CSR values + columns + rows
 template <typename T>
@@ -99,20 +120,22 @@ __device__ __forceinline__ int clampi(int j, int n) { return j < 0 ? 0 : (j >= n
 template <int ND, int NR, typename T, typename XF, typename F>
 __device__ __forceinline__ void for_rows(const Mat<T> &A, int r0, int r1, XF X, F f)
 {
-    if constexpr (ND > 0) {
+    constexpr bool SYM = ND < 0;
+    constexpr int NA = SYM ? -ND : ND;
+    if constexpr (NA > 0) {
         for (int i = r0 + (int)threadIdx.x; i < r1; i += BLOCK) {
             double acc[NR];
 #pragma unroll
             for (int k = 0; k < NR; k++) acc[k] = 0.0;
-            double a[ND];
-            int j[ND];
+            double a[NA];
+            int j[NA];
 #pragma unroll
-            for (int d = 0; d < ND; d++) {
-                a[d] = (double)A.dia[(size_t)d * A.n + i];
+            for (int d = 0; d < NA; d++) {
+                a[d] = dia_at<SYM>(A, d, i);
                 j[d] = clampi(i + A.off[d], A.n);
             }
 #pragma unroll
-            for (int d = 0; d < ND; d++) {
+            for (int d = 0; d < NA; d++) {
 #pragma unroll
                 for (int k = 0; k < NR; k++) acc[k] += a[d] * X(k, j[d]);
             }
@@ -149,11 +172,22 @@ __device__ __forceinline__ void for_rows(const Mat<T> &A, int r0, int r1, XF X, 
 
 // Launch a kernel template over the supported diagonal counts; KERNEL names
 // the count ND_ (7, 27, or 0 for the CSR path).
-#define MC_DISPATCH_ND(nd, KERNEL, ...)                                                   \
-    do {                                                                                  \
-        if ((nd) == 7) { constexpr int ND_ = 7; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }   \
-        else if ((nd) == 27) { constexpr int ND_ = 27; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); } \
-        else { constexpr int ND_ = 0; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }          \
+// Kernels take the storage as a compile-time ND: the number of diagonals,
+// negative for symmetric storage, 0 for CSR.
+template <typename T>
+inline int nd_code(const Mat<T> &A)
+{
+    return A.dia ? (A.sym ? -A.nd : A.nd) : 0;
+}
+
+#define MC_DISPATCH_ND(code, KERNEL, ...)                                                         \
+    do {                                                                                          \
+        const int code_ = (code);                                                                 \
+        if (code_ == 7) { constexpr int ND_ = 7; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }        \
+        else if (code_ == -7) { constexpr int ND_ = -7; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); } \
+        else if (code_ == 27) { constexpr int ND_ = 27; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); } \
+        else if (code_ == -27) { constexpr int ND_ = -27; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); } \
+        else { constexpr int ND_ = 0; hipLaunchKernelGGL(KERNEL, __VA_ARGS__); }                  \
     } while (0)
 
 // ------------------------------------------ CSR -> diagonal storage --
@@ -233,6 +267,7 @@ static Mat<double> fine_mat(Ctx *c, const double *csr, const double *dia)
     A.n = c->N; A.rp = c->row_ptr; A.col = c->col; A.csr = csr;
     A.dia = (c->fine_nd && dia) ? dia : nullptr;
     A.nd = A.dia ? c->fine_nd : 0;
+    A.sym = 0;
     for (int d = 0; d < DIA_MAX; d++) A.off[d] = d < c->fine_nd ? c->fine_off[d] : 0;
     return A;
 }
@@ -262,7 +297,7 @@ int spmv(Ctx *c, const double *vals, const double *x, double *y)
     // the momentum / scalar matrices through their diagonal storage, any other
     // (the pressure matrix) through CSR
     const Mat<double> A = fine_mat(c, vals, fine_dia_of(c, vals));
-    MC_DISPATCH_ND(A.nd, (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, x, y, g.chunk);
+    MC_DISPATCH_ND(nd_code(A), (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, x, y, g.chunk);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
@@ -549,11 +584,11 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
             const BiVecs &W = w[it & 1];
             {
                 KTimer kt(c, MC_KC_BICG, mbytes);
-                MC_DISPATCH_ND(A.nd, (k_bi_pv<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
+                MC_DISPATCH_ND(nd_code(A), (k_bi_pv<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
                                g.chunk);
             }
             launch_finish<NR>(c->stream, c->red, g.blocks, EpiBiAlpha<NR>{ks});
-            MC_DISPATCH_ND(A.nd, (k_bi_st<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
+            MC_DISPATCH_ND(nd_code(A), (k_bi_st<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, W, ks, c->red,
                            g.chunk);
             launch_finish<3 * NR>(c->stream, c->red, g.blocks, EpiBiOmega<NR>{ks});
             hipLaunchKernelGGL(k_bi_xr<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, W, ks, c->red, g.chunk);
@@ -779,17 +814,20 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
 // A level with diagonal storage: one pass over its CSR rows fills the FP64 and
 // FP32 diagonal copies and the inverse diagonal.
 __global__ void k_level_dia(int n, const int *rp, const int *col, const int *map, const double *v, double *dia,
-                            float *dia32, double *dinv, float *dinv32)
+                            float *dia32, double *dinv, float *dinv32, int sym_skip)
 {
+    // sym_skip > 0: the FP64 copy is symmetric storage without the sym_skip
+    // lower diagonals; the FP32 copy keeps all of them
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d = 0.0;
+    const size_t shift = (size_t)sym_skip * n;
     for (int k = rp[i]; k < rp[i + 1]; k++) {
         const double x = v[k];
         const int m = map[k];
-        dia[m] = x;
-        dia32[m] = (float)x;
         if (col[k] == i) d = x;
+        dia32[m] = (float)x;
+        if (!sym_skip || col[k] >= i) dia[(size_t)m - shift] = x;
     }
     const double di = (d != 0.0) ? 1.0 / d : 0.0;
     dinv[i] = di;
@@ -916,11 +954,16 @@ int amg_setup(Ctx *c, const double *vals)
             if (cur.n > coarse_max && dia_analyze(cur.n, cur.rp.data(), cur.col.data(), offs, map)) {
                 L.nd = (int)offs.size();
                 for (int d = 0; d < L.nd; d++) L.off[d] = offs[d];
+                // pressure levels are symmetric: the fine matrix entry for a face is one value
+                // written to both rows, and Galerkin levels of it are taken as their upper part
+                L.sym = (L.nd % 2 == 1);
+                for (int d = 0; d < L.nd && L.sym; d++) L.sym = (L.off[d] == -L.off[L.nd - 1 - d]);
+                const int ns = L.sym ? L.nd / 2 + 1 : L.nd;
                 MC_TRY(dalloc(c, &L.dia_map, L.nnz));
                 MC_HIP(c, hipMemcpy(L.dia_map, map.data(), (size_t)L.nnz * 4, hipMemcpyHostToDevice));
-                MC_TRY(dalloc(c, &L.dia, (size_t)L.nd * L.n));
+                MC_TRY(dalloc(c, &L.dia, (size_t)ns * L.n));
                 MC_TRY(dalloc(c, &L.dia32, (size_t)L.nd * L.n));
-                MC_HIP(c, hipMemset(L.dia, 0, (size_t)L.nd * L.n * 8));
+                MC_HIP(c, hipMemset(L.dia, 0, (size_t)ns * L.n * 8));
                 MC_HIP(c, hipMemset(L.dia32, 0, (size_t)L.nd * L.n * 4));
             }
             if (!L.nd) MC_TRY(dalloc(c, &L.vals32, L.nnz));   // FP32 CSR only where no diagonal copy
@@ -989,7 +1032,7 @@ int amg_update_values(Ctx *c, const double *vals)
             MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
         if (L.nd)
             hipLaunchKernelGGL(k_level_dia, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
-                               L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32);
+                               L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32, L.sym ? L.nd / 2 : 0);
         else
             hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
                                L.vals, L.dinv, L.dinv32, L.vals32, L.nnz);
@@ -1017,6 +1060,7 @@ static Mat<T> level_mat(AmgLevel &L)
     A.n = L.n; A.rp = L.row_ptr; A.col = L.col;
     if constexpr (sizeof(T) == 8) { A.csr = (const T *)L.vals; A.dia = L.nd ? (const T *)L.dia : nullptr; }
     else { A.csr = (const T *)L.vals32; A.dia = L.nd ? (const T *)L.dia32 : nullptr; }
+    A.sym = (sizeof(T) == 8) ? L.sym : 0;   // FP32 copies keep every diagonal (faster there, measured)
     A.nd = L.nd;
     for (int d = 0; d < DIA_MAX; d++) A.off[d] = d < L.nd ? L.off[d] : 0;
     return A;
@@ -1109,13 +1153,15 @@ __global__ void __launch_bounds__(BLOCK) k_resid_restrict(Mat<T> A, int nc, cons
     double s = 0.0;
     if (I < nc) {
         const int t0 = aptr[I], t1 = aptr[I + 1];
-        if constexpr (ND > 0) {
+        constexpr bool SYM = ND < 0;
+        constexpr int NA = SYM ? -ND : ND;
+        if constexpr (NA > 0) {
             if (t0 + lane8 < t1) {
                 const int i = amem[t0 + lane8];
                 double ax = 0.0;
 #pragma unroll
-                for (int d = 0; d < ND; d++)
-                    ax += (double)A.dia[(size_t)d * A.n + i] * (double)x[clampi(i + A.off[d], A.n)];
+                for (int d = 0; d < NA; d++)
+                    ax += dia_at<SYM>(A, d, i) * (double)x[clampi(i + A.off[d], A.n)];
                 s = (double)b[i] - ax;
             }
         } else {
@@ -1241,11 +1287,11 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     const Mat<T> A = level_mat<T>(L);
     T *dinv = lv_dinv<T>(L), *tmp = lv_t<T>(L);
     T *cur = x, *oth = tmp;
-    MC_DISPATCH_ND(A.nd, (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, (T)wts[0],
+    MC_DISPATCH_ND(nd_code(A), (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, (T)wts[0],
                    (T)wts[nu >= 2 ? 1 : 0],
                    g.chunk, nu >= 2 ? 1 : 0, stop);
     for (int s = 2; s < nu; s++) {
-        MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+        MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
                        (const int *)nullptr, (const T *)nullptr, (T)0, oth, (T)wts[s], g.chunk, (const T *)nullptr,
                        (double *)nullptr, stop);
         std::swap(cur, oth);
@@ -1254,12 +1300,12 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     T *bc = lv_b<T>(Lc), *xc = lv_x<T>(Lc);
     if (A.nd) {
         T *r = (cur == x) ? tmp : x;   // free until post-smoothing
-        MC_DISPATCH_ND(A.nd, (k_resid<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, (const T *)cur, r,
+        MC_DISPATCH_ND(nd_code(A), (k_resid<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, (const T *)cur, r,
                        g.chunk, stop);
         hipLaunchKernelGGL(k_restrict<T>, dim3(div_up(L.nc, BLOCK)), dim3(BLOCK), 0, c->stream, L.nc, L.agg_ptr,
                            L.agg_members, (const T *)r, bc, stop);
     } else {
-        MC_DISPATCH_ND(A.nd, (k_resid_restrict<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, A,
+        MC_DISPATCH_ND(nd_code(A), (k_resid_restrict<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0, c->stream, A,
                        L.nc, L.agg_ptr, L.agg_members, b, cur, bc, stop);
     }
     MC_TRY(vcycle<T>(c, l + 1, bc, xc, nullptr, stop));
@@ -1278,7 +1324,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
                                        : 0.0;
         {
             KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
-            MC_DISPATCH_ND(A.nd, (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+            MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
                            first ? L.agg : (const int *)nullptr, first ? xc : (const T *)nullptr, first ? cs : (T)0,
                            dst, (T)wts[nu - 1 - s], g.chunk, dotv, c->red, stop);
         }
@@ -1429,7 +1475,7 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
     }
     if constexpr (sizeof(T) == 8) {
         if (x0 && mean) {
-            MC_DISPATCH_ND(A.nd, (k_pcg_init_x0<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A,
+            MC_DISPATCH_ND(nd_code(A), (k_pcg_init_x0<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A,
                            (const double *)b, mean, w, c->red, g.chunk);
         } else {
             hipLaunchKernelGGL(k_pcg_init<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, w, c->red,
@@ -1453,7 +1499,7 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
             }
             {
                 KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32, vbytes);
-                MC_DISPATCH_ND(A.nd, (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb,
+                MC_DISPATCH_ND(nd_code(A), (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb,
                                w.q, ks, c->red, g.chunk);
             }
             launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgAlpha{ks});
@@ -1575,7 +1621,7 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     for (outer = 0; outer < 50; outer++) {
         {
             KTimer kt(c, MC_KC_IR_RESID, rbytes);
-            MC_DISPATCH_ND(A.nd, (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
+            MC_DISPATCH_ND(nd_code(A), (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
                            g.chunk);
         }
         launch_finish<1>(c->stream, c->red, g.blocks, EpiSqrt{rn});
@@ -1676,7 +1722,7 @@ static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double 
     MC_TRY(project_mean_zero(c, gv));
     const Grid g = grid_for(n);
     const Mat<double> A = level_mat<double>(c->amg[0]);
-    MC_DISPATCH_ND(A.nd, (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, (const double *)gv, Ag, g.chunk);
+    MC_DISPATCH_ND(nd_code(A), (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, (const double *)gv, Ag, g.chunk);
     hipLaunchKernelGGL(k_guess_dots, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, (const double *)gv,
                        (const double *)Ag, c->red, g.chunk);
     launch_finish<3>(c->stream, c->red, g.blocks, EpiGuess1{sc, 1.0 / n});
