@@ -34,9 +34,9 @@ KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "pcg_spmv64", "
             "flux_correct", "mpnn_fwd", "jac64", "jac32", "bicg_spmv", "scalar_batch", "ir_resid", "mpnn_bwd"]
 # kernel symbol (as a profiler lists it) of each single-kernel class
 KC_KERNELS = {"grad11": "mc::k_grad<11, false>", "grad1": "mc::k_grad<1, false>", "mom_asm": "mc::k_momentum",
-              "p_asm": "mc::k_mass_pressure", "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<7, double>",
+              "p_asm": "mc::k_mass_pressure", "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<-7, double>",
               "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
-              "scalar_batch": "mc::k_scalar_batch", "ir_resid": "mc::k_ir_resid<7>"}
+              "scalar_batch": "mc::k_scalar_batch<true>", "ir_resid": "mc::k_ir_resid<-7>"}
 KC_COUNT = len(KC_NAMES)
 MEM_NAMES = ["mesh", "fields", "matrix", "amg", "work", "mpnn", "dataset"]
 
