@@ -1002,7 +1002,10 @@ int mc_mpnn_forward_pair(mc_ctx *x, int64_t pair, float *y_host)
     if (!m || !m->ready) { c->err = "mpnn: no weights loaded"; return MC_ERR_STATE; }
     const float *f, *e, *t;
     MC_TRY(pair_ptrs(c, pair, &f, &e, &t));
-    forward(c, f, e, false);
+    {
+        KTimer kt(c, MC_KC_MPNN, forward_flops(c->N));
+        forward(c, f, e, false);
+    }
     MC_LAUNCH_CHECK(c);
     MC_HIP(c, hipMemcpyAsync(y_host, m->y, (size_t)c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
