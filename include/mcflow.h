@@ -210,6 +210,10 @@ int  mc_get_phi_gradient(mc_ctx *ctx, int field, int pressure);
    reference has no counterpart; its call is commented out,
    gpu/FlowSolver.inl:1333).                                                   */
 int  mc_limit_phi_gradients(mc_ctx *ctx);
+/* The non-negativity repair Scalar_solve applies to TE and ED after their
+   solve (FlowSolver.inl:2956-2985): negative cells, in ascending order, take
+   the mean of their interior neighbours as they stand (at least 1e-12).     */
+int  mc_repair_nonneg(mc_ctx *ctx, int field);
 /* C_kernel_precomp_AU (gpu_kernels.cuh:38) + calculate_mass_flux:
    set_up_field (FlowSolver.inl:1065-1113).                                   */
 int  mc_set_up_field(mc_ctx *ctx);

@@ -312,6 +312,13 @@ int mc_simple_iteration(mc_ctx *x, int inner)
 int mc_get_phi_gradients(mc_ctx *x) { CTX_OR_FAIL return grad_multi(c, false); }
 int mc_limit_phi_gradients(mc_ctx *x) { CTX_OR_FAIL return limit_gradients(c); }
 
+int mc_repair_nonneg(mc_ctx *x, int field)
+{
+    CTX_OR_FAIL
+    if (field < 0 || field >= MC_NFIELDS) return MC_ERR_ARG;
+    return clip_nonneg(c, c->phi[field], field == MC_ED ? 1 : 0);
+}
+
 int mc_get_phi_gradient(mc_ctx *x, int field, int pressure)
 {
     CTX_OR_FAIL

@@ -1433,13 +1433,22 @@ __global__ void __launch_bounds__(256) k_flag_neg(int n, const double *v, uint8_
     if (i < n) flag[i] = v[i] < 0.0 ? 1 : 0;
 }
 
+// One relaxation pass.  After the first (all != 0), a cell is evaluated only
+// when a lower neighbour changed in the previous pass (dirty_in), the only
+// way its inputs can change; a cell that changes marks its higher neighbours
+// in dirty_out for the next pass.
 __global__ void __launch_bounds__(256) k_clip_list(Geo g, const int *list, int count, const double *orig, double *phi,
-                                                   const int *prev, int *changed)
+                                                   const int *prev, int *changed, uint8_t *dirty_in,
+                                                   uint8_t *dirty_out, int all)
 {
     if (prev && *prev == 0) return;   // the previous pass changed nothing: fixed point reached
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const int c = list[i];
+    if (!all) {
+        if (!dirty_in[c]) return;
+        dirty_in[c] = 0;
+    }
     int cnt = 0;
     double phisum = 0.0;
     for (int j = 0; j < 6; j++) {
@@ -1454,6 +1463,10 @@ __global__ void __launch_bounds__(256) k_clip_list(Geo g, const int *list, int c
     if (out != cur && !(out != out && cur != cur)) {
         phi[c] = out;
         atomicAdd(changed, 1);
+        for (int j = 0; j < 6; j++) {
+            const int nb = g.nbr[6 * c + j];
+            if (nb > c) dirty_out[nb] = 1;
+        }
     }
 }
 
@@ -1479,7 +1492,7 @@ int flow_workspace(Ctx *c)
 // field needed last time.
 constexpr int CLIP_BATCH_MAX = 64;
 
-static int clip_nonneg(Ctx *c, double *phi, int slot)
+int clip_nonneg(Ctx *c, double *phi, int slot)
 {
     const int N = c->N;
     int *dcount = (int *)(c->dscal + 700);
@@ -1499,13 +1512,16 @@ static int clip_nonneg(Ctx *c, double *phi, int slot)
     if (K == 0) return MC_OK;
     double *orig = c->wk[14];
     MC_HIP(c, hipMemcpyAsync(orig, phi, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
+    uint8_t *dirty[2] = {flag + N, flag + 2 * (size_t)N};   // inside the same 8N-byte work vector
+    MC_HIP(c, hipMemsetAsync(dirty[0], 0, 2 * (size_t)N, c->stream));
     int B = std::min(std::max(c->clip_batch[slot], 4), CLIP_BATCH_MAX);
-    int used = 0;
+    int used = 0, pass = 0;
     for (int batch = 0; batch < 100000; batch++) {
         MC_HIP(c, hipMemsetAsync(changed, 0, B * sizeof(int), c->stream));
-        for (int p = 0; p < B; p++)
+        for (int p = 0; p < B; p++, pass++)
             hipLaunchKernelGGL(k_clip_list, dim3(div_up(K, 256)), dim3(256), 0, c->stream, geo(c), c->neg_list, K,
-                               orig, phi, p ? changed + p - 1 : (const int *)nullptr, changed + p);
+                               orig, phi, p ? changed + p - 1 : (const int *)nullptr, changed + p, dirty[pass & 1],
+                               dirty[(pass + 1) & 1], pass == 0 ? 1 : 0);
         MC_LAUNCH_CHECK(c);
         MC_HIP(c, hipMemcpyAsync(hchanged, changed, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         MC_HIP(c, hipStreamSynchronize(c->stream));

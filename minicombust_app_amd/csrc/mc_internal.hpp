@@ -403,6 +403,7 @@ int init_fields(Ctx *c);   // constructor fields and boundary slots   // geometr
 // stages (mc_flow.hip)
 int grad_multi(Ctx *c, bool limit);                      // 11 fields, optionally limited
 int limit_gradients(Ctx *c);                             // limiter alone, 11 fields
+int clip_nonneg(Ctx *c, double *phi, int slot);          // TE / ED repair (slot 0 / 1: pass history)
 int grad_single(Ctx *c, int field, bool pressure);
 int set_up_field(Ctx *c);
 int momentum_assemble(Ctx *c);

@@ -107,6 +107,7 @@ def lib():
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
         L.mc_write_vtk.argtypes = [P_, C.c_char_p, C.c_int64, C.c_int]
+        L.mc_repair_nonneg.argtypes = [P_, C.c_int]
         L.mc_particle_terms_reset.argtypes = [P_]
         L.mc_particle_terms_add.argtypes = [P_, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int64]
         L.mc_interpolate_to_nodes.argtypes = [P_, C.POINTER(C.c_uint64), C.c_int64, C.POINTER(C.c_uint64), C.c_int64,
@@ -246,6 +247,9 @@ class Context:
                                                    out.ctypes.data_as(C.POINTER(C.c_double))),
                     "mc_interpolate_to_nodes")
         return out
+
+    def repair_nonneg(self, field):
+        self._check(self.L.mc_repair_nonneg(self.h, int(field)), "mc_repair_nonneg")
 
     def limit_phi_gradients(self):
         self._check(self.L.mc_limit_phi_gradients(self.h), "mc_limit_phi_gradients")

@@ -1587,28 +1587,34 @@ void mco_scalar_assemble(mco_state *s, int type, int field)
     setup_sparse_matrix(s, underrelax, s->A_V, phi, s->S_U);
 }
 
-void mco_scalar_solve(mco_state *s, int type, int field)
+/* FlowSolver.inl:2956-2985: non-negativity repair, in place, ascending cell
+   order (a negative cell takes the mean of its interior neighbours as they
+   stand, at least 1e-12) */
+void mco_repair_nonneg(mco_state *s, int field)
 {
     const mco_mesh *M = &s->mesh;
     const int64_t N = s->N;
     double *phi = s->phi[field];
-    mco_scalar_assemble(s, type, field);
-    s->stats.scalar_iters += solve_sparse_matrix(s, phi);
-    if (type == MCO_TERBTE || type == MCO_TERBED) {
-        /* FlowSolver.inl:2956-2985: in-place, ascending cell order */
-        for (int64_t i = 0; i < N; i++) {
-            int count = 0;
-            double phisum = 0.0;
-            if (phi[i] < 0.0) {
-                for (int j = 0; j < 6; j++) {
-                    const uint64_t nb = M->cell_neighbours[i * 6 + j];
-                    if (nb < (uint64_t)N) { count++; phisum += phi[nb]; }
-                }
-                phisum = (phisum / count);
-                phi[i] = dmax(phisum, 0.000000000001);
+    for (int64_t i = 0; i < N; i++) {
+        int count = 0;
+        double phisum = 0.0;
+        if (phi[i] < 0.0) {
+            for (int j = 0; j < 6; j++) {
+                const uint64_t nb = M->cell_neighbours[i * 6 + j];
+                if (nb < (uint64_t)N) { count++; phisum += phi[nb]; }
             }
+            phisum = (phisum / count);
+            phi[i] = dmax(phisum, 0.000000000001);
         }
     }
+}
+
+void mco_scalar_solve(mco_state *s, int type, int field)
+{
+    double *phi = s->phi[field];
+    mco_scalar_assemble(s, type, field);
+    s->stats.scalar_iters += solve_sparse_matrix(s, phi);
+    if (type == MCO_TERBTE || type == MCO_TERBED) mco_repair_nonneg(s, field);
 }
 
 /* ======================================================================

@@ -131,6 +131,7 @@ void mco_destroy(mco_state *s);
 /* ---- stages (names follow FlowSolver.inl) ---- */
 void mco_get_phi_gradients(mco_state *s);
 void mco_limit_phi_gradients(mco_state *s);
+void mco_repair_nonneg(mco_state *s, int field);
 void mco_particle_terms_add(mco_state *s, int64_t n, const uint64_t *cells, const double *terms, double *pt);
 void mco_interpolate_to_nodes(mco_state *s, int64_t ncells, const uint64_t *cells, int64_t nnodes,
                               const uint64_t *nodes, double *out);

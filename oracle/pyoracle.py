@@ -106,6 +106,7 @@ def lib():
                      "mco_calculate_pressure", "mco_timestep"]:
             getattr(L, name).argtypes = [SP]
         L.mco_get_phi_gradient.argtypes = [SP, C.c_int, C.c_int]
+        L.mco_repair_nonneg.argtypes = [SP, C.c_int]
         _UP = C.POINTER(C.c_uint64)
         L.mco_particle_terms_add.argtypes = [SP, C.c_int64, _UP, _DP, _DP]
         L.mco_interpolate_to_nodes.argtypes = [SP, C.c_int64, _UP, C.c_int64, _UP, _DP]

@@ -366,3 +366,31 @@ def test_particle_coupling_bits():
     assert_bits(g.interpolate_to_nodes(shuffled, nodes), out, "node values")
     o.close()
     g.close()
+
+
+@pytest.mark.parametrize("m", [4, 12])
+def test_repair_nonneg_bits(m):
+    """The TE / ED non-negativity repair (FlowSolver.inl:2956-2985) bit for bit:
+    the reference's in-order sweep against the library's relaxation passes, on
+    smooth fields with large negative regions (long chains of lower
+    neighbours) plus noise; twice per field, so the second call starts from
+    the first one's pass history."""
+    print("repair m=%d" % m, flush=True)
+    o, g = make_pair(m)
+    M = o.mesh_arrays()
+    cc = M["cell_centers"]
+    rng = np.random.default_rng(30 + m)
+    for field in [po.TE, po.ED]:
+        for rep in range(2):
+            k = rng.uniform(5.0, 20.0, size=(3, 3))
+            ph = o.phi(field)
+            ph[:o.N] = (np.cos(cc @ k[0]) * np.cos(cc @ k[1] + 1.0) + 0.3 * np.sin(cc @ k[2])
+                        + 0.2 * rng.standard_normal(o.N))
+            assert np.mean(ph[:o.N] < 0) > 0.2
+            push_state(o, g)
+            o.repair_nonneg(field)
+            g.repair_nonneg(field)
+            assert_bits(g.phi(field), o.phi(field), "repaired %s (call %d)" % (po.FIELD_NAMES[field], rep))
+            assert np.all(o.phi(field)[:o.N] > 0)
+    o.close()
+    g.close()
