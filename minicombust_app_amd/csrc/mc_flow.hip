@@ -1151,8 +1151,10 @@ struct ScalBatchArgs {
 
 // TEM: with its own diffusion coefficient, every expression as in k_scalar
 // for TEMP, so its matrix and right-hand side are the ones k_scalar forms.
+// Three waves per SIMD (168 VGPRs, a 44-byte spill) run it 10% faster than
+// the two its natural 180 VGPRs allow (measured at m = 128).
 template <bool TEM>
-__global__ void __launch_bounds__(256) k_scalar_batch(Geo g, ScalBatchArgs a)
+__global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
 {
     const int c = walk_cell(g.w);
     if (c < 0) return;
