@@ -457,3 +457,51 @@ def test_node_interpolation_closed_forms():
             np.testing.assert_allclose(out1[k], base + phi_n / 8.0, rtol=1e-12, atol=1e-12)
         else:
             assert np.array_equal(out1[k], base)
+
+
+def _interior_face_pairs(o, A):
+    M = o.mesh_arrays()
+    faces = M["faces"].astype(np.int64)
+    inter = np.nonzero(faces[:, 1] < o.N)[0]
+    c0, c1 = faces[inter, 0], faces[inter, 1]
+    Ad = A.tocsr()
+    a = np.asarray(Ad[c0, c1]).ravel()   # row cell0, column cell1
+    b = np.asarray(Ad[c1, c0]).ravel()
+    return inter, c0, c1, a, b
+
+
+@pytest.mark.parametrize("which", ["momentum", "TEM", "FUL", "VARP"])
+def test_face_coefficients_are_upwind_convection_plus_diffusion(which):
+    """Momentum and scalar matrices (A7, A9, A16): on every interior face the
+    antisymmetric part of the coefficient pair is the face's mass flux
+    (first-order upwind convection, cell0 -> cell1 positive), the rest is a
+    diffusion coefficient that is positive and, divided by the face's
+    rlencos, the same on every face (one effective viscosity per equation);
+    and every cell without a boundary face satisfies
+    A_c * URF + sum of its off-diagonals = rho V / dt (the unsteady term
+    alone, before under-relaxation).  A swapped owner / neighbour, a flipped
+    flux sign or a lost term breaks one of these."""
+    o = _state_after_momentum(4)
+    urf = 0.5
+    if which != "momentum":
+        o.calculate_pressure()
+        t, f = {"TEM": (po.TEMP, po.TEM), "FUL": (po.FUEL, po.FUL), "VARP": (po.VARPR, po.VARP)}[which]
+        o.scalar_assemble(t, f)
+        urf = 0.95
+    A = _csr(o, "A_vals")
+    mflux = o.face("mflux")
+    inter, c0, c1, a, b = _interior_face_pairs(o, A)
+    mf = mflux[inter]
+    scale = np.maximum(np.abs(a), np.abs(mf))
+    assert np.all(np.abs((a - b) - mf) <= 8 * np.finfo(float).eps * scale)
+    D = -(a + b) / 2 - np.abs(mf) / 2
+    assert np.all(D > 0)
+    visc = D / o.face("face_rlencos")[inter]
+    assert np.ptp(visc) <= 1e-10 * np.abs(visc).max()
+    # cells with six interior faces
+    inner = np.all(o.mesh_arrays()["cell_neighbours"] < o.N, axis=1)
+    d = A.diagonal()
+    offsum = np.asarray((A - sp.diags(d)).sum(axis=1)).ravel()
+    unsteady = o.cell("cell_densities") * o.cell("cell_volumes") / o.opts.delta
+    lhs = d * urf + offsum
+    np.testing.assert_allclose(lhs[inner], unsteady[inner], rtol=1e-11, atol=0)
