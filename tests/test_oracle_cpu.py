@@ -505,3 +505,33 @@ def test_face_coefficients_are_upwind_convection_plus_diffusion(which):
     unsteady = o.cell("cell_densities") * o.cell("cell_volumes") / o.opts.delta
     lhs = d * urf + offsum
     np.testing.assert_allclose(lhs[inner], unsteady[inner], rtol=1e-11, atol=0)
+
+
+@pytest.mark.parametrize("which", ["TEM", "FUL", "VARF"])
+def test_deferred_correction_vanishes_on_the_orthogonal_box(which):
+    """Scalar source assembly (A16): on the box every face normal is parallel
+    to the centre-to-centre vector, so each interior face's explicit diffusion
+    (Visac grad . n) equals its implicit counterpart (VisFace grad . Xpn) and
+    the deferred correction cancels for any gradient.  A cell away from the
+    boundary therefore carries only its cell terms, rho V / dt phi and the
+    under-relaxation (1 - URF) A_c phi.  Checked on rough random fields, so
+    the gradients are arbitrary; a flipped sign or swapped operand in the face
+    source terms shows at the scale of the diffusion flux.  (The momentum
+    sources also hold the transpose-gradient stress, which does not cancel.)"""
+    rng = np.random.default_rng(12)
+    t, f = {"TEM": (po.TEMP, po.TEM), "FUL": (po.FUEL, po.FUL), "VARF": (po.VARFU, po.VARF)}[which]
+    o = _state_after_momentum(4)
+    o.calculate_pressure()
+    ph = o.phi(f)
+    ph[:] = rng.uniform(0.0, 1.0, size=ph.size) * (1000.0 if which == "TEM" else 1.0)
+    o.get_phi_gradients()
+    o.scalar_assemble(t, f)
+    M = o.mesh_arrays()
+    inner = np.all(M["cell_neighbours"] < o.N, axis=1)
+    unsteady = o.cell("cell_densities") * o.cell("cell_volumes") / o.opts.delta
+    A = o.cell("A_V")
+    S = o.cell("S_U")
+    phi = ph[:o.N]
+    expect = unsteady * phi + (1.0 - 0.95) * A * phi
+    np.testing.assert_allclose(S[inner], expect[inner], rtol=1e-12, atol=0)
+    assert np.abs(o.grad(f)[:o.N]).max() > 1.0   # the gradients are far from zero
