@@ -838,19 +838,22 @@ __global__ void k_level_dia(int n, const int *rp, const int *col, const int *map
 // for a mean-zero right-hand side its inverse returns the mean-zero solution
 // of the singular Neumann system.  In-place Gauss-Jordan inversion (SPD, no
 // pivoting) of n <= 128 rows in one workgroup of 1024 threads, the matrix
-// held in registers: thread t owns row t / 8 and the 16 columns
-// j = t % 8 + 8 q.  A step k passes the pivot column and the scaled pivot
-// row through LDS (double-buffered by the parity of k, so two barriers per
-// step suffice) and updates every other element in place.
+// held in registers and padded with zeros to 128 x 128: thread t owns row
+// t / 8 and the 16 consecutive columns 16 (t % 8) .. 16 (t % 8) + 15, so no
+// element needs a bounds test and the pivot row is read as contiguous words.
+// Only the first n pivots are taken; the zero padding then stays zero and the
+// leading n x n block holds the inverse.  A step k passes the pivot column and
+// the scaled pivot row through LDS (double-buffered by the parity of k, so two
+// barriers per step suffice) and updates every other element in place.
 __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, const int *col, const double *v,
                                                         double *inv)
 {
     constexpr int QN = DENSE_MAX / 8;
     __shared__ double colk[2][DENSE_MAX];
-    __shared__ double prow[2][DENSE_MAX];
+    __shared__ __align__(16) double prow[2][DENSE_MAX];
     __shared__ double red[1024 / 64];
     const int tid = threadIdx.x;
-    const int i = tid >> 3, jb = tid & 7;
+    const int i = tid >> 3, j0 = QN * (tid & 7);
     double s = 0.0;
     for (int r = tid; r < n; r += blockDim.x)
         for (int k = rp[r]; k < rp[r + 1]; k++) if (col[k] == r) s += v[k];
@@ -862,45 +865,37 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
     const double alpha = (tot / n) / n;
     double m[QN];
 #pragma unroll
-    for (int q = 0; q < QN; q++) m[q] = alpha;
+    for (int q = 0; q < QN; q++) m[q] = (i < n && j0 + q < n) ? alpha : 0.0;
     if (i < n)
         for (int k = rp[i]; k < rp[i + 1]; k++) {
-            const int j = col[k];
-            if ((j & 7) != jb) continue;
+            const int jq = col[k] - j0;
 #pragma unroll
-            for (int q = 0; q < QN; q++) if (q == (j >> 3)) m[q] += v[k];
+            for (int q = 0; q < QN; q++) if (q == jq) m[q] += v[k];
         }
     for (int k = 0; k < n; k++) {
-        const int buf = k & 1, kq = k >> 3;
-        if (i < n && jb == (k & 7)) {
+        const int buf = k & 1, kq = k - j0;
+        if (kq >= 0 && kq < QN) {
             double mk = 0.0;
 #pragma unroll
             for (int q = 0; q < QN; q++) if (q == kq) mk = m[q];
             colk[buf][i] = mk;
         }
         __syncthreads();
-        const double akk = colk[buf][k];
         if (i == k) {
-            const double rk = 1.0 / akk;
+            const double rk = 1.0 / colk[buf][k];
 #pragma unroll
             for (int q = 0; q < QN; q++) {
-                const int j = jb + 8 * q;
-                if (j < n) {
-                    m[q] = ((j == k) ? 1.0 : m[q]) * rk;
-                    prow[buf][j] = m[q];
-                }
+                m[q] = ((q == kq) ? 1.0 : m[q]) * rk;
+                prow[buf][j0 + q] = m[q];
             }
         }
         __syncthreads();
-        if (i < n && i != k) {
+        if (i != k) {
             const double cik = colk[buf][i];
 #pragma unroll
             for (int q = 0; q < QN; q++) {
-                const int j = jb + 8 * q;
-                if (j < n) {
-                    const double mij = (j == k) ? 0.0 : m[q];
-                    m[q] = mij - cik * prow[buf][j];
-                }
+                const double mij = (q == kq) ? 0.0 : m[q];
+                m[q] = mij - cik * prow[buf][j0 + q];
             }
         }
     }
@@ -908,7 +903,7 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
     if (i < n)
 #pragma unroll
         for (int q = 0; q < QN; q++) {
-            const int j = jb + 8 * q;
+            const int j = j0 + q;
             if (j < n) inv[(size_t)j * n + i] = m[q];
         }
 }

@@ -32,9 +32,10 @@ ARR_PARTICLE_TERMS = 80
 
 KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "pcg_spmv64", "pcg_spmv32", "vcycle",
             "flux_correct", "mpnn_fwd", "jac64", "jac32", "bicg_spmv", "scalar_batch", "ir_resid", "mpnn_bwd"]
-# kernel symbol (as a profiler lists it) of each single-kernel class
+# kernel symbol (as a profiler lists it) of each single-kernel class (p_asm
+# times two kernels, mc::k_mass_faces then mc::k_mass_rows)
 KC_KERNELS = {"grad11": "mc::k_grad<11, false>", "grad1": "mc::k_grad<1, false>", "mom_asm": "mc::k_momentum",
-              "p_asm": "mc::k_mass_pressure", "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<-7, double>",
+              "scalar_asm": "mc::k_scalar(", "pcg_spmv64": "mc::k_pcg_pq<-7, double>",
               "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
               "scalar_batch": "mc::k_scalar_batch<true>", "ir_resid": "mc::k_ir_resid<-7>"}
 KC_COUNT = len(KC_NAMES)
