@@ -201,6 +201,12 @@ static int create_common(const mc_options *opt, mc_ctx **out, mc_ctx **made)
     if (v) { delete x; return v; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return MC_ERR_HIP; }
     c->own_stream = true;
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_coarse, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_dense, hipEventDisableTiming) != hipSuccess) {
+        (void)mc_ctx_destroy(x);
+        return MC_ERR_HIP;
+    }
     set_constants(c);
     *made = x;
     return MC_OK;
@@ -233,11 +239,15 @@ int mc_ctx_destroy(mc_ctx *x)
     if (!x) return MC_OK;
     Ctx *c = &x->c;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     mpnn_destroy(c);
     profile_release(c);
     dfree_all(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+    if (c->ev_coarse) (void)hipEventDestroy(c->ev_coarse);
+    if (c->ev_dense) (void)hipEventDestroy(c->ev_dense);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete x;
     return MC_OK;
 }

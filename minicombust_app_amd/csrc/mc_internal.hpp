@@ -73,6 +73,11 @@ struct AmgLevel {
 struct Ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // the coarsest AMG level's inverse is formed on `side` (one workgroup)
+    // while `stream` runs the first V-cycle's descent; ev_dense marks it done
+    hipStream_t side = nullptr;
+    hipEvent_t ev_coarse = nullptr, ev_dense = nullptr;
+    bool dense_pending = false;
     mc_options opt{};
     Consts k{};
     std::string err;

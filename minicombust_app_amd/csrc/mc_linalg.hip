@@ -1017,9 +1017,19 @@ int amg_setup(Ctx *c, const double *vals)
     return amg_update_values(c, vals);
 }
 
+// The main stream waits for the coarsest inverse formed on the side stream.
+static int dense_join(Ctx *c)
+{
+    if (!c->dense_pending) return MC_OK;
+    MC_HIP(c, hipStreamWaitEvent(c->stream, c->ev_dense, 0));
+    c->dense_pending = false;
+    return MC_OK;
+}
+
 int amg_update_values(Ctx *c, const double *vals)
 {
     if (!c->amg_ready) return amg_setup(c, vals);
+    MC_TRY(dense_join(c));   // the previous inverse still reads the coarsest values
     const double *vf = vals;
     for (size_t l = 0; l < c->amg.size(); l++) {
         AmgLevel &L = c->amg[l];
@@ -1042,9 +1052,15 @@ int amg_update_values(Ctx *c, const double *vals)
     }
     AmgLevel &Lc = c->amg.back();
     if (Lc.n > DENSE_MAX) { c->err = "coarsest AMG level larger than the dense limit"; return MC_ERR_STATE; }
-    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(1024), 0, c->stream, Lc.n, Lc.row_ptr, Lc.col, Lc.vals,
+    // one workgroup for ~0.2 ms: run it beside the work that precedes the
+    // first coarsest-level solve (residual, first V-cycle's descent)
+    MC_HIP(c, hipEventRecord(c->ev_coarse, c->stream));
+    MC_HIP(c, hipStreamWaitEvent(c->side, c->ev_coarse, 0));
+    hipLaunchKernelGGL(k_dense_inverse, dim3(1), dim3(1024), 0, c->side, Lc.n, Lc.row_ptr, Lc.col, Lc.vals,
                        c->amg_dense);
     MC_LAUNCH_CHECK(c);
+    MC_HIP(c, hipEventRecord(c->ev_dense, c->side));
+    c->dense_pending = true;
     return MC_OK;
 }
 
@@ -1270,6 +1286,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     smoother_weights(c, std::max(1, std::min(c->opt.amg_nu, 8)), wts);
     const int nu = std::max(1, c->opt.amg_nu);
     if (l + 1 == c->amg.size()) {
+        MC_TRY(dense_join(c));
         hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(1024), 0, c->stream, L.n, c->amg_dense, b, x, stop);
         if (ks) {  // one-level hierarchy: no smoothing sweep carries the (r, z) reduction
             const Grid g = grid_for(L.n);
