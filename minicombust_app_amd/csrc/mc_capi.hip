@@ -293,6 +293,7 @@ int mc_reset_fields(mc_ctx *x)
     MC_HIP(c, hipMemsetAsync(c->PA_vals, 0, (size_t)c->nnz * 8, c->stream));
     MC_HIP(c, hipMemsetAsync(c->Pb, 0, (size_t)c->N * 8, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
+    c->pa_dia_fresh = false;
     c->first_pressure = true;
     c->field_set_up = false;
     c->timestep_count = 0;
@@ -542,6 +543,7 @@ int mc_array_get(mc_ctx *x, int id, void *host, int64_t count)
 int mc_array_set(mc_ctx *x, int id, const void *host, int64_t count)
 {
     CTX_OR_FAIL
+    c->pa_dia_fresh = false;   // the AMG fine level is refreshed from whatever the arrays now hold
     ArrDesc d;
     MC_TRY(describe(c, id, d));
     if (!host || count != d.count) { c->err = "array size mismatch (expected " + std::to_string(d.count) + ")"; return MC_ERR_ARG; }

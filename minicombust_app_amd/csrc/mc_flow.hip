@@ -627,6 +627,12 @@ struct MassArgs {
     double *mflux, *ff0, *ff1, *SU, *AV, *Pb, *PAvals;
     int assemble;      // 1: setup_pressure_matrix too
     int into_pressure; // write the pressure matrix and RHS
+    // with into_pressure: the AMG fine level's diagonal copies of the matrix
+    // (FP64 without the pskip lower diagonals, FP32 in full) and inverse
+    // diagonal, written here instead of by a pass over the CSR values
+    double *pdia, *pdinv;
+    float *pdia32, *pdinv32;
+    int pskip;
 };
 
 // The per-cell values the Rhie-Chow flux reads on either side of a face.
@@ -754,6 +760,20 @@ __global__ void __launch_bounds__(256) k_mass_rows(Geo g, MassArgs a)
         const int rp = g.row_ptr[c];
         a.PAvals[rp + dpos] = AV;
         for (int k = 0; k < nint; k++) a.PAvals[rp + pos[k]] = own_v[k];
+        if (a.pdia) {   // the values and order of k_level_dia's pass
+            const size_t n = (size_t)N;
+            const int d0 = dslot(g, 0);
+            a.pdia[(size_t)(d0 - a.pskip) * n + c] = AV;
+            a.pdia32[(size_t)d0 * n + c] = (float)AV;
+            for (int k = 0; k < nint; k++) {
+                const int d = dslot(g, own_nb[k] - c);
+                a.pdia32[(size_t)d * n + c] = (float)own_v[k];
+                if (d >= a.pskip) a.pdia[(size_t)(d - a.pskip) * n + c] = own_v[k];
+            }
+            const double di = (AV != 0.0) ? 1.0 / AV : 0.0;
+            a.pdinv[c] = di;
+            a.pdinv32[c] = (float)di;
+        }
     }
 }
 
@@ -775,6 +795,7 @@ static MassArgs mass_args(Ctx *c, int assemble, int into_pressure)
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.SU = c->SU; a.AV = c->AV; a.Pb = c->Pb;
     a.PAvals = c->PA_vals;
     a.assemble = assemble; a.into_pressure = into_pressure;
+    a.pdia = nullptr; a.pdinv = nullptr; a.pdia32 = nullptr; a.pdinv32 = nullptr; a.pskip = 0;
     return a;
 }
 
@@ -792,9 +813,25 @@ int calculate_mass_flux_assemble(Ctx *c)
 {
     const bool into = c->first_pressure || c->opt.mode == MC_MODE_SIMPLE;
     MC_TRY(boundary_mass_flux(c));
-    KTimer kt(c, MC_KC_P_ASM, 184.0 * c->N + 88.0 * c->F + 8.0 * c->nnz);
-    launch_mass(c, mass_args(c, 1, into ? 1 : 0));
+    MassArgs a = mass_args(c, 1, into ? 1 : 0);
+    // an AMG hierarchy built on this pattern takes its fine level's
+    // diagonal copies from the assembly
+    bool l0 = into && c->amg_ready && !c->amg.empty();
+    if (l0) {
+        const AmgLevel &L = c->amg[0];
+        l0 = L.vals == c->PA_vals && L.nd > 0 && L.nd == c->fine_nd && L.dia && L.dia32;
+        for (int d = 0; l0 && d < L.nd; d++) l0 = L.off[d] == c->fine_off[d];
+        if (l0) {
+            a.pdia = L.dia; a.pdia32 = L.dia32; a.pdinv = L.dinv; a.pdinv32 = L.dinv32;
+            a.pskip = L.sym ? L.nd / 2 : 0;
+        }
+    }
+    {
+        KTimer kt(c, MC_KC_P_ASM, 184.0 * c->N + 88.0 * c->F + 8.0 * c->nnz);
+        launch_mass(c, a);
+    }
     MC_LAUNCH_CHECK(c);
+    c->pa_dia_fresh = l0;
     return MC_OK;
 }
 

@@ -1031,17 +1031,22 @@ int amg_update_values(Ctx *c, const double *vals)
     if (!c->amg_ready) return amg_setup(c, vals);
     MC_TRY(dense_join(c));   // the previous inverse still reads the coarsest values
     const double *vf = vals;
+    // the pressure assembly wrote level 0's diagonal copies with the values
+    const bool l0_done = c->pa_dia_fresh && vals == c->PA_vals;
+    c->pa_dia_fresh = false;
     for (size_t l = 0; l < c->amg.size(); l++) {
         AmgLevel &L = c->amg[l];
         if (L.vals != vf)
             MC_HIP(c, hipMemcpyAsync(L.vals, vf, (size_t)L.nnz * 8, hipMemcpyDeviceToDevice, c->stream));
-        if (L.nd)
-            hipLaunchKernelGGL(k_level_dia, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
-                               L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32, L.sym ? L.nd / 2 : 0);
-        else
-            hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr, L.col,
-                               L.vals, L.dinv, L.dinv32, L.vals32, L.nnz);
-        MC_LAUNCH_CHECK(c);
+        if (!(l == 0 && l0_done)) {
+            if (L.nd)
+                hipLaunchKernelGGL(k_level_dia, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr,
+                                   L.col, L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32, L.sym ? L.nd / 2 : 0);
+            else
+                hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr,
+                                   L.col, L.vals, L.dinv, L.dinv32, L.vals32, L.nnz);
+            MC_LAUNCH_CHECK(c);
+        }
         if (l + 1 < c->amg.size()) {
             AmgLevel &Lc = c->amg[l + 1];
             hipLaunchKernelGGL(k_gather_sum, dim3(div_up(Lc.nnz, 256)), dim3(256), 0, c->stream, Lc.nnz, L.gptr, L.gmap,
