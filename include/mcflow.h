@@ -87,6 +87,10 @@ typedef struct mc_options {
     int    limit_grad;     /* 1: Venkatakrishnan limiter on the 11 gradients of each
                               SIMPLE iteration (FLOW_SOLVER_LIMIT_GRAD,
                               utils.hpp:26, FlowSolver.inl:3197; 0 as there)     */
+    int    amg_refresh;    /* the coarse AMG levels (Galerkin values, coarsest inverse)
+                              are recomputed at every amg_refresh-th pressure solve
+                              and reused in between (1: every solve); the fine
+                              level always holds the current matrix              */
 } mc_options;
 
 typedef struct mc_solve_status {

@@ -140,6 +140,7 @@ struct Ctx {
     double *A_dia = nullptr;       // momentum / scalar matrix in diagonal storage; with a diagonal
                                    // pattern the assembly kernels write it directly (A_vals unused)
     double *A_dia2 = nullptr;      // the same for the temperature matrix (A_vals2)
+    int64_t amg_updates = 0;       // pressure-matrix value updates since the hierarchy was built
     bool pa_dia_fresh = false;     // the pressure assembly also wrote the AMG fine level's diagonal copies
     bool first_pressure = true;
     bool field_set_up = false;   // set_up_field has run (first pass of timestep 0)

@@ -1014,6 +1014,7 @@ int amg_setup(Ctx *c, const double *vals)
     c->amg_coarse_n = nco;
     MC_TRY(dalloc(c, &c->amg_dense, (size_t)nco * nco));
     c->amg_ready = true;
+    c->amg_updates = 0;
     return amg_update_values(c, vals);
 }
 
@@ -1034,6 +1035,11 @@ int amg_update_values(Ctx *c, const double *vals)
     // the pressure assembly wrote level 0's diagonal copies with the values
     const bool l0_done = c->pa_dia_fresh && vals == c->PA_vals;
     c->pa_dia_fresh = false;
+    // between refreshes the coarse levels keep the values of an earlier
+    // matrix: the V-cycle stays a fixed symmetric operator for the solve
+    const bool coarse =
+        c->amg.size() == 1 || c->opt.amg_refresh <= 1 || c->amg_updates % c->opt.amg_refresh == 0;
+    c->amg_updates++;
     for (size_t l = 0; l < c->amg.size(); l++) {
         AmgLevel &L = c->amg[l];
         if (L.vals != vf)
@@ -1048,6 +1054,7 @@ int amg_update_values(Ctx *c, const double *vals)
             MC_LAUNCH_CHECK(c);
         }
         if (l + 1 < c->amg.size()) {
+            if (!coarse) return MC_OK;
             AmgLevel &Lc = c->amg[l + 1];
             hipLaunchKernelGGL(k_gather_sum, dim3(div_up(Lc.nnz, 256)), dim3(256), 0, c->stream, Lc.nnz, L.gptr, L.gmap,
                                L.vals, Lc.vals);

@@ -394,3 +394,24 @@ def test_repair_nonneg_bits(m):
             assert np.all(o.phi(field)[:o.N] > 0)
     o.close()
     g.close()
+
+
+def test_amg_coarse_levels_reused_between_refreshes():
+    """amg_refresh = 4 keeps the coarse AMG levels of an earlier pressure matrix
+    for three solves in four; the fine level always holds the current matrix,
+    so every solve still meets its tolerance on the current system and the
+    fields track those of a hierarchy refreshed at every solve."""
+    print("amg refresh m=16", flush=True)
+    runs = {}
+    for r in (1, 4):
+        with mc.Context(16, mode=mc.MODE_SIMPLE, p_rtol=1e-10, mom_rtol=1e-12, amg_refresh=r) as g:
+            for it in range(9):
+                g.simple_iteration(it % 3)
+            st = g.stats()
+            assert st["p_relres_max"] <= 1.05e-10, (r, st["p_relres_max"])
+            runs[r] = (st["p_iters"], {f: g.phi(f).copy() for f in (po.U, po.V, po.W, po.P, po.TE, po.TEM)})
+    it1, f1 = runs[1]
+    it4, f4 = runs[4]
+    assert it4 <= 1.2 * it1 + 2, (it1, it4)
+    for f in f1:
+        assert_close(f4[f], f1[f], "field %s, refresh 4 vs 1" % po.FIELD_NAMES[f], 1e-7)
