@@ -91,6 +91,9 @@ typedef struct mc_options {
                               are recomputed at every amg_refresh-th pressure solve
                               and reused in between (1: every solve); the fine
                               level always holds the current matrix              */
+    int    amg_tail;       /* 1: the V-cycle from the first level of at most 2048
+                              rows down runs in one workgroup (same results, fewer
+                              launches); 0: one launch per pass                  */
 } mc_options;
 
 typedef struct mc_solve_status {

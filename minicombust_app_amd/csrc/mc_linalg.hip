@@ -1288,15 +1288,207 @@ template <> float *lv_b<float>(AmgLevel &L) { return L.b32; }
 template <> float *lv_t<float>(AmgLevel &L) { return L.t32; }
 template <> float *lv_dinv<float>(AmgLevel &L) { return L.dinv32; }
 
+// ----------------------------------------------------- V-cycle tail --
+// The levels of at most TAIL_ROWS rows, and the dense coarsest level, in one
+// workgroup: each launch of vcycle() on those levels becomes a pass of the
+// 1024 threads over the level's rows, the passes separated by barriers, so
+// half a dozen launches of a few microseconds each become one.  Every row is
+// formed by the expressions of the per-pass kernels (k_jac_first2, k_jac,
+// k_resid, k_restrict, k_dense_apply) in the same order, so the result is the
+// same.  Measured at m = 128: a tail from the 1,024-row level saves 0.05 ms per
+// pressure solve; one from the 8,192-row level costs 0.3 ms more than the
+// launches it replaces (one CU's passes over 8K rows are slower).
+constexpr int TAIL_LEVELS = 4, TAIL_ROWS = 2048, TAIL_THREADS = 1024;
+
+template <typename T>
+struct TailLevel {
+    Mat<T> A;
+    const T *dinv;
+    T *b, *x, *t;
+    const int *agg, *aptr, *amem;   // aggregation to the next level
+    int nc;
+};
+
+template <typename T>
+struct TailArgs {
+    TailLevel<T> lv[TAIL_LEVELS + 1];   // lv[nl]: the dense level (b, x only)
+    int nl;                             // smoothed levels
+    const double *inv;                  // dense inverse, transposed
+    int nd;                             // dense level size
+    int nu;
+    T cs;
+    double w[8];
+};
+
+template <int ND, typename T, typename XF>
+__device__ __forceinline__ double tail_ax(const Mat<T> &A, int i, XF X)
+{
+    constexpr bool SYM = ND < 0;
+    constexpr int NA = SYM ? -ND : ND;
+    double a[NA > 0 ? NA : 1];
+    int j[NA > 0 ? NA : 1];
+#pragma unroll
+    for (int d = 0; d < NA; d++) {
+        a[d] = dia_at<SYM>(A, d, i);
+        j[d] = clampi(i + A.off[d], A.n);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < NA; d++) acc += a[d] * X(j[d]);
+    return acc;
+}
+
+template <int ND, typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_vtail(TailArgs<T> a, const KState *stop)
+{
+    if constexpr (ND != 7 && ND != -7) {
+        return;   // 7-diagonal storage only (the host launches no other)
+    } else {
+        if (stop && stop->all_done) return;
+        __shared__ double bs[DENSE_MAX];
+        __shared__ double part[TAIL_THREADS / DENSE_MAX][DENSE_MAX];
+        const int tid = threadIdx.x;
+        T *curs[TAIL_LEVELS], *oths[TAIL_LEVELS];
+        const int nu = a.nu;
+#pragma unroll
+        for (int l = 0; l < TAIL_LEVELS; l++) {
+            if (l >= a.nl) break;
+            const TailLevel<T> &L = a.lv[l];
+            const int n = L.A.n;
+            T *cur = L.x, *oth = L.t;
+            const T w1 = (T)a.w[0], w2 = (T)a.w[nu >= 2 ? 1 : 0];
+            for (int i = tid; i < n; i += TAIL_THREADS) {
+                if (nu >= 2) {
+                    const double ax = tail_ax<ND>(L.A, i, [&](int jj) { return (double)(w1 * L.dinv[jj] * L.b[jj]); });
+                    const T x1 = w1 * L.dinv[i] * L.b[i];
+                    cur[i] = (T)(x1 + w2 * L.dinv[i] * (L.b[i] - (T)ax));
+                } else {
+                    cur[i] = w1 * L.dinv[i] * L.b[i];
+                }
+            }
+            __syncthreads();
+            for (int s = 2; s < nu; s++) {
+                const T omega = (T)a.w[s];
+                for (int i = tid; i < n; i += TAIL_THREADS) {
+                    const double ax = tail_ax<ND>(L.A, i, [&](int jj) { return (double)cur[jj]; });
+                    const T y = cur[i];
+                    oth[i] = y + omega * L.dinv[i] * (L.b[i] - (T)ax);
+                }
+                __syncthreads();
+                T *sw = cur; cur = oth; oth = sw;
+            }
+            T *r = (cur == L.x) ? L.t : L.x;
+            for (int i = tid; i < n; i += TAIL_THREADS)
+                r[i] = (T)((double)L.b[i] - tail_ax<ND>(L.A, i, [&](int jj) { return (double)cur[jj]; }));
+            __syncthreads();
+            T *bc = a.lv[l + 1].b;
+            for (int I = tid; I < L.nc; I += TAIL_THREADS) {
+                double sum = 0.0;
+                for (int t = L.aptr[I]; t < L.aptr[I + 1]; t++) sum += (double)r[L.amem[t]];
+                bc[I] = (T)sum;
+            }
+            __syncthreads();
+            curs[l] = cur;
+            oths[l] = oth;
+        }
+        {   // dense coarsest level
+            constexpr int PARTS = TAIL_THREADS / DENSE_MAX, PLEN = DENSE_MAX / PARTS;
+            const int nd = a.nd;
+            const T *bd = a.lv[a.nl].b;
+            T *xd = a.lv[a.nl].x;
+            if (tid < nd) bs[tid] = (double)bd[tid];
+            __syncthreads();
+            const int i = tid % DENSE_MAX, p = tid / DENSE_MAX;
+            double sum = 0.0;
+            if (i < nd) {
+                const int j0 = p * PLEN;
+#pragma unroll
+                for (int jj = 0; jj < PLEN; jj++) {
+                    const int j = j0 + jj;
+                    if (j < nd) sum += a.inv[(size_t)j * nd + i] * bs[j];
+                }
+            }
+            part[p][i] = sum;
+            __syncthreads();
+            if (tid < nd) {
+                double y = 0.0;
+#pragma unroll
+                for (int q = 0; q < PARTS; q++) y += part[q][tid];
+                xd[tid] = (T)y;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int l = TAIL_LEVELS - 1; l >= 0; l--) {
+            if (l >= a.nl) continue;
+            const TailLevel<T> &L = a.lv[l];
+            const int n = L.A.n;
+            const T *xc = a.lv[l + 1].x;
+            T *cur = curs[l], *oth = oths[l], *x = L.x, *tmp = L.t;
+            for (int s = 0; s < nu; s++) {
+                const bool first = s == 0, last = s == nu - 1;
+                T *dst = last ? x : oth;
+                if (dst == cur) dst = tmp;
+                const T omega = (T)a.w[nu - 1 - s];
+                auto yv = [&](int jj) -> T { return first ? (T)(cur[jj] + a.cs * xc[L.agg[jj]]) : cur[jj]; };
+                for (int i = tid; i < n; i += TAIL_THREADS) {
+                    const double ax = tail_ax<ND>(L.A, i, [&](int jj) { return (double)yv(jj); });
+                    const T y = yv(i);
+                    dst[i] = y + omega * L.dinv[i] * (L.b[i] - (T)ax);
+                }
+                __syncthreads();
+                oth = cur;
+                cur = dst;
+            }
+            if (cur != x) {
+                for (int i = tid; i < n; i += TAIL_THREADS) x[i] = cur[i];
+                __syncthreads();
+            }
+        }
+    }
+}
+
 // V-cycle on level l: x_l = M^-1 b_l.  On level 0 with ks != nullptr the
 // last sweep also reduces (b, x) into the PCG's rho / beta.
 template <typename T>
 static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *stop)
 {
     AmgLevel &L = c->amg[l];
-    double wts[8];
+    double wts[8] = {};
     smoother_weights(c, std::max(1, std::min(c->opt.amg_nu, 8)), wts);
     const int nu = std::max(1, c->opt.amg_nu);
+    if (l >= 1 && l + 1 < c->amg.size() && c->opt.amg_tail) {
+        // the rest of the cycle in one workgroup when every level left is
+        // small and in diagonal storage of one kind
+        const size_t nl = c->amg.size() - 1 - l;
+        const int code = nd_code(level_mat<T>(L));
+        bool ok = nl <= (size_t)TAIL_LEVELS && (code == 7 || code == -7) && nu <= 8 && b == lv_b<T>(L) &&
+                  x == lv_x<T>(L);
+        for (size_t k = l; ok && k + 1 < c->amg.size(); k++)
+            ok = c->amg[k].n <= TAIL_ROWS && nd_code(level_mat<T>(c->amg[k])) == code;
+        if (ok) {
+            TailArgs<T> ta;
+            memset(&ta, 0, sizeof(ta));
+            ta.nl = (int)nl;
+            for (size_t k = 0; k < nl; k++) {
+                AmgLevel &Lk = c->amg[l + k];
+                TailLevel<T> &t = ta.lv[k];
+                t.A = level_mat<T>(Lk);
+                t.dinv = lv_dinv<T>(Lk);
+                t.b = lv_b<T>(Lk); t.x = lv_x<T>(Lk); t.t = lv_t<T>(Lk);
+                t.agg = Lk.agg; t.aptr = Lk.agg_ptr; t.amem = Lk.agg_members; t.nc = Lk.nc;
+            }
+            AmgLevel &Ld = c->amg.back();
+            ta.lv[nl].b = lv_b<T>(Ld); ta.lv[nl].x = lv_x<T>(Ld);
+            ta.inv = c->amg_dense; ta.nd = Ld.n;
+            ta.nu = nu; ta.cs = (T)c->opt.amg_cscale;
+            for (int k = 0; k < 8; k++) ta.w[k] = wts[k];
+            MC_TRY(dense_join(c));
+            MC_DISPATCH_ND(code, (k_vtail<ND_, T>), dim3(1), dim3(TAIL_THREADS), 0, c->stream, ta, stop);
+            MC_LAUNCH_CHECK(c);
+            return MC_OK;
+        }
+    }
     if (l + 1 == c->amg.size()) {
         MC_TRY(dense_join(c));
         hipLaunchKernelGGL(k_dense_apply<T>, dim3(1), dim3(1024), 0, c->stream, L.n, c->amg_dense, b, x, stop);

@@ -415,3 +415,29 @@ def test_amg_coarse_levels_reused_between_refreshes():
     assert it4 <= 1.2 * it1 + 2, (it1, it4)
     for f in f1:
         assert_close(f4[f], f1[f], "field %s, refresh 4 vs 1" % po.FIELD_NAMES[f], 1e-7)
+
+
+@pytest.mark.parametrize("psolve", [mc.PSOLVE_FP64, mc.PSOLVE_MIXED])
+def test_amg_tail_is_the_same_cycle(psolve):
+    """The single-workgroup pass over the small AMG levels (amg_tail = 1)
+    forms every row as the per-pass launches do (amg_tail = 0): the pressure
+    solves agree bit for bit, iteration for iteration."""
+    print("amg tail psolve=%d" % psolve, flush=True)
+    m = 32   # 65,536 cells: levels of 8,192 and 1,024 rows and a dense 128 below the fine one,
+             # the tail from the 1,024-row level
+    o = po.Oracle(m, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    o.get_phi_gradients()
+    o.set_up_field()
+    o.calculate_UVW()
+    oracle_pressure_assemble(o)
+    b = o.cell("S_U").copy()
+    out = {}
+    for tail in (0, 1):
+        with mc.Context(m, mode=mc.MODE_SIMPLE, psolve=psolve, p_rtol=1e-10, amg_tail=tail) as g:
+            push_state(o, g)
+            x, st = g.solve_pressure(b)
+            assert st.converged
+            out[tail] = (x.copy(), st.iterations)
+    assert out[0][1] == out[1][1]
+    assert_bits(out[1][0], out[0][0], "pressure correction, tail vs per-pass launches")
+    o.close()
