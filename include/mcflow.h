@@ -83,7 +83,8 @@ typedef struct mc_options {
     int    cell_tile;      /* generated box mesh: x extent of the 256-cell brick
                               (cell_tile x 1 x 256/cell_tile) one workgroup of the
                               cell-centric assembly kernels visits; a power of two
-                              in [1, 256], 0: the default (64)                   */
+                              in [1, 256], 0: the widest that leaves at most 1/8
+                              of a row's lanes idle                              */
     int    limit_grad;     /* 1: Venkatakrishnan limiter on the 11 gradients of each
                               SIMPLE iteration (FLOW_SOLVER_LIMIT_GRAD,
                               utils.hpp:26, FlowSolver.inl:3197; 0 as there)     */
@@ -91,6 +92,9 @@ typedef struct mc_options {
                               are recomputed at every amg_refresh-th pressure solve
                               and reused in between (1: every solve); the fine
                               level always holds the current matrix              */
+    int    cell_ychunk;    /* generated box mesh: y rows per chunk of the bricks'
+                              visiting order (chunks in y; within one, z-slab by
+                              z-slab, row by row, tile by tile), 0: 16          */
     int    amg_tail;       /* 1: the V-cycle from the first level of at most 2048
                               rows down runs in one workgroup (same results, fewer
                               launches); 0: one launch per pass                  */

@@ -242,6 +242,7 @@ struct CellWalk {
     int N = 0;
     int nx = 0, ny = 0, nz = 0;   // nx == 0: consecutive cells
     int lgtx = 0, ntx = 0, ntz = 0;
+    int yc = 1;                   // y rows per chunk of the block order
 };
 
 __device__ __forceinline__ int walk_cell(const CellWalk &w)
@@ -252,8 +253,15 @@ __device__ __forceinline__ int walk_cell(const CellWalk &w)
         const int c = b * 256 + t;
         return c < w.N ? c : -1;
     }
-    const int y = b % w.ny, r = b / w.ny;
-    const int xt = r % w.ntx, zt = r / w.ntx;
+    // blocks in chunks of yc consecutive y rows; within a chunk z-slab by
+    // z-slab, each slab row by row, each row tile by tile: a cell's x, y and z
+    // neighbours are visited a few blocks apart (L2-resident), except across a
+    // chunk's two y ends
+    const int xt = b % w.ntx, r = b / w.ntx;
+    const int per = w.yc * w.ntz;
+    const int ch = r / per, q = r - ch * per;
+    const int zt = q / w.yc, y = ch * w.yc + (q - zt * w.yc);
+    if (y >= w.ny) return -1;
     const int x = (xt << w.lgtx) + (t & ((1 << w.lgtx) - 1));
     const int z = zt * (256 >> w.lgtx) + (t >> w.lgtx);
     if (x >= w.nx || z >= w.nz) return -1;
@@ -269,7 +277,7 @@ inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 // Workgroups (of 256 threads) that cover every cell in CellWalk order.
 inline int walk_blocks(const CellWalk &w)
 {
-    return w.nx ? w.ny * w.ntx * w.ntz : div_up(w.N, 256);
+    return w.nx ? div_up(w.ny, w.yc) * w.yc * w.ntx * w.ntz : div_up(w.N, 256);
 }
 
 inline CellWalk cell_walk(const Ctx *c)
@@ -277,11 +285,18 @@ inline CellWalk cell_walk(const Ctx *c)
     CellWalk w;
     w.N = c->N;
     if (c->m > 0) {   // generated box: 2m x m x m cells
-        const int tx = c->opt.cell_tile > 0 ? c->opt.cell_tile : 64;
+        w.nx = (int)(2 * c->m); w.ny = (int)c->m; w.nz = (int)c->m;
+        // default tile: the widest power of two up to 256 that leaves at most
+        // 1/8 of the lanes of a row idle (m = 128: 256, the whole row)
+        int tx = c->opt.cell_tile;
+        if (tx <= 0) {
+            tx = 256;
+            while (tx > 1 && (int64_t)8 * (div_up(w.nx, tx) * tx - w.nx) > w.nx) tx >>= 1;
+        }
         int lg = 0;
         while ((1 << lg) < tx) lg++;
-        w.nx = (int)(2 * c->m); w.ny = (int)c->m; w.nz = (int)c->m;
         w.lgtx = lg; w.ntx = div_up(w.nx, tx); w.ntz = div_up(w.nz, 256 / tx);
+        w.yc = c->opt.cell_ychunk > 0 ? c->opt.cell_ychunk : 16;
     }
     return w;
 }

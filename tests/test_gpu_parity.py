@@ -441,3 +441,20 @@ def test_amg_tail_is_the_same_cycle(psolve):
     assert out[0][1] == out[1][1]
     assert_bits(out[1][0], out[0][0], "pressure correction, tail vs per-pass launches")
     o.close()
+
+
+def test_cell_visiting_order_does_not_change_results():
+    """The bricks and the order in which the cell-centric kernels visit them
+    (cell_tile, cell_ychunk) change only which workgroup forms a cell's row:
+    three SIMPLE iterations give the same bits for every field and flux."""
+    print("visiting order m=12", flush=True)
+    runs = []
+    for tile, yc in ((0, 0), (64, 1), (4, 5), (256, 16), (1, 12)):
+        with mc.Context(12, mode=mc.MODE_SIMPLE, cell_tile=tile, cell_ychunk=yc) as g:
+            for it in range(3):
+                g.simple_iteration(it)
+            runs.append(({f: g.phi(f).copy() for f in range(po.NFIELDS)}, g.get(mc.ARR_MFLUX).copy()))
+    for k, (fields, mflux) in enumerate(runs[1:], 1):
+        for f in fields:
+            assert_bits(fields[f], runs[0][0][f], "order %d field %s" % (k, po.FIELD_NAMES[f]))
+        assert_bits(mflux, runs[0][1], "order %d mass flux" % k)
