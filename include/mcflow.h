@@ -95,6 +95,10 @@ typedef struct mc_options {
     int    cell_ychunk;    /* generated box mesh: y rows per chunk of the bricks'
                               visiting order (chunks in y; within one, z-slab by
                               z-slab, row by row, tile by tile), 0: 16          */
+    int    warm_start;     /* momentum and scalar BiCGStab start from the current field
+                              (a guess no better than zero is dropped for x0 = 0);
+                              0: always x0 = 0 (VecZeroEntries, FlowSolver.inl:1348);
+                              -1: on in MC_MODE_SIMPLE, off in MC_MODE_COMPAT    */
     int    amg_tail;       /* 1: the V-cycle from the first level of at most 2048
                               rows down runs in one workgroup (same results, fewer
                               launches); 0: one launch per pass                  */

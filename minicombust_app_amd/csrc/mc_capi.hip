@@ -167,6 +167,7 @@ int mc_options_default(mc_options *o)
     o->limit_grad = 0;
     o->amg_refresh = 4;   // m = 128: same PCG iteration counts as 1, pressure solve 5% faster (measured)
     o->cell_ychunk = 0;
+    o->warm_start = -1;
     o->amg_tail = 1;
     return MC_OK;
 }
@@ -190,6 +191,7 @@ static int validate(const mc_options *o, std::string &err)
     if (o->amg_refresh < 1 || o->amg_refresh > 1000) { err = "amg_refresh must be in [1, 1000]"; return MC_ERR_ARG; }
     if (o->amg_tail != 0 && o->amg_tail != 1) { err = "amg_tail must be 0 or 1"; return MC_ERR_ARG; }
     if (o->cell_ychunk < 0 || o->cell_ychunk > 4096) { err = "cell_ychunk must be in [0, 4096]"; return MC_ERR_ARG; }
+    if (o->warm_start < -1 || o->warm_start > 1) { err = "warm_start must be -1, 0 or 1"; return MC_ERR_ARG; }
     return MC_OK;
 }
 
@@ -352,7 +354,7 @@ int mc_momentum_solve(mc_ctx *x, mc_solve_status *st)
     double *b[3] = {c->SU, c->SV, c->SW};
     double *xx[3] = {c->phi[MC_U], c->phi[MC_V], c->phi[MC_W]};
     mc_solve_status s{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, xx, c->opt.mom_rtol, c->opt.mom_maxit, &s));
+    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, xx, c->opt.mom_rtol, c->opt.mom_maxit, &s, warm_start(c)));
     c->st.mom_iters += s.iterations;
     if (st) *st = s;
     return s.converged ? MC_OK : MC_ERR_NOCONV;

@@ -528,7 +528,7 @@ int momentum_solve(Ctx *c)
     double *b[3] = {c->SU, c->SV, c->SW};
     double *x[3] = {c->phi[U], c->phi[V], c->phi[W]};
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
     c->st.mom_iters += st.iterations;
     if (!st.converged) { c->err = "momentum BiCGStab did not converge"; return MC_ERR_NOCONV; }
     return MC_OK;
@@ -1450,14 +1450,14 @@ static int batch_solve(Ctx *c, bool tem)
     if (tem) {
         double *bt[1] = {c->SU}, *xt[1] = {c->phi[TEM]};
         mc_solve_status stt{};
-        MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt));
+        MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt, warm_start(c)));
         c->st.scalar_iters += stt.iterations;
         if (!stt.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     }
     double *b[SB], *x[SB];
     for (int s = 0; s < SB; s++) { b[s] = c->srhs[s]; x[s] = c->phi[fields[s]]; }
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     // S_phi.U holds the last equation's right-hand side, as after VARP
@@ -1628,7 +1628,7 @@ int scalar_solve(Ctx *c, int type)
     double *b[1] = {c->SU};
     double *x[1] = {c->phi[field]};
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st));
+    MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     if (type == TERBTE || type == TERBED) MC_TRY(clip_nonneg(c, c->phi[field], type == TERBTE ? 0 : 1));

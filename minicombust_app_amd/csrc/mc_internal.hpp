@@ -448,8 +448,14 @@ int boundary_mass_flux(Ctx *c);
 const double *fine_dia_of(const Ctx *c, const double *csr);
 int dia_to_csr(Ctx *c, const double *dia, double *csr);   // array access in CSR order
 int csr_to_dia(Ctx *c, const double *csr, double *dia);
+// warm: start from the values in x (else x0 = 0)
 int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x,
-                   double rtol, int maxit, mc_solve_status *st);
+                   double rtol, int maxit, mc_solve_status *st, bool warm);
+// the momentum and scalar solves start from the current field
+inline bool warm_start(const Ctx *c)
+{
+    return c->opt.warm_start > 0 || (c->opt.warm_start < 0 && c->opt.mode == MC_MODE_SIMPLE);
+}
 int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, double atol = 0.0);
 int amg_setup(Ctx *c, const double *vals);
 int amg_update_values(Ctx *c, const double *vals);

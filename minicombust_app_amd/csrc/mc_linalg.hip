@@ -67,6 +67,7 @@ struct KState {
     double tol[4], bnorm[4], rnorm[4], sn[4], mean;
     double atol;
     int done[4], half[4], iters[4], conv[4];
+    int zero_start[4];   // BiCGStab: the initial guess is dropped for x0 = 0
     int all_done, nr, maxit, pad;
 };
 
@@ -340,6 +341,72 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init(int n, BiVecs w, double *part
     block_partials<NR>(acc, partials, b, gridDim.x);
 }
 
+// Warm start: r = b - A x0 with the starting values the caller left in x;
+// partials of (b, b), (r, r) and (r^, r) with r^ = b.
+template <int ND, int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_init_x0(Mat<double> A, BiVecs w, double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    double acc[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) acc[k] = 0.0;
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return w.x[k][j]; }, [&](int i, const double (&ax)[NR]) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const double bi = w.b[k][i], ri = bi - ax[k];
+            w.r[k][i] = ri;
+            acc[3 * k] += bi * bi;
+            acc[3 * k + 1] += ri * ri;
+            acc[3 * k + 2] += bi * ri;
+        }
+    });
+    block_partials<3 * NR>(acc, partials, bb, gridDim.x);
+}
+
+// The criterion stays ||r|| <= rtol ||b|| (PETSc's default test, as with a
+// zero start).  A guess whose residual is not below ||b|| is dropped: that
+// right-hand side restarts from x0 = 0 (k_bi_zero_start), so a warm start
+// never converges later than the zero start would.
+template <int NR>
+struct EpiBiInitX0 {
+    KState *ks;
+    double rtol;
+    int maxit;
+    __device__ void operator()(const double (&v)[3 * NR]) const
+    {
+        int all = 1;
+        for (int k = 0; k < NR; k++) {
+            const double bn = sqrt(v[3 * k]), rn = sqrt(v[3 * k + 1]);
+            const int zs = !(rn < bn);
+            ks->zero_start[k] = zs;
+            ks->bnorm[k] = bn; ks->tol[k] = rtol * bn; ks->rnorm[k] = zs ? bn : rn;
+            ks->rho_old[k] = 1.0; ks->alpha[k] = 1.0; ks->omega[k] = 1.0;
+            ks->rho[k] = zs ? v[3 * k] : v[3 * k + 2];   // (r^, r0)
+            ks->beta[k] = 0.0;
+            ks->iters[k] = 0; ks->half[k] = 0; ks->sn[k] = 0.0;
+            const double r0n = zs ? bn : rn;
+            ks->done[k] = (bn == 0.0 || r0n <= ks->tol[k]) ? 1 : 0;
+            ks->conv[k] = ks->done[k];
+            all &= ks->done[k];
+        }
+        ks->nr = NR; ks->maxit = maxit;
+        ks->all_done = all;
+    }
+};
+
+// x = 0, r = b for the right-hand sides whose guess was dropped (b = 0
+// included: its solution is zero)
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_zero_start(int n, BiVecs w, const KState *ks)
+{
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+        if (ks->zero_start[k] || ks->bnorm[k] == 0.0) { w.x[k][i] = 0.0; w.r[k][i] = w.b[k][i]; }
+}
+
 template <int NR>
 struct EpiBiInit {
     KState *ks;
@@ -544,7 +611,7 @@ static int predict_batch(double rn_prev, double rn, int its, double tol, int cap
 
 template <int NR>
 static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *const *x, double rtol, int maxit,
-                       mc_solve_status *st)
+                       mc_solve_status *st, bool warm)
 {
     static_assert(NR <= 4, "NR");
     const int n = c->N;
@@ -572,8 +639,15 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     const Grid g = grid_for(n);
-    hipLaunchKernelGGL(k_bi_init<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w[0], c->red, g.chunk);
-    launch_finish<NR>(c->stream, c->red, g.blocks, EpiBiInit<NR>{ks, rtol, maxit});
+    if (warm) {
+        MC_DISPATCH_ND(nd_code(A), (k_bi_init_x0<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w[0], c->red,
+                       g.chunk);
+        launch_finish<3 * NR>(c->stream, c->red, g.blocks, EpiBiInitX0<NR>{ks, rtol, maxit});
+        hipLaunchKernelGGL(k_bi_zero_start<NR>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n, w[0], ks);
+    } else {
+        hipLaunchKernelGGL(k_bi_init<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w[0], c->red, g.chunk);
+        launch_finish<NR>(c->stream, c->red, g.blocks, EpiBiInit<NR>{ks, rtol, maxit});
+    }
     MC_LAUNCH_CHECK(c);
     // A; r, p', v', r^ read and p, v written per RHS
     const double mbytes = mat_bytes(A, c->nnz, 8) + 48.0 * NR * n;
@@ -618,13 +692,13 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
 }
 
 int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x, double rtol, int maxit,
-                   mc_solve_status *st)
+                   mc_solve_status *st, bool warm)
 {
     switch (nr) {
-    case 1: return bicgstab_nr<1>(c, vals, b, x, rtol, maxit, st);
-    case 2: return bicgstab_nr<2>(c, vals, b, x, rtol, maxit, st);
-    case 3: return bicgstab_nr<3>(c, vals, b, x, rtol, maxit, st);
-    case 4: return bicgstab_nr<4>(c, vals, b, x, rtol, maxit, st);
+    case 1: return bicgstab_nr<1>(c, vals, b, x, rtol, maxit, st, warm);
+    case 2: return bicgstab_nr<2>(c, vals, b, x, rtol, maxit, st, warm);
+    case 3: return bicgstab_nr<3>(c, vals, b, x, rtol, maxit, st, warm);
+    case 4: return bicgstab_nr<4>(c, vals, b, x, rtol, maxit, st, warm);
     default: c->err = "bicgstab: 1..4 right-hand sides"; return MC_ERR_ARG;
     }
 }

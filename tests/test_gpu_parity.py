@@ -23,7 +23,9 @@ def make_pair(m, mode=po.MODE_COMPAT, **kw):
     opts = dict(TIGHT)
     opts.update(kw)
     o = po.Oracle(m, mode=mode, mom_rtol=opts["mom_rtol"], p_rtol=opts["p_rtol"], p_maxit=100000)
-    g = mc.Context(m, mode=mode, mom_rtol=opts["mom_rtol"], p_rtol=opts["p_rtol"], p_maxit=5000)
+    # warm_start=0: the oracle starts every BiCGStab from zero, as the
+    # reference does (FlowSolver.inl:1348)
+    g = mc.Context(m, mode=mode, mom_rtol=opts["mom_rtol"], p_rtol=opts["p_rtol"], p_maxit=5000, warm_start=0)
     return o, g
 
 
@@ -208,8 +210,10 @@ def test_timesteps_m16(mode):
     test_timesteps(16, mode)
 
 
-def test_scalar_batch_equals_single():
-    """The 4-RHS passive-scalar solve reproduces four single solves bit for bit."""
+@pytest.mark.parametrize("warm", [0, 1])
+def test_scalar_batch_equals_single(warm):
+    """The 4-RHS passive-scalar solve reproduces four single solves bit for bit
+    (from zero, and warm-started from the current fields)."""
     m = 8
     o = po.Oracle(m, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
     o.get_phi_gradients()
@@ -219,8 +223,8 @@ def test_scalar_batch_equals_single():
     o.scalar_solve(po.TERBTE, po.TE)
     o.scalar_solve(po.TERBED, po.ED)
     o.scalar_solve(po.TEMP, po.TEM)
-    g1 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
-    g2 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    g1 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12, warm_start=warm)
+    g2 = mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12, warm_start=warm)
     push_state(o, g1)
     push_state(o, g2)
     for t in [po.FUEL, po.PROG, po.VARFU, po.VARPR]:
@@ -298,7 +302,7 @@ def test_timesteps_limited(mode):
     m = 8
     print("limited timesteps m=%d mode=%d" % (m, mode), flush=True)
     o = po.Oracle(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=100000, limit_grad=1)
-    g = mc.Context(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, limit_grad=1)
+    g = mc.Context(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, limit_grad=1, warm_start=0)
     for ts in range(2):
         o.timestep()
         g.timestep()
@@ -458,3 +462,42 @@ def test_cell_visiting_order_does_not_change_results():
         for f in fields:
             assert_bits(fields[f], runs[0][0][f], "order %d field %s" % (k, po.FIELD_NAMES[f]))
         assert_bits(mflux, runs[0][1], "order %d mass flux" % k)
+
+
+@pytest.mark.parametrize("guess", ["current", "worse_than_zero"])
+def test_warm_start_momentum_solve(guess):
+    """warm_start = 1: the momentum BiCGStab starts from the current velocity
+    and still meets ||b - A x|| <= rtol ||b|| on every component; a guess whose
+    residual is not below ||b|| is dropped, and the solve is then the zero
+    start's, bit for bit."""
+    print("warm start guess=%s" % guess, flush=True)
+    m, rtol = 16, 1e-6
+    rng = np.random.default_rng(5)
+    out = {}
+    for warm in (0, 1):
+        with mc.Context(m, mode=mc.MODE_SIMPLE, mom_rtol=rtol, warm_start=0) as g:
+            for it in range(3):   # the same state for both starts
+                g.simple_iteration(it)
+            g.set_options(warm_start=warm)
+            g.get_phi_gradients()
+            g.momentum_assemble()
+            N = g.N
+            if guess == "worse_than_zero":
+                for f in (po.U, po.V, po.W):
+                    v = g.phi(f)
+                    v[:N] = 1e6 * rng.standard_normal(N)
+                    g.set(mc.ARR_PHI + f, v)
+            bs = [g.get(a)[:N].copy() for a in (mc.ARR_SU, mc.ARR_SV, mc.ARR_SW)]
+            st = g.momentum_solve()
+            assert st.converged
+            xs = [g.phi(f)[:N].copy() for f in (po.U, po.V, po.W)]
+            for b, x in zip(bs, xs):
+                res = np.linalg.norm(b - g.spmv(x, which=0)) / np.linalg.norm(b)
+                assert res <= 1.05 * rtol, (warm, res)
+            out[warm] = (xs, st.iterations)
+    if guess == "worse_than_zero":
+        assert out[1][1] == out[0][1]
+        for k in range(3):
+            assert_bits(out[1][0][k], out[0][0][k], "component %d, dropped guess vs zero start" % k)
+    else:
+        assert out[1][1] <= out[0][1], (out[1][1], out[0][1])
