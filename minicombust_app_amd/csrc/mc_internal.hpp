@@ -180,6 +180,8 @@ struct Ctx {
     // one Krylov iteration; the guess stays on while it pays for itself
     double lg_it_on = -1.0, lg_it_off = -1.0, lg_cost_ms = -1.0, lg_iter_ms = -1.0;
     int64_t lg_count = 0;
+    int64_t lg_last = 0;       // solve count of the last re-check of the other choice
+    int lg_period = 16;        // solves between re-checks (doubles while they agree)
     bool lg_on = true;
 
     // ---- device allocations owned by the context ----

@@ -2044,13 +2044,18 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     if (!c->amg_ready) MC_TRY(amg_setup(c, c->PA_vals));
     // learned = 2: every loop-1 solve starts from the guess; learned = 1: only
     // while its measured saving (iterations x time per iteration) exceeds its
-    // cost, re-checking the other choice every 16th solve
+    // cost, re-checking the other choice after 16 solves, then after twice as
+    // many each time a re-check confirms the choice (at most 1024)
+    bool recheck = false;
     if (c->opt.learned && c->mpnn && atol == 0.0) {
         bool want = true;
         if (c->opt.learned == 1) {
             if (c->lg_it_on < 0.0) want = true;
             else if (c->lg_it_off < 0.0) want = false;
-            else want = (c->lg_count % 16 == 15) ? !c->lg_on : c->lg_on;
+            else {
+                recheck = c->lg_count - c->lg_last >= c->lg_period;
+                want = recheck ? !c->lg_on : c->lg_on;
+            }
         }
         if (want) {
             MC_TRY(learned_guess(c, b, x, &used, &bn0));
@@ -2075,8 +2080,13 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
         } else {
             c->lg_it_off = ema(c->lg_it_off, s.iterations);
         }
+        const bool was_on = c->lg_on;
         if (c->lg_it_on >= 0.0 && c->lg_it_off >= 0.0)
             c->lg_on = (c->lg_it_off - c->lg_it_on) * c->lg_iter_ms > c->lg_cost_ms;
+        if (recheck) {
+            c->lg_last = c->lg_count;
+            c->lg_period = (c->lg_on == was_on) ? std::min(2 * c->lg_period, 1024) : 16;
+        }
         c->lg_count++;
     }
     c->st.p_guesses += used ? 1 : 0;
