@@ -234,12 +234,13 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks)
 
 // Cell order of the cell-centric kernels.  On the generated box mesh (cells
 // numbered x fastest, then y, then z) workgroup b visits a brick of
-// tx x 1 x tz cells (tx tz = 256), bricks ordered y fastest after the XCD
-// remap: a cell's neighbours at z +- 1 and the z faces it shares with them
-// then mostly lie in its own workgroup, and those at y +- 1 in the
-// workgroups running beside it on the same XCD, so a neighbour's values come
-// from L2 rather than from HBM a second time.  Other meshes: 256 consecutive
-// cells per workgroup.  Every cell is visited exactly once either way.
+// tx x 1 x tz cells (tx tz = 256; m = 128: one whole x row), the bricks after
+// the XCD remap in chunks of yc y rows, within a chunk z-slab by z-slab, row
+// by row, tile by tile: a cell's neighbours at x, y and z +- 1 lie in bricks
+// a few dozen workgroups apart on the same XCD, so a neighbour's values mostly
+// come from L2 rather than from HBM a second time (the chunk's two end rows
+// excepted).  Other meshes: 256 consecutive cells per workgroup.  Every cell
+// is visited exactly once either way.
 struct CellWalk {
     int N = 0;
     int nx = 0, ny = 0, nz = 0;   // nx == 0: consecutive cells
