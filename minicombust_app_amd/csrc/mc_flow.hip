@@ -343,6 +343,7 @@ struct MomArgs {
     double *U, *V, *W;
     const double *gUx, *gUy, *gUz, *gVx, *gVy, *gVz, *gWx, *gWy, *gWz, *gPx, *gPy, *gPz, *TEM;
     double *mflux, *ff0, *ff1, *AU, *SU, *SV, *SW, *vals, *dia;
+    double *r0[3];   // non-null: also b - A x0 of the three systems, x0 = the current velocity
     double ev, inlet_ev, rdelta, urf, gas_tem, vel_x, vel_y, vel_z;
 };
 
@@ -357,6 +358,7 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     const int N = g.N;
     const double GammaBlend = 0.0;
     double SU = 0.0, SV = 0.0, SW = 0.0, A = 0.0;
+    double xU = 0.0, xV = 0.0, xW = 0.0;   // off-diagonal part of A x0 (warm-started solve)
     double own_ff[6];
     int own_nb[6];
     int nint = 0;
@@ -421,6 +423,7 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
                 SW = SW + blend_w - fwde + fwdi;
                 own_ff[nint] = ff0;
             }
+            xU += own_ff[nint] * Un; xV += own_ff[nint] * Vn; xW += own_ff[nint] * Wn;
             own_nb[nint] = nb;
             nint++;
         } else {
@@ -495,6 +498,11 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     SW = SW + (1.0 - a.urf) * A * Wc;
     a.AU[c] = A;
     a.SU[c] = SU; a.SV[c] = SV; a.SW[c] = SW;
+    if (a.r0[0]) {
+        a.r0[0][c] = SU - (A * Uc + xU);
+        a.r0[1][c] = SV - (A * Vc + xV);
+        a.r0[2][c] = SW - (A * Wc + xW);
+    }
     store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
 
@@ -512,23 +520,41 @@ static MomArgs mom_args(Ctx *c)
     a.dia = (double *)fine_dia_of(c, c->A_vals);
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.5; a.gas_tem = c->k.gas_tem;
     a.vel_x = c->k.gas_vel_x; a.vel_y = c->k.gas_vel_y; a.vel_z = c->k.gas_vel_z;
+    a.r0[0] = a.r0[1] = a.r0[2] = nullptr;
     return a;
 }
 
-int momentum_assemble(Ctx *c)
+// Buffers for the starting residuals b - A x0 that the assembly kernels form
+// for warm-started solves (allocated on first use).
+static int r0_buffers(Ctx *c)
 {
+    if (c->r0buf[0]) return MC_OK;
+    MemScope ms(c, MC_MEM_WORK);
+    for (int k = 0; k < 3; k++) MC_TRY(dalloc(c, &c->r0buf[k], (size_t)c->N));
+    return MC_OK;
+}
+
+int momentum_assemble(Ctx *c, bool r0)
+{
+    MomArgs a = mom_args(c);
+    if (r0 && warm_start(c)) {
+        MC_TRY(r0_buffers(c));
+        for (int k = 0; k < 3; k++) a.r0[k] = c->r0buf[k];
+    }
     KTimer kt(c, MC_KC_MOM_ASM, 216.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
-    hipLaunchKernelGGL(k_momentum, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), mom_args(c));
+    hipLaunchKernelGGL(k_momentum, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
 }
 
-int momentum_solve(Ctx *c)
+int momentum_solve(Ctx *c, bool r0)
 {
     double *b[3] = {c->SU, c->SV, c->SW};
     double *x[3] = {c->phi[U], c->phi[V], c->phi[W]};
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
+    const bool warm = warm_start(c);
+    MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm,
+                          (r0 && warm) ? c->r0buf : nullptr));
     c->st.mom_iters += st.iterations;
     if (!st.converged) { c->err = "momentum BiCGStab did not converge"; return MC_ERR_NOCONV; }
     return MC_OK;
@@ -1028,6 +1054,7 @@ struct ScalArgs {
     double *TE, *ED, *TP;
     const double *mflux;
     double *ff0, *ff1, *AV, *SU, *vals, *dia;
+    double *r0;      // non-null: also b - A x0, x0 = the current field
     int type;
     double ev, inlet_ev, rdelta, urf, inlet_phi, cmu75;
 };
@@ -1063,6 +1090,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     // from memory and these for its own side
     const dv3 gc = ld3(a.gx, a.gy, a.gz, c), cc = ld3(g.cx, g.cy, g.cz, c);
     const double pc = a.phi[c];
+    double xo = 0.0;   // off-diagonal part of A x0 (warm-started solve)
     for (int k = 0; k < 6; k++) {
         const int f = fo[k];
         const bool own0 = ot[k] >= 0;            // c is the face's cell0
@@ -1100,6 +1128,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
                 S = S + blend - fde1 + fdi;
                 own_ff[nint] = ff0;
             }
+            xo += own_ff[nint] * pn;
             own_nb[nint] = nb;
             nint++;
         } else {
@@ -1204,6 +1233,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     S = S + (1.0 - a.urf) * A * phic;
     a.AV[c] = A;
     a.SU[c] = S;
+    if (a.r0) a.r0[c] = S - (A * pc + xo);   // x0 = phi[c], also where phic is ED's wall value
     store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
 
@@ -1438,6 +1468,9 @@ static int batch_solve(Ctx *c, bool tem)
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
     a.tphi = c->phi[TEM]; a.tgx = c->gx[TEM]; a.tgy = c->gy[TEM]; a.tgz = c->gz[TEM];
     a.tS = c->SU; a.tvals = c->A_vals2; a.tdia = (double *)fine_dia_of(c, c->A_vals2); a.tinlet = c->k.gas_tem;
+    // the warm-started solves form b - A x0 in their own pass here: in this
+    // kernel the extra accumulators cost more (168 VGPRs, +0.2 ms) than the pass
+    const bool warm = warm_start(c);
     {
         const int nsc = SB + (tem ? 1 : 0);
         KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * nsc * c->N + 56.0 * c->F + 8.0 * c->nnz * (tem ? 2 : 1));
@@ -1450,14 +1483,14 @@ static int batch_solve(Ctx *c, bool tem)
     if (tem) {
         double *bt[1] = {c->SU}, *xt[1] = {c->phi[TEM]};
         mc_solve_status stt{};
-        MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt, warm_start(c)));
+        MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt, warm));
         c->st.scalar_iters += stt.iterations;
         if (!stt.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     }
     double *b[SB], *x[SB];
     for (int s = 0; s < SB; s++) { b[s] = c->srhs[s]; x[s] = c->phi[fields[s]]; }
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
+    MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     // S_phi.U holds the last equation's right-hand side, as after VARP
@@ -1473,10 +1506,11 @@ static int scalar_field(int type)
     return f[type];
 }
 
-int scalar_assemble(Ctx *c, int type)
+int scalar_assemble(Ctx *c, int type, double *r0)
 {
     const int field = scalar_field(type);
     ScalArgs a;
+    a.r0 = r0;
     a.phi = c->phi[field];
     a.gx = c->gx[field]; a.gy = c->gy[field]; a.gz = c->gz[field];
     a.U = c->phi[U]; a.V = c->phi[V]; a.W = c->phi[W];
@@ -1624,11 +1658,18 @@ int clip_nonneg(Ctx *c, double *phi, int slot)
 int scalar_solve(Ctx *c, int type)
 {
     const int field = scalar_field(type);
-    MC_TRY(scalar_assemble(c, type));
+    const bool warm = warm_start(c);
+    // TE and ED: the assembly forms b - A x0 for the warm start; the scalars
+    // the batched kernel assembles take the solver's own pass, as there, so a
+    // single solve and the batch give the same bits
+    const bool fused = warm && (type == TERBTE || type == TERBED);
+    if (fused) MC_TRY(r0_buffers(c));
+    MC_TRY(scalar_assemble(c, type, fused ? c->r0buf[0] : nullptr));
     double *b[1] = {c->SU};
     double *x[1] = {c->phi[field]};
     mc_solve_status st{};
-    MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm_start(c)));
+    MC_TRY(bicgstab_solve(c, c->A_vals, 1, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm,
+                          fused ? c->r0buf : nullptr));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     if (type == TERBTE || type == TERBED) MC_TRY(clip_nonneg(c, c->phi[field], type == TERBTE ? 0 : 1));
@@ -1644,8 +1685,8 @@ int simple_iteration(Ctx *c, int inner)
         MC_TRY(set_up_field(c));
         c->field_set_up = true;
     }
-    MC_TRY(momentum_assemble(c));
-    MC_TRY(momentum_solve(c));
+    MC_TRY(momentum_assemble(c, true));
+    MC_TRY(momentum_solve(c, true));
     MC_TRY(calculate_pressure(c));
     if (c->opt.solve_scalars) {
         MC_TRY(scalar_solve(c, TERBTE));

@@ -149,6 +149,7 @@ struct Ctx {
 
     // ---- solver workspace ----
     double *srhs[4] = {};          // right-hand sides of the batched scalar solve
+    double *r0buf[3] = {};         // b - A x0 formed by an assembly for the warm-started solve after it
     int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
     void *cub_tmp = nullptr;       // hipcub scratch for the compaction
     size_t cub_tmp_bytes = 0;
@@ -431,11 +432,13 @@ int limit_gradients(Ctx *c);                             // limiter alone, 11 fi
 int clip_nonneg(Ctx *c, double *phi, int slot);          // TE / ED repair (slot 0 / 1: pass history)
 int grad_single(Ctx *c, int field, bool pressure);
 int set_up_field(Ctx *c);
-int momentum_assemble(Ctx *c);
-int momentum_solve(Ctx *c);
+// r0: the assembly also forms b - A x0 for the warm-started solve that
+// follows it directly (Ctx::r0buf; the solve must be passed the same flag)
+int momentum_assemble(Ctx *c, bool r0 = false);
+int momentum_solve(Ctx *c, bool r0 = false);
 int calculate_mass_flux_assemble(Ctx *c);               // mass flux + pressure matrix
 int calculate_pressure(Ctx *c);
-int scalar_assemble(Ctx *c, int type);
+int scalar_assemble(Ctx *c, int type, double *r0 = nullptr);
 int scalar_solve(Ctx *c, int type);
 int scalar_batch_solve(Ctx *c);
 int flow_workspace(Ctx *c);
@@ -453,7 +456,7 @@ int dia_to_csr(Ctx *c, const double *dia, double *csr);   // array access in CSR
 int csr_to_dia(Ctx *c, const double *csr, double *dia);
 // warm: start from the values in x (else x0 = 0)
 int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x,
-                   double rtol, int maxit, mc_solve_status *st, bool warm);
+                   double rtol, int maxit, mc_solve_status *st, bool warm, double *const *r0 = nullptr);
 // the momentum and scalar solves start from the current field
 inline bool warm_start(const Ctx *c)
 {

@@ -364,6 +364,27 @@ __global__ void __launch_bounds__(BLOCK) k_bi_init_x0(Mat<double> A, BiVecs w, d
     block_partials<3 * NR>(acc, partials, bb, gridDim.x);
 }
 
+// The same partials when the assembly already formed r = b - A x0.
+template <int NR>
+__global__ void __launch_bounds__(BLOCK) k_bi_init_r0(int n, BiVecs w, double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[3 * NR];
+#pragma unroll
+    for (int k = 0; k < 3 * NR; k++) acc[k] = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const double bi = w.b[k][i], ri = w.r[k][i];
+            acc[3 * k] += bi * bi;
+            acc[3 * k + 1] += ri * ri;
+            acc[3 * k + 2] += bi * ri;
+        }
+    }
+    block_partials<3 * NR>(acc, partials, bb, gridDim.x);
+}
+
 // The criterion stays ||r|| <= rtol ||b|| (PETSc's default test, as with a
 // zero start).  A guess whose residual is not below ||b|| is dropped: that
 // right-hand side restarts from x0 = 0 (k_bi_zero_start), so a warm start
@@ -611,7 +632,7 @@ static int predict_batch(double rn_prev, double rn, int its, double tol, int cap
 
 template <int NR>
 static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *const *x, double rtol, int maxit,
-                       mc_solve_status *st, bool warm)
+                       mc_solve_status *st, bool warm, double *const *r0)
 {
     static_assert(NR <= 4, "NR");
     const int n = c->N;
@@ -639,7 +660,13 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
     const Grid g = grid_for(n);
-    if (warm) {
+    if (warm && r0) {
+        // the assembly formed r = b - A x0 (same x0, same matrix): it is the r vector
+        for (int k = 0; k < NR; k++) w[0].r[k] = w[1].r[k] = r0[k];
+        hipLaunchKernelGGL(k_bi_init_r0<NR>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w[0], c->red, g.chunk);
+        launch_finish<3 * NR>(c->stream, c->red, g.blocks, EpiBiInitX0<NR>{ks, rtol, maxit});
+        hipLaunchKernelGGL(k_bi_zero_start<NR>, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, c->stream, n, w[0], ks);
+    } else if (warm) {
         MC_DISPATCH_ND(nd_code(A), (k_bi_init_x0<ND_, NR>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w[0], c->red,
                        g.chunk);
         launch_finish<3 * NR>(c->stream, c->red, g.blocks, EpiBiInitX0<NR>{ks, rtol, maxit});
@@ -692,13 +719,13 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
 }
 
 int bicgstab_solve(Ctx *c, const double *vals, int nr, double *const *b, double *const *x, double rtol, int maxit,
-                   mc_solve_status *st, bool warm)
+                   mc_solve_status *st, bool warm, double *const *r0)
 {
     switch (nr) {
-    case 1: return bicgstab_nr<1>(c, vals, b, x, rtol, maxit, st, warm);
-    case 2: return bicgstab_nr<2>(c, vals, b, x, rtol, maxit, st, warm);
-    case 3: return bicgstab_nr<3>(c, vals, b, x, rtol, maxit, st, warm);
-    case 4: return bicgstab_nr<4>(c, vals, b, x, rtol, maxit, st, warm);
+    case 1: return bicgstab_nr<1>(c, vals, b, x, rtol, maxit, st, warm, r0);
+    case 2: return bicgstab_nr<2>(c, vals, b, x, rtol, maxit, st, warm, r0);
+    case 3: return bicgstab_nr<3>(c, vals, b, x, rtol, maxit, st, warm, r0);
+    case 4: return bicgstab_nr<4>(c, vals, b, x, rtol, maxit, st, warm, r0);
     default: c->err = "bicgstab: 1..4 right-hand sides"; return MC_ERR_ARG;
     }
 }
