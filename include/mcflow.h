@@ -99,6 +99,10 @@ typedef struct mc_options {
                               (a guess no better than zero is dropped for x0 = 0);
                               0: always x0 = 0 (VecZeroEntries, FlowSolver.inl:1348);
                               -1: on in MC_MODE_SIMPLE, off in MC_MODE_COMPAT    */
+    int    amg_half;       /* 1: the mixed form's V-cycle smooths the fine level of a
+                              7-diagonal pressure pattern with an FP16 copy of
+                              D^-1 A's off-diagonals (the PCG product and the FP64
+                              refinement residual keep the full matrix); 0: FP32 */
     int    amg_tail;       /* 1: the V-cycle from the first level of at most 2048
                               rows down runs in one workgroup (same results, fewer
                               launches); 0: one launch per pass                  */

@@ -168,6 +168,7 @@ int mc_options_default(mc_options *o)
     o->amg_refresh = 4;   // m = 128: same PCG iteration counts as 1, pressure solve 5% faster (measured)
     o->cell_ychunk = 0;
     o->warm_start = -1;
+    o->amg_half = 1;
     o->amg_tail = 1;
     return MC_OK;
 }
@@ -192,6 +193,7 @@ static int validate(const mc_options *o, std::string &err)
     if (o->amg_tail != 0 && o->amg_tail != 1) { err = "amg_tail must be 0 or 1"; return MC_ERR_ARG; }
     if (o->cell_ychunk < 0 || o->cell_ychunk > 4096) { err = "cell_ychunk must be in [0, 4096]"; return MC_ERR_ARG; }
     if (o->warm_start < -1 || o->warm_start > 1) { err = "warm_start must be -1, 0 or 1"; return MC_ERR_ARG; }
+    if (o->amg_half != 0 && o->amg_half != 1) { err = "amg_half must be 0 or 1"; return MC_ERR_ARG; }
     return MC_OK;
 }
 

@@ -659,6 +659,8 @@ struct MassArgs {
     double *pdia, *pdinv;
     float *pdia32, *pdinv32;
     int pskip;
+    _Float16 *phoff;   // with pdia: the FP16 copy of D^-1 A's off-diagonals
+    float *pdiag32;
 };
 
 // The per-cell values the Rhie-Chow flux reads on either side of a face.
@@ -799,6 +801,13 @@ __global__ void __launch_bounds__(256) k_mass_rows(Geo g, MassArgs a)
             const double di = (AV != 0.0) ? 1.0 / AV : 0.0;
             a.pdinv[c] = di;
             a.pdinv32[c] = (float)di;
+            if (a.phoff) {   // as k_level_dia writes them
+                a.pdiag32[c] = (float)AV;
+                for (int k = 0; k < nint; k++) {
+                    const int d = dslot(g, own_nb[k] - c);
+                    a.phoff[(size_t)(d < d0 ? d : d - 1) * n + c] = (_Float16)(own_v[k] / AV);
+                }
+            }
         }
     }
 }
@@ -822,6 +831,7 @@ static MassArgs mass_args(Ctx *c, int assemble, int into_pressure)
     a.PAvals = c->PA_vals;
     a.assemble = assemble; a.into_pressure = into_pressure;
     a.pdia = nullptr; a.pdinv = nullptr; a.pdia32 = nullptr; a.pdinv32 = nullptr; a.pskip = 0;
+    a.phoff = nullptr; a.pdiag32 = nullptr;
     return a;
 }
 
@@ -850,6 +860,7 @@ int calculate_mass_flux_assemble(Ctx *c)
         if (l0) {
             a.pdia = L.dia; a.pdia32 = L.dia32; a.pdinv = L.dinv; a.pdinv32 = L.dinv32;
             a.pskip = L.sym ? L.nd / 2 : 0;
+            a.phoff = L.hoff; a.pdiag32 = L.diag32;
         }
     }
     {

@@ -58,6 +58,11 @@ struct AmgLevel {
     float *vals32 = nullptr;                  // FP32 copy (mixed form)
     double *dinv = nullptr;                   // 1/diag
     float *dinv32 = nullptr;
+    // fine level of a 7-diagonal pattern: the off-diagonals of D^-1 A in FP16
+    // (6 arrays, main diagonal left out) and the FP32 diagonal, for the FP32
+    // V-cycle's smoothing sweeps and residual
+    _Float16 *hoff = nullptr;
+    float *diag32 = nullptr;
     // aggregation to the next level
     int nc = 0;
     int *agg = nullptr;                       // [n] aggregate id

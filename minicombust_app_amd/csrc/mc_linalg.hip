@@ -915,7 +915,7 @@ __global__ void k_diag_inv(int n, const int *rp, const int *col, const double *v
 // A level with diagonal storage: one pass over its CSR rows fills the FP64 and
 // FP32 diagonal copies and the inverse diagonal.
 __global__ void k_level_dia(int n, const int *rp, const int *col, const int *map, const double *v, double *dia,
-                            float *dia32, double *dinv, float *dinv32, int sym_skip)
+                            float *dia32, double *dinv, float *dinv32, int sym_skip, _Float16 *hoff, float *diag32)
 {
     // sym_skip > 0: the FP64 copy is symmetric storage without the sym_skip
     // lower diagonals; the FP32 copy keeps all of them
@@ -933,6 +933,14 @@ __global__ void k_level_dia(int n, const int *rp, const int *col, const int *map
     const double di = (d != 0.0) ? 1.0 / d : 0.0;
     dinv[i] = di;
     dinv32[i] = (float)di;
+    if (hoff) {   // 7 diagonals, the main one in the middle
+        diag32[i] = (float)d;
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            if (col[k] == i) continue;
+            const int dd = (map[k] - i) / n;
+            hoff[(size_t)(dd < 3 ? dd : dd - 1) * n + i] = (_Float16)(v[k] / d);
+        }
+    }
 }
 
 // Dense regularised coarsest operator A + alpha 11^T (alpha = mean diag / n):
@@ -1061,6 +1069,11 @@ int amg_setup(Ctx *c, const double *vals)
                 MC_TRY(dalloc(c, &L.dia32, (size_t)L.nd * L.n));
                 MC_HIP(c, hipMemset(L.dia, 0, (size_t)ns * L.n * 8));
                 MC_HIP(c, hipMemset(L.dia32, 0, (size_t)L.nd * L.n * 4));
+                if (first && L.nd == 7 && L.sym && c->opt.amg_half) {
+                    MC_TRY(dalloc(c, &L.hoff, (size_t)6 * L.n));
+                    MC_TRY(dalloc(c, &L.diag32, (size_t)L.n));
+                    MC_HIP(c, hipMemset(L.hoff, 0, (size_t)6 * L.n * sizeof(_Float16)));
+                }
             }
             if (!L.nd) MC_TRY(dalloc(c, &L.vals32, L.nnz));   // FP32 CSR only where no diagonal copy
         }
@@ -1148,7 +1161,8 @@ int amg_update_values(Ctx *c, const double *vals)
         if (!(l == 0 && l0_done)) {
             if (L.nd)
                 hipLaunchKernelGGL(k_level_dia, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr,
-                                   L.col, L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32, L.sym ? L.nd / 2 : 0);
+                                   L.col, L.dia_map, L.vals, L.dia, L.dia32, L.dinv, L.dinv32, L.sym ? L.nd / 2 : 0,
+                                   L.hoff, L.diag32);
             else
                 hipLaunchKernelGGL(k_diag_inv, dim3(div_up(L.n, 256)), dim3(256), 0, c->stream, L.n, L.row_ptr,
                                    L.col, L.vals, L.dinv, L.dinv32, L.vals32, L.nnz);
@@ -1559,6 +1573,78 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vtail(TailArgs<T> a, const KSt
     }
 }
 
+// ----------------------------------------- FP16 fine-level smoothing --
+// The FP32 V-cycle's fine-level passes on an FP16 copy of D^-1 A's six
+// off-diagonals (entries in [-1, 0] for the pressure M-matrix) with the FP32
+// inverse diagonal and diagonal: x' = x + w (D^-1 b - x - (D^-1 A)_off x),
+// r = D (D^-1 b - x - (D^-1 A)_off x).  A row reads 12 bytes of matrix
+// instead of 28.  The PCG's own product keeps the FP32 matrix, and the FP64
+// refinement residual checks every step, so the rounding of the copy (5e-4
+// relative) changes only the preconditioner.
+struct HMat {
+    int n;
+    const _Float16 *h;
+    const float *dinv, *diag;
+    int off[6];
+};
+
+template <typename XF>
+__device__ __forceinline__ float hmat_offx(const HMat &H, int i, XF X)
+{
+    float a[6];
+    int j[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        a[d] = (float)H.h[(size_t)d * H.n + i];
+        j[d] = clampi(i + H.off[d], H.n);
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 6; d++) acc += a[d] * X(j[d]);
+    return acc;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_jac_first2_h(HMat H, const float *b, float *xout, float w1, float w2,
+                                                        int chunk, int two, const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(H.n, r0 + chunk);
+    for (int i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float db = H.dinv[i] * b[i];
+        const float x1 = w1 * db;
+        if (!two) { xout[i] = x1; continue; }
+        const float ax = hmat_offx(H, i, [&](int j) { return w1 * (H.dinv[j] * b[j]); });
+        xout[i] = x1 + w2 * (db - x1 - ax);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_jac_h(HMat H, const float *b, const float *x, float *xout, float omega,
+                                                 int chunk, const float *dotv, double *partials, const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(H.n, r0 + chunk);
+    double acc[1] = {0.0};
+    for (int i = r0 + threadIdx.x; i < r1; i += BLOCK) {
+        const float y = x[i];
+        const float o = y + omega * (H.dinv[i] * b[i] - y - hmat_offx(H, i, [&](int j) { return x[j]; }));
+        xout[i] = o;
+        if (dotv) acc[0] += (double)dotv[i] * (double)o;
+    }
+    if (dotv) block_partials<1>(acc, partials, bb, gridDim.x);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_resid_h(HMat H, const float *b, const float *x, float *r, int chunk,
+                                                   const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(H.n, r0 + chunk);
+    for (int i = r0 + threadIdx.x; i < r1; i += BLOCK)
+        r[i] = H.diag[i] * (H.dinv[i] * b[i] - x[i] - hmat_offx(H, i, [&](int j) { return x[j]; }));
+}
+
 // V-cycle on level l: x_l = M^-1 b_l.  On level 0 with ks != nullptr the
 // last sweep also reduces (b, x) into the PCG's rho / beta.
 template <typename T>
@@ -1614,20 +1700,44 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     const Mat<T> A = level_mat<T>(L);
     T *dinv = lv_dinv<T>(L), *tmp = lv_t<T>(L);
     T *cur = x, *oth = tmp;
-    MC_DISPATCH_ND(nd_code(A), (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, (T)wts[0],
-                   (T)wts[nu >= 2 ? 1 : 0],
-                   g.chunk, nu >= 2 ? 1 : 0, stop);
+    // FP32 cycle on a level with the FP16 copy: its passes read that copy
+    const bool half = sizeof(T) == 4 && L.hoff != nullptr;
+    HMat H{};
+    if (half) {
+        H.n = L.n; H.h = L.hoff; H.dinv = L.dinv32; H.diag = L.diag32;
+        for (int d = 0, q = 0; d < L.nd; d++) if (L.off[d] != 0) H.off[q++] = L.off[d];
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (half)
+            hipLaunchKernelGGL(k_jac_first2_h, dim3(g.blocks), dim3(BLOCK), 0, c->stream, H, b, cur, (float)wts[0],
+                               (float)wts[nu >= 2 ? 1 : 0], g.chunk, nu >= 2 ? 1 : 0, stop);
+    }
+    if (!half)
+        MC_DISPATCH_ND(nd_code(A), (k_jac_first2<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+                       (T)wts[0], (T)wts[nu >= 2 ? 1 : 0], g.chunk, nu >= 2 ? 1 : 0, stop);
     for (int s = 2; s < nu; s++) {
-        MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur, oth,
-                       (T)wts[s], g.chunk, (const T *)nullptr, (double *)nullptr, stop);
+        if constexpr (sizeof(T) == 4) {
+            if (half)
+                hipLaunchKernelGGL(k_jac_h, dim3(g.blocks), dim3(BLOCK), 0, c->stream, H, b, (const float *)cur, oth,
+                                   (float)wts[s], g.chunk, (const float *)nullptr, (double *)nullptr, stop);
+        }
+        if (!half)
+            MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
+                           oth, (T)wts[s], g.chunk, (const T *)nullptr, (double *)nullptr, stop);
         std::swap(cur, oth);
     }
     AmgLevel &Lc = c->amg[l + 1];
     T *bc = lv_b<T>(Lc), *xc = lv_x<T>(Lc);
     if (A.nd) {
         T *r = (cur == x) ? tmp : x;   // free until post-smoothing
-        MC_DISPATCH_ND(nd_code(A), (k_resid<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, (const T *)cur, r,
-                       g.chunk, stop);
+        if constexpr (sizeof(T) == 4) {
+            if (half)
+                hipLaunchKernelGGL(k_resid_h, dim3(g.blocks), dim3(BLOCK), 0, c->stream, H, b, (const float *)cur, r,
+                                   g.chunk, stop);
+        }
+        if (!half)
+            MC_DISPATCH_ND(nd_code(A), (k_resid<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b,
+                           (const T *)cur, r, g.chunk, stop);
         hipLaunchKernelGGL(k_restrict<T>, dim3(div_up(L.nc, BLOCK)), dim3(BLOCK), 0, c->stream, L.nc, L.agg_ptr,
                            L.agg_members, (const T *)r, bc, stop);
     } else {
@@ -1648,8 +1758,14 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         const double jbytes = (l == 0) ? mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n : 0.0;
         {
             KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
-            MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b, cur,
-                           dst, (T)wts[nu - 1 - s], g.chunk, dotv, c->red, stop);
+            if constexpr (sizeof(T) == 4) {
+                if (half)
+                    hipLaunchKernelGGL(k_jac_h, dim3(g.blocks), dim3(BLOCK), 0, c->stream, H, b, (const float *)cur,
+                                       dst, (float)wts[nu - 1 - s], g.chunk, dotv, c->red, stop);
+            }
+            if (!half)
+                MC_DISPATCH_ND(nd_code(A), (k_jac<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b,
+                               cur, dst, (T)wts[nu - 1 - s], g.chunk, dotv, c->red, stop);
         }
         if (dotv) launch_finish<1>(c->stream, c->red, g.blocks, EpiRz{ks});
         oth = cur;

@@ -527,3 +527,30 @@ def test_warm_start_te_solve_meets_criterion():
     # the TE repair after the solve only lifts negative values, none here
     assert np.all(out[1] > 0)
     assert_close(out[1], out[0], "TE warm vs zero start", 1e-5)
+
+
+def test_fp16_smoothing_copy_keeps_the_tolerance():
+    """amg_half = 1: the mixed form's fine-level smoothing reads an FP16 copy of
+    D^-1 A.  Only the preconditioner changes: the solve still reaches 1e-8 on
+    the assembled system (independent FP64 product) and agrees with the FP32
+    smoothing's solution to the accuracy both reach."""
+    print("fp16 smoothing m=32", flush=True)
+    m = 32
+    o = po.Oracle(m, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    o.get_phi_gradients()
+    o.set_up_field()
+    o.calculate_UVW()
+    oracle_pressure_assemble(o)
+    b = o.cell("S_U").copy()
+    xs = {}
+    for half in (0, 1):
+        with mc.Context(m, mode=mc.MODE_SIMPLE, psolve=mc.PSOLVE_MIXED, p_rtol=1e-8, amg_half=half) as g:
+            push_state(o, g)
+            x, st = g.solve_pressure(b)
+            assert st.converged
+            bp = b - b.mean()
+            res = np.linalg.norm(bp - g.spmv(x, which=1)) / np.linalg.norm(bp)
+            assert res <= 1.05e-8, (half, res)
+            xs[half] = x
+    assert_close(xs[1], xs[0], "pressure correction, FP16 vs FP32 smoothing", 1e-6)
+    o.close()
