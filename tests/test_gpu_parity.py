@@ -554,3 +554,40 @@ def test_fp16_smoothing_copy_keeps_the_tolerance():
             xs[half] = x
     assert_close(xs[1], xs[0], "pressure correction, FP16 vs FP32 smoothing", 1e-6)
     o.close()
+
+
+FIELDS_ALL = [po.U, po.V, po.W, po.P, po.TE, po.ED, po.TEM, po.FUL, po.PRO, po.VARF, po.VARP]
+
+
+@pytest.mark.parametrize("mode", [po.MODE_COMPAT, po.MODE_SIMPLE])
+def test_reference_cpu_case_m20_ten_timesteps(mode):
+    """The reference's own CPU case: modifier 20 (16,000 cells), default
+    boundary conditions and initial fields, 10 timesteps (30 SIMPLE
+    iterations).  Every field and the mass flux agree with the oracle to 1e-8
+    of the field's maximum after each timestep (both sides solve to 1e-12)."""
+    print("m=20 x 10 timesteps mode=%d" % mode, flush=True)
+    o, g = make_pair(20, mode)
+    for ts in range(10):
+        o.timestep()
+        g.timestep()
+        for f in FIELDS_ALL:
+            assert_close(g.phi(f), o.phi(f), "timestep %d field %s" % (ts, po.FIELD_NAMES[f]), 1e-8)
+        assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "timestep %d mass flux" % ts, 1e-8)
+    o.close()
+    g.close()
+
+
+def test_fp64_gpu_case_m64_against_oracle():
+    """The FP64 GPU case's mesh, modifier 64 (524,288 cells), checked against
+    the oracle's method on the same mesh for one timestep (3 SIMPLE
+    iterations, simple mode): fields and mass flux to 1e-8."""
+    print("m=64 x 3 iterations", flush=True)
+    o, g = make_pair(64, po.MODE_SIMPLE)
+    for it in range(3):
+        o.simple_iteration(it)
+        g.simple_iteration(it)
+    for f in FIELDS_ALL:
+        assert_close(g.phi(f), o.phi(f), "field %s" % po.FIELD_NAMES[f], 1e-8)
+    assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "mass flux", 1e-8)
+    o.close()
+    g.close()
