@@ -198,6 +198,113 @@ __global__ void __launch_bounds__(256) k_mm(int n, const float *X1, int ld1, con
     }
 }
 
+// Inference: the node update of one layer in one pass,
+//   a = s W2 + deg b2,  u = relu([h, a] Wn1 + bn1),  h' = h + u Wn2 + bn2,
+// a wave holding its 16 rows' a and u in LDS between the products instead of
+// writing and re-reading them (the training pass keeps them for backward).
+// Fragments, operand rounding and epilogues are k_mm's, in the same order,
+// so h' is the same bits as the three k_mm launches give.
+__global__ void __launch_bounds__(256) k_node_fused(int n, const float *S, const float *H, const float *deg,
+                                                   const bf16_t *W2t, const float *b2, const bf16_t *N1t,
+                                                   const float *bn1, const bf16_t *N2t, const float *bn2, float *Hout)
+{
+    __shared__ float tA[4][16][MH + 4];
+    __shared__ float tU[4][16][MH + 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // every wave reaches the barriers; rows past n compute on zeros and store nothing
+    const int row0 = (blockIdx.x * 4 + wave) * 16;
+    const int ar = row0 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+    auto frag = [&](float4 xv) {
+        short4v a;
+        a[0] = (short)to_bf16(xv.x); a[1] = (short)to_bf16(xv.y);
+        a[2] = (short)to_bf16(xv.z); a[3] = (short)to_bf16(xv.w);
+        return a;
+    };
+    float4v acc[MH / 16];
+    // a = s W2 + deg b2
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < MH / 16; kk++) {
+        const int k = kk * 16 + kq;
+        const float4 xv = (ar < n) ? *(const float4 *)(S + (size_t)ar * MH + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const short4v a = frag(xv);
+#pragma unroll
+        for (int j = 0; j < MH / 16; j++) {
+            const short4v b = *(const short4v *)(W2t + (size_t)(j * 16 + (lane & 15)) * MH + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) {
+        const int col = j * 16 + (lane & 15);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int row = row0 + kq + t;
+            float v = acc[j][t];
+            v += b2[col] * (row < n ? deg[row] : 0.f);
+            tA[wave][kq + t][col] = v;
+        }
+    }
+    __syncthreads();
+    // u = relu([h, a] Wn1 + bn1)
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 2 * MH / 16; kk++) {
+        const int k = kk * 16 + kq;
+        float4 xv;
+        if (k < MH) xv = (ar < n) ? *(const float4 *)(H + (size_t)ar * MH + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+        else xv = (ar < n) ? *(const float4 *)&tA[wave][lane & 15][k - MH] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const short4v a = frag(xv);
+#pragma unroll
+        for (int j = 0; j < MH / 16; j++) {
+            const short4v b = *(const short4v *)(N1t + (size_t)(j * 16 + (lane & 15)) * (2 * MH) + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) {
+        const int col = j * 16 + (lane & 15);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float v = acc[j][t];
+            v += bn1[col];
+            v = fmaxf(v, 0.f);
+            tU[wave][kq + t][col] = v;
+        }
+    }
+    __syncthreads();
+    // h' = h + u Wn2 + bn2
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < MH / 16; kk++) {
+        const int k = kk * 16 + kq;
+        const float4 xv = (ar < n) ? *(const float4 *)&tU[wave][lane & 15][k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const short4v a = frag(xv);
+#pragma unroll
+        for (int j = 0; j < MH / 16; j++) {
+            const short4v b = *(const short4v *)(N2t + (size_t)(j * 16 + (lane & 15)) * MH + k);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MH / 16; j++) {
+        const int col = j * 16 + (lane & 15);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int row = row0 + kq + t;
+            if (row >= n) continue;
+            float v = acc[j][t];
+            v = H[(size_t)row * MH + col] + v;
+            v += bn2[col];
+            Hout[(size_t)row * MH + col] = v;
+        }
+    }
+}
+
 // Block partial of G = [X1 | X2]^T D (KP x MM) over rows [b*TN_ROWS, ...),
 // plus row KP = column sums of D (weighted by rowscale when given):
 // partial[b][(KP + 1) * MM].  MFMA with the row index as the reduction
@@ -375,13 +482,24 @@ __global__ void k_edge_reduce(const float *partial, int nblocks, int off_we, int
 }
 
 // ------------------------------------------------- decoder and loss --
-__global__ void k_decode(int n, const float *Hf, const float *w, float *y)
+// y = h wdec + bdec for 64 rows per workgroup: the rows are staged through LDS
+// with coalesced loads, then each row is summed in column order
+__global__ void __launch_bounds__(256) k_decode(int n, const float *Hf, const float *w, float *y)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ float tile[64][MH + 1];
+    const int r0 = blockIdx.x * 64;
+    for (int q = threadIdx.x; q < 64 * (MH / 4); q += 256) {
+        const int r = q / (MH / 4), c4 = 4 * (q % (MH / 4));
+        const float4 v = (r0 + r < n) ? *(const float4 *)(Hf + (size_t)(r0 + r) * MH + c4)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+        tile[r][c4] = v.x; tile[r][c4 + 1] = v.y; tile[r][c4 + 2] = v.z; tile[r][c4 + 3] = v.w;
+    }
+    __syncthreads();
+    const int i = r0 + (int)threadIdx.x;
+    if (threadIdx.x >= 64 || i >= n) return;
     float s = 0.f;
     for (int c = 0; c < MH; c++)
-        s += from_bf16(to_bf16(Hf[(size_t)i * MH + c])) * from_bf16(to_bf16(w[OFF_WDEC + c]));
+        s += from_bf16(to_bf16(tile[threadIdx.x][c])) * from_bf16(to_bf16(w[OFF_WDEC + c]));
     y[i] = s + w[OFF_BDEC];
 }
 
@@ -686,6 +804,11 @@ static void forward(Ctx *c, const float *f, const float *e, bool train)
                            (const float *)nullptr, 0, PQ, 2 * MH);
         hipLaunchKernelGGL(k_edge_fwd, dim3(div_up((int64_t)n * 16, 256)), blk, 0, c->stream, n, PQ, m->nbr, e,
                            Wl + L_WE, Wl + L_B1, S);
+        if (!train) {
+            hipLaunchKernelGGL(k_node_fused, gmm, blk, 0, c->stream, n, (const float *)S, (const float *)hin, m->deg,
+                               Pk + P_W2T, Wl + L_B2, Pk + P_N1T, Wl + L_BN1, Pk + P_N2T, Wl + L_BN2, hout);
+            continue;
+        }
         hipLaunchKernelGGL((k_mm<MH, 0, MH, MH, EPI_BIAS | EPI_ROWSCALE>), gmm, blk, 0, c->stream, n, S, MH,
                            (const float *)nullptr, 0, Pk + P_W2T, Wl + L_B2, m->deg, (const float *)nullptr, 0,
                            (const float *)nullptr, 0, A, MH);
@@ -697,7 +820,7 @@ static void forward(Ctx *c, const float *f, const float *e, bool train)
                            (const float *)nullptr, 0, hout, MH);
     }
     const float *hl = train ? m->H[ML] : m->H[ML & 1];
-    hipLaunchKernelGGL(k_decode, dim3(div_up(n, 256)), blk, 0, c->stream, n, hl, W, m->y);
+    hipLaunchKernelGGL(k_decode, dim3(div_up(n, 64)), blk, 0, c->stream, n, hl, W, m->y);
 }
 
 // MFMA work of one forward pass (flops)
