@@ -202,8 +202,7 @@ __global__ void __launch_bounds__(256) k_mm(int n, const float *X1, int ld1, con
 //   a = s W2 + deg b2,  u = relu([h, a] Wn1 + bn1),  h' = h + u Wn2 + bn2,
 // a wave holding its 16 rows' a and u in LDS between the products instead of
 // writing and re-reading them (the training pass keeps them for backward).
-// Fragments, operand rounding and epilogues are k_mm's, in the same order,
-// so h' is the same bits as the three k_mm launches give.
+// Fragments, operand rounding and epilogues are k_mm's, in the same order.
 __global__ void __launch_bounds__(256) k_node_fused(int n, const float *S, const float *H, const float *deg,
                                                    const bf16_t *W2t, const float *b2, const bf16_t *N1t,
                                                    const float *bn1, const bf16_t *N2t, const float *bn2, float *Hout)
