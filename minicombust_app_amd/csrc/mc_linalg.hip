@@ -1701,7 +1701,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     T *dinv = lv_dinv<T>(L), *tmp = lv_t<T>(L);
     T *cur = x, *oth = tmp;
     // FP32 cycle on a level with the FP16 copy: its passes read that copy
-    const bool half = sizeof(T) == 4 && L.hoff != nullptr;
+    const bool half = sizeof(T) == 4 && L.hoff != nullptr && c->opt.amg_half;
     HMat H{};
     if (half) {
         H.n = L.n; H.h = L.hoff; H.dinv = L.dinv32; H.diag = L.diag32;
