@@ -95,10 +95,11 @@ typedef struct mc_options {
     int    cell_ychunk;    /* generated box mesh: y rows per chunk of the bricks'
                               visiting order (chunks in y; within one, z-slab by
                               z-slab, row by row, tile by tile), 0: 16          */
-    int    warm_start;     /* momentum and scalar BiCGStab start from the current field
-                              (a guess no better than zero is dropped for x0 = 0);
-                              0: always x0 = 0 (VecZeroEntries, FlowSolver.inl:1348);
-                              -1: on in MC_MODE_SIMPLE, off in MC_MODE_COMPAT    */
+    int    warm_start;     /* 1: momentum and scalar BiCGStab start from the current
+                              field (a guess no better than zero is dropped for
+                              x0 = 0); 0: always x0 = 0 (VecZeroEntries,
+                              FlowSolver.inl:1348); -1: 1 in MC_MODE_SIMPLE, 0 in
+                              MC_MODE_COMPAT                                     */
     int    amg_half;       /* 1: the mixed form's V-cycle smooths the fine level of a
                               7-diagonal pressure pattern with an FP16 copy of
                               D^-1 A's off-diagonals (the PCG product and the FP64
