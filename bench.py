@@ -279,7 +279,9 @@ def main():
         "dtype": dtype,
         "data": "synthetic (reference box mesh and default initial/boundary fields)",
         "config": {"workload": "box mesh modifier %d (%d hex cells), %s mode, pressure rel. residual 1e-8, "
-                               "momentum/scalar BiCGStab rtol 1e-5" % (args.m, N, args.mode),
+                               "momentum/scalar BiCGStab rtol 1e-5 (||r|| <= rtol ||b||, %s)"
+                               % (args.m, N, args.mode,
+                                  "started from the current fields" if args.mode == "simple" else "started from zero"),
                    "modifier": args.m, "cells": N, "form": head, "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
