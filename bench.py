@@ -249,6 +249,9 @@ def main():
             form, forms[form]["iter_per_s"], forms[form]["pressure_ms_per_step"],
             forms[form]["pressure_iters_per_step"], forms[form]["last_rel_residual"]), file=sys.stderr, flush=True)
 
+    # the library's warm_start default (-1) starts the solves from the current fields in simple mode
+    ws = [int(kv.split("=", 1)[1]) for kv in args.opt if kv.split("=", 1)[0] == "warm_start"]
+    warm = (ws[-1] > 0 or (ws[-1] < 0 and args.mode == "simple")) if ws else args.mode == "simple"
     probe = copy_probe(mc, args.copy_probe_mb, 10) if args.copy_probe_mb > 0 else None
 
     ok = {k: v for k, v in forms.items() if v["last_rel_residual"] <= 1e-8}
@@ -280,8 +283,7 @@ def main():
         "data": "synthetic (reference box mesh and default initial/boundary fields)",
         "config": {"workload": "box mesh modifier %d (%d hex cells), %s mode, pressure rel. residual 1e-8, "
                                "momentum/scalar BiCGStab rtol 1e-5 (||r|| <= rtol ||b||, %s)"
-                               % (args.m, N, args.mode,
-                                  "started from the current fields" if args.mode == "simple" else "started from zero"),
+                               % (args.m, N, args.mode, "started from the current fields" if warm else "started from zero"),
                    "modifier": args.m, "cells": N, "form": head, "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
