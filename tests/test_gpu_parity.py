@@ -591,3 +591,27 @@ def test_fp64_gpu_case_m64_against_oracle():
     assert_close(g.get(mc.ARR_MFLUX), o.face("mflux"), "mass flux", 1e-8)
     o.close()
     g.close()
+
+
+@pytest.mark.parametrize("mode", [po.MODE_COMPAT, po.MODE_SIMPLE])
+def test_host_mesh_context(mode):
+    """A context built from host mesh arrays (the reference's Mesh<double>
+    layout, mc_ctx_create: consecutive-cell order, faces in reference order)
+    against the oracle for one timestep, and against the generated-box context
+    bit for bit (same sums in the same order, only the storage differs)."""
+    print("host mesh m=6 mode=%d" % mode, flush=True)
+    m = 6
+    mesh = po.oracle_mesh(m)
+    o = po.Oracle(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=100000)
+    kw = dict(mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, warm_start=0)
+    with mc.Context(mesh=mesh, **kw) as gh, mc.Context(m, **kw) as gb:
+        assert gh.N == gb.N and gh.F == gb.F and gh.nnz == gb.nnz
+        assert np.array_equal(gh.get(mc.ARR_FACES), gb.get(mc.ARR_FACES))
+        o.timestep()
+        gh.timestep()
+        gb.timestep()
+        for f in FIELDS_ALL:
+            assert_close(gh.phi(f), o.phi(f), "host mesh field %s" % po.FIELD_NAMES[f], 1e-8)
+            assert_bits(gh.phi(f), gb.phi(f), "host mesh vs box field %s" % po.FIELD_NAMES[f])
+        assert_bits(gh.get(mc.ARR_MFLUX), gb.get(mc.ARR_MFLUX), "host mesh vs box mass flux")
+    o.close()
