@@ -615,3 +615,25 @@ def test_host_mesh_context(mode):
             assert_bits(gh.phi(f), gb.phi(f), "host mesh vs box field %s" % po.FIELD_NAMES[f])
         assert_bits(gh.get(mc.ARR_MFLUX), gb.get(mc.ARR_MFLUX), "host mesh vs box mass flux")
     o.close()
+
+
+def test_cpp_driver_matches_the_library(tmp_path):
+    """The C++ driver with the reference's call surface (host FlowSolver:
+    constructor, timestep(), output_data(t), print_logger_stats, compute_time)
+    runs 2 timesteps at modifier 4 and writes the flow VTK file the ctypes
+    binding writes after the same 2 timesteps, byte for byte."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(mc.LIB_PATH), "minicombust_amd")
+    assert os.path.exists(exe), "C++ driver not built"
+    env = dict(os.environ, MINICOMBUST_CELLS="4", MINICOMBUST_ITERS="2", MINICOMBUST_OUTPUT_ITER="2")
+    print("driver m=4", flush=True)
+    out = subprocess.run([exe], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Flow Compute Time" in out.stdout
+    drv = (tmp_path / "out" / "flow" / "minicombust_flow0_timestep2.vtk").read_bytes()
+    with mc.Context(4) as g:   # the driver's defaults: compat mode, FP64 pressure solve
+        g.timestep()
+        g.timestep()
+        ours = open(g.write_vtk(str(tmp_path / "lib"), 2), "rb").read()
+    assert drv == ours
