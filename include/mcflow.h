@@ -196,7 +196,10 @@ enum {
  *                 pattern (:491-502) and the first assemblies (:538-539).
  * mc_ctx_create_box  load_mesh (src/examples/mesh_examples.cpp:120-571, one flow
  *                 rank, box of src/minicombust.cpp:123-124) generated on the GPU,
- *                 followed by the constructor above; nothing crosses PCIe.     */
+ *                 followed by the constructor above; nothing crosses PCIe.
+ * Both check the options first (MC_ERR_ARG, *out = NULL), then the device
+ * (MC_ERR_NODEV).  A later failure still returns the context in *out, so that
+ * mc_last_error can say why; the caller then destroys it.                   */
 typedef struct mc_ctx mc_ctx;
 int  mc_options_default(mc_options *opt);
 int  mc_ctx_create(const mc_mesh_desc *mesh, const mc_options *opt, mc_ctx **out);

@@ -201,14 +201,15 @@ static int create_common(const mc_options *opt, mc_ctx **out, mc_ctx **made)
 {
     if (!out) return MC_ERR_ARG;
     *out = nullptr;
+    // options first: a bad option is an argument error with or without a device
+    std::string why;
+    if (opt && validate(opt, why)) return MC_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MC_ERR_NODEV;
     mc_ctx *x = new (std::nothrow) mc_ctx();
     if (!x) return MC_ERR_NOMEM;
     Ctx *c = &x->c;
     if (opt) c->opt = *opt; else mc_options_default(&c->opt);
-    const int v = validate(&c->opt, c->err);
-    if (v) { delete x; return v; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return MC_ERR_HIP; }
     c->own_stream = true;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
@@ -359,7 +360,8 @@ int mc_momentum_solve(mc_ctx *x, mc_solve_status *st)
     MC_TRY(bicgstab_solve(c, c->A_vals, 3, b, xx, c->opt.mom_rtol, c->opt.mom_maxit, &s, warm_start(c)));
     c->st.mom_iters += s.iterations;
     if (st) *st = s;
-    return s.converged ? MC_OK : MC_ERR_NOCONV;
+    if (!s.converged) { c->err = "momentum BiCGStab did not converge"; return MC_ERR_NOCONV; }
+    return MC_OK;
 }
 
 int mc_pressure_assemble(mc_ctx *x) { CTX_OR_FAIL return calculate_mass_flux_assemble(c); }

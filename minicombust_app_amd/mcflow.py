@@ -301,9 +301,13 @@ class Context:
         self._check(self.L.mc_sync(self.h), "mc_sync")
 
     def set_options(self, **kw):
+        """Change options; when the library refuses them (MCError) the context
+        and this object keep the previous ones."""
+        new = Options.from_buffer_copy(self.opts)
         for k, v in kw.items():
-            setattr(self.opts, k, v)
-        self._check(self.L.mc_set_options(self.h, C.byref(self.opts)), "mc_set_options")
+            setattr(new, k, v)
+        self._check(self.L.mc_set_options(self.h, C.byref(new)), "mc_set_options")
+        self.opts = new
 
     # ---- linear algebra ----
     def spmv(self, x, which=0):

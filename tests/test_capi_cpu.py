@@ -112,6 +112,24 @@ def test_null_context_is_an_argument_error():
     assert L.mc_ctx_destroy(None) == mc.MC_OK
 
 
+@pytest.mark.parametrize("field,value", [("mode", 7), ("psolve", -1), ("amg_nu", 0), ("amg_coarse_max", 4096),
+                                         ("cell_tile", 48), ("warm_start", 2), ("amg_half", 3), ("p_rtol", 0.0),
+                                         ("inlet_scale", -1.0), ("amg_refresh", 0)])
+def test_bad_options_are_refused_before_any_device_call(field, value):
+    """Context creation checks the options before it asks for a device: the
+    call returns MC_ERR_ARG and leaves no context behind (so this runs on a
+    machine with no GPU)."""
+    L = C.CDLL(_ensure_built())
+    L.mc_options_default.argtypes = [C.POINTER(mc.Options)]
+    L.mc_ctx_create_box.argtypes = [C.c_int64, C.POINTER(mc.Options), C.POINTER(C.c_void_p)]
+    o = mc.Options()
+    assert L.mc_options_default(C.byref(o)) == mc.MC_OK
+    setattr(o, field, value)
+    h = C.c_void_p(12345)
+    assert L.mc_ctx_create_box(4, C.byref(o), C.byref(h)) == mc.MC_ERR_ARG
+    assert h.value is None
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)

@@ -637,3 +637,44 @@ def test_cpp_driver_matches_the_library(tmp_path):
         g.timestep()
         ours = open(g.write_vtk(str(tmp_path / "lib"), 2), "rb").read()
     assert drv == ours
+
+
+def test_error_paths_leave_the_context_usable():
+    """Refusals come back as status codes with a message and change nothing:
+    a box modifier out of range, options mc_set_options refuses (the previous
+    ones stay in force), array calls with a wrong size, an unknown id or a
+    read-only array, particle terms read before any were added.  A momentum
+    solve stopped by its iteration limit returns MC_ERR_NOCONV; after
+    mc_reset_fields the context's next timestep matches a fresh context's."""
+    print("error paths m=4", flush=True)
+    with pytest.raises(mc.MCError, match="modifier out of range"):
+        mc.Context(0)
+    kw = dict(mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, warm_start=0)
+    with mc.Context(4, **kw) as g, mc.Context(4, **kw) as fresh:
+        L, h = g.L, g.h
+        for bad in [dict(amg_nu=0), dict(warm_start=5), dict(cell_tile=3)]:
+            with pytest.raises(mc.MCError, match="rc=%d" % mc.MC_ERR_ARG):
+                g.set_options(**bad)
+        assert g.opts.amg_nu == fresh.opts.amg_nu and g.opts.warm_start == 0
+        buf = np.zeros(g.N + 6)
+        assert L.mc_array_get(h, mc.ARR_PHI, buf.ctypes.data, g.N) == mc.MC_ERR_ARG
+        assert "expected %d" % (g.N + 6) in L.mc_last_error(h).decode()
+        assert L.mc_array_get(h, 9999, buf.ctypes.data, g.N) == mc.MC_ERR_ARG
+        assert "unknown array id" in L.mc_last_error(h).decode()
+        faces = g.get(mc.ARR_FACES)
+        assert L.mc_array_set(h, mc.ARR_FACES, faces.ctypes.data, faces.size) == mc.MC_ERR_ARG
+        assert "read-only" in L.mc_last_error(h).decode()
+        terms = np.zeros(5 * g.N)
+        assert L.mc_array_get(h, mc.ARR_PARTICLE_TERMS, terms.ctypes.data, terms.size) == mc.MC_ERR_STATE
+        g.set_options(mom_maxit=1, mom_rtol=1e-30)
+        g.get_phi_gradients()
+        g.set_up_field()
+        g.momentum_assemble()
+        with pytest.raises(mc.MCError, match="rc=%d.*did not converge" % mc.MC_ERR_NOCONV):
+            g.momentum_solve()
+        g.set_options(mom_maxit=fresh.opts.mom_maxit, mom_rtol=fresh.opts.mom_rtol)
+        g.reset_fields()
+        g.timestep()
+        fresh.timestep()
+        for f in FIELDS_ALL:
+            assert_close(g.phi(f), fresh.phi(f), "after the errors, field %s" % po.FIELD_NAMES[f], 1e-10)
