@@ -2012,6 +2012,26 @@ __global__ void __launch_bounds__(BLOCK) k_ir_resid(Mat<double> A, const double 
     block_partials<1>(dsum, partials, bb, gridDim.x);
 }
 
+// The first refinement residual of a solve started from zero: r = b - mean
+// (the product A 0 adds +0.0, so these are the bits k_ir_resid gives for a
+// matrix in diagonal storage, whose rows it visits in the same order) with x
+// set to zero in the same pass; A and x are not read.
+__global__ void __launch_bounds__(BLOCK) k_ir_resid0(int n, const double *b, const double *mean, double *x,
+                                                     double *r, double *partials, int chunk)
+{
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(n, r0 + chunk);
+    double dsum[1] = {0.0};
+    const double mu = *mean;
+    for (int row = r0 + threadIdx.x; row < r1; row += BLOCK) {
+        const double rr = b[row] - mu;
+        r[row] = rr;
+        x[row] = 0.0;
+        dsum[0] += rr * rr;
+    }
+    block_partials<1>(dsum, partials, bb, gridDim.x);
+}
+
 struct EpiSqrt {
     double *out;
     __device__ void operator()(const double (&v)[1]) const { *out = sqrt(v[0]); }
@@ -2051,14 +2071,19 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5], c->wk32[6]};
     const Mat<double> A = level_mat<double>(c->amg[0]);
     sum_scaled(c, b, n, mean, 1.0 / (double)n);
-    if (!x0) hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
+    // diagonal storage: the first residual of a zero start skips A (same bits,
+    // the same rows per thread); CSR rows are reduced in another order
+    const bool fast0 = !x0 && nd_code(A) != 0;
+    if (!x0 && !fast0) hipLaunchKernelGGL(k_zero, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x);
     double bnorm = x0 ? bnorm0 : -1.0, rnorm = 0.0;
     int total = 0, outer = 0;
     bool conv = false;
     const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
     const double rbytes = mat_bytes(A, c->nnz, 8) + 24.0 * n;   // A; b, x read, r written
     for (outer = 0; outer < 50; outer++) {
-        {
+        if (outer == 0 && fast0) {
+            hipLaunchKernelGGL(k_ir_resid0, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, x, r, c->red, g.chunk);
+        } else {
             KTimer kt(c, MC_KC_IR_RESID, rbytes);
             MC_DISPATCH_ND(nd_code(A), (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
                            g.chunk);
