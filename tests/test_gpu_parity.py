@@ -678,3 +678,21 @@ def test_error_paths_leave_the_context_usable():
         fresh.timestep()
         for f in FIELDS_ALL:
             assert_close(g.phi(f), fresh.phi(f), "after the errors, field %s" % po.FIELD_NAMES[f], 1e-10)
+
+
+def test_pressure_form_switched_between_iterations():
+    """mc_set_options may change the pressure solve's form on a running
+    context (the FP32 hierarchy copies exist in both forms): FP64, then
+    mixed, then FP64 again over six SIMPLE iterations (m = 16, simple mode,
+    1e-10) gives the fields of an all-FP64 run to 1e-8 of each field's
+    maximum, every solve meeting its tolerance."""
+    print("psolve switch m=16", flush=True)
+    kw = dict(mode=po.MODE_SIMPLE, p_rtol=1e-10, mom_rtol=1e-12, warm_start=0)
+    with mc.Context(16, psolve=mc.PSOLVE_FP64, **kw) as g, mc.Context(16, psolve=mc.PSOLVE_FP64, **kw) as ref:
+        for it in range(6):
+            g.set_options(psolve=mc.PSOLVE_MIXED if it in (2, 3) else mc.PSOLVE_FP64)
+            g.simple_iteration(it % 3)
+            ref.simple_iteration(it % 3)
+        assert g.stats()["p_relres_max"] <= 1e-10 * 1.0001
+        for f in FIELDS_ALL:
+            assert_close(g.phi(f), ref.phi(f), "switched form, field %s" % po.FIELD_NAMES[f], 1e-8)
