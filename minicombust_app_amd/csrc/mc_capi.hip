@@ -21,6 +21,8 @@ int alloc(Ctx *c, void **p, size_t bytes)
         return MC_ERR_NOMEM;
     }
     allocs_of(c).push_back(q);
+    c->alloc_cat.push_back(c->mem_cat);
+    c->alloc_bytes.push_back((int64_t)bytes);
     c->mem_bytes[c->mem_cat] += (int64_t)bytes;
     *p = q;
     return MC_OK;
@@ -30,6 +32,31 @@ int dfree_all(Ctx *c)
 {
     for (void *p : allocs_of(c)) (void)hipFree(p);
     allocs_of(c).clear();
+    c->alloc_cat.clear();
+    c->alloc_bytes.clear();
+    return MC_OK;
+}
+
+int dfree_category(Ctx *c, int cat)
+{
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->side) MC_HIP(c, hipStreamSynchronize(c->side));
+    std::vector<void *> &a = allocs_of(c);
+    size_t k = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        if (c->alloc_cat[i] == cat) {
+            MC_HIP(c, hipFree(a[i]));
+            c->mem_bytes[cat] -= c->alloc_bytes[i];
+            continue;
+        }
+        a[k] = a[i];
+        c->alloc_cat[k] = c->alloc_cat[i];
+        c->alloc_bytes[k] = c->alloc_bytes[i];
+        k++;
+    }
+    a.resize(k);
+    c->alloc_cat.resize(k);
+    c->alloc_bytes.resize(k);
     return MC_OK;
 }
 

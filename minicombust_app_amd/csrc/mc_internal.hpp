@@ -192,6 +192,8 @@ struct Ctx {
 
     // ---- device allocations owned by the context ----
     std::vector<void *> allocs;
+    std::vector<int> alloc_cat;            // category of each, as charged
+    std::vector<int64_t> alloc_bytes;
     int mem_cat = MC_MEM_MESH;             // category charged by alloc()
     int64_t mem_bytes[MC_MEM_COUNT] = {};
 
@@ -404,6 +406,8 @@ inline int dalloc(Ctx *c, T **p, size_t count)
     return alloc(c, (void **)p, count * sizeof(T) + 16);
 }
 int dfree_all(Ctx *c);
+// Frees every allocation charged to one category (after both streams drain).
+int dfree_category(Ctx *c, int cat);
 // charge the allocations of a scope to one memory category
 struct MemScope {
     Ctx *c;

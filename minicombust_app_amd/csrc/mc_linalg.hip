@@ -1029,7 +1029,14 @@ int amg_setup(Ctx *c, const double *vals)
     MC_HIP(c, hipMemcpyAsync(A.col.data(), c->col, (size_t)c->nnz * 4, hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipMemcpyAsync(A.val.data(), vals, (size_t)c->nnz * 8, hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
-    c->amg.clear();
+    // a rebuild (e.g. after amg_coarse_max changed) first releases the old
+    // hierarchy: every allocation charged to MC_MEM_AMG is made below
+    if (!c->amg.empty()) {
+        c->amg.clear();
+        c->amg_dense = nullptr;
+        c->amg_ready = false;
+        MC_TRY(dfree_category(c, MC_MEM_AMG));
+    }
     const int coarse_max = std::max(8, std::min(c->opt.amg_coarse_max, DENSE_MAX));
     HostCSR cur = std::move(A);
     bool first = true;

@@ -696,3 +696,22 @@ def test_pressure_form_switched_between_iterations():
         assert g.stats()["p_relres_max"] <= 1e-10 * 1.0001
         for f in FIELDS_ALL:
             assert_close(g.phi(f), ref.phi(f), "switched form, field %s" % po.FIELD_NAMES[f], 1e-8)
+
+
+def test_amg_rebuild_releases_the_old_hierarchy():
+    """Changing amg_coarse_max rebuilds the AMG hierarchy at the next solve;
+    the old levels are freed first, so the context's AMG memory stays near
+    one hierarchy's (aggregates follow the matrix values, so a rebuild at a
+    later iteration may differ by a few aggregates; four rebuilds kept would
+    be five times it), and every solve still meets its tolerance (m = 16,
+    simple mode, mixed form)."""
+    print("amg rebuild m=16", flush=True)
+    with mc.Context(16, mode=po.MODE_SIMPLE, psolve=mc.PSOLVE_MIXED, p_rtol=1e-10, warm_start=0) as g:
+        g.simple_iteration(0)
+        amg0 = g.memory()["amg"]
+        assert amg0 > 0
+        for it, cmax in enumerate([32, 128, 32, 128]):
+            g.set_options(amg_coarse_max=cmax)
+            g.simple_iteration(1 + it % 2)
+            assert g.memory()["amg"] < 1.5 * amg0
+        assert g.stats()["p_relres_max"] <= 1e-10 * 1.0001
