@@ -42,10 +42,14 @@ int dfree_category(Ctx *c, int cat)
     MC_HIP(c, hipStreamSynchronize(c->stream));
     if (c->side) MC_HIP(c, hipStreamSynchronize(c->side));
     std::vector<void *> &a = allocs_of(c);
+    // a failing hipFree is recorded and the compaction finishes, so that no
+    // entry is left twice in the list (mc_ctx_destroy frees what remains)
+    hipError_t first_err = hipSuccess;
     size_t k = 0;
     for (size_t i = 0; i < a.size(); i++) {
         if (c->alloc_cat[i] == cat) {
-            MC_HIP(c, hipFree(a[i]));
+            const hipError_t e = hipFree(a[i]);
+            if (e != hipSuccess && first_err == hipSuccess) first_err = e;
             c->mem_bytes[cat] -= c->alloc_bytes[i];
             continue;
         }
@@ -57,6 +61,10 @@ int dfree_category(Ctx *c, int cat)
     a.resize(k);
     c->alloc_cat.resize(k);
     c->alloc_bytes.resize(k);
+    if (first_err != hipSuccess) {
+        c->err = std::string("hipFree: ") + hipGetErrorString(first_err);
+        return MC_ERR_HIP;
+    }
     return MC_OK;
 }
 
@@ -307,7 +315,9 @@ int mc_set_options(mc_ctx *x, const mc_options *opt)
     if (!x || !opt) return MC_ERR_ARG;
     Ctx *c = &x->c;
     MC_TRY(validate(opt, c->err));
-    const bool amg_change = opt->amg_coarse_max != c->opt.amg_coarse_max;
+    // the hierarchy is rebuilt for a new coarse bound, and when the FP16
+    // smoothing copy is switched on (it is allocated with the hierarchy)
+    const bool amg_change = opt->amg_coarse_max != c->opt.amg_coarse_max || (opt->amg_half && !c->opt.amg_half);
     c->opt = *opt;
     set_constants(c);
     if (amg_change) c->amg_ready = false;

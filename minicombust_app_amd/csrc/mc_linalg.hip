@@ -1136,6 +1136,9 @@ int amg_setup(Ctx *c, const double *vals)
     MC_TRY(dalloc(c, &c->amg_dense, (size_t)nco * nco));
     c->amg_ready = true;
     c->amg_updates = 0;
+    // level 0's diagonal copies were just allocated and zeroed: whatever an
+    // earlier pressure assembly wrote went to the hierarchy released above
+    c->pa_dia_fresh = false;
     return amg_update_values(c, vals);
 }
 

@@ -503,12 +503,15 @@ def test_warm_start_momentum_solve(guess):
         assert out[1][1] <= out[0][1], (out[1][1], out[0][1])
 
 
-def test_warm_start_te_solve_meets_criterion():
-    """The TE solve's starting residual b - A x0 is formed inside the assembly
-    kernel (no separate product): the warm-started solve still meets
-    ||b - A x|| <= rtol ||b|| on the assembled system, and agrees with the
-    zero start to the accuracy both solves promise."""
-    print("warm start TE", flush=True)
+@pytest.mark.parametrize("eq", [po.TERBTE, po.TERBED])
+def test_warm_start_te_solve_meets_criterion(eq):
+    """The TE and ED solves' starting residual b - A x0 is formed inside the
+    assembly kernel (no separate product), including the ED wall rows whose
+    fixed value differs from the current field: the warm-started solve still
+    meets ||b - A x|| <= rtol ||b|| on the assembled system, and agrees with
+    the zero start to the accuracy both solves promise."""
+    field = po.TE if eq == po.TERBTE else po.ED
+    print("warm start %s" % po.FIELD_NAMES[field], flush=True)
     m, rtol = 16, 1e-7
     out = {}
     for warm in (0, 1):
@@ -517,16 +520,18 @@ def test_warm_start_te_solve_meets_criterion():
                 g.simple_iteration(it)
             g.set_options(warm_start=warm)
             g.get_phi_gradients()
-            g.scalar_solve(po.TERBTE)
+            if eq == po.TERBED:
+                g.scalar_solve(po.TERBTE)
+            g.scalar_solve(eq)
             N = g.N
             b = g.get(mc.ARR_SU)[:N].copy()
-            x = g.phi(po.TE)[:N].copy()
+            x = g.phi(field)[:N].copy()
             res = np.linalg.norm(b - g.spmv(x, which=0)) / np.linalg.norm(b)
             assert res <= 1.05 * rtol, (warm, res)
             out[warm] = x
-    # the TE repair after the solve only lifts negative values, none here
+    # the repair after the solve only lifts negative values, none here
     assert np.all(out[1] > 0)
-    assert_close(out[1], out[0], "TE warm vs zero start", 1e-5)
+    assert_close(out[1], out[0], "%s warm vs zero start" % po.FIELD_NAMES[field], 1e-5)
 
 
 def test_fp16_smoothing_copy_keeps_the_tolerance():
@@ -715,3 +720,56 @@ def test_amg_rebuild_releases_the_old_hierarchy():
             g.simple_iteration(1 + it % 2)
             assert g.memory()["amg"] < 1.5 * amg0
         assert g.stats()["p_relres_max"] <= 1e-10 * 1.0001
+
+
+def test_amg_rebuild_between_assembly_and_solve():
+    """mc_pressure_assemble, then mc_set_options with a new amg_coarse_max,
+    then mc_pressure_solve: the rebuilt hierarchy's fine level is refreshed
+    from the assembled matrix (the assembly's diagonal copies went to the
+    released hierarchy), so the solve is the one of a context whose options
+    changed before the assembly, bit for bit, and meets its tolerance."""
+    print("amg rebuild after assembly m=16", flush=True)
+    kw = dict(mode=po.MODE_SIMPLE, psolve=mc.PSOLVE_MIXED, p_rtol=1e-10, mom_rtol=1e-12, warm_start=0)
+    out = {}
+    for order in ("options_first", "assembly_first"):
+        with mc.Context(16, **kw) as g:
+            g.simple_iteration(0)
+            if order == "options_first":
+                g.set_options(amg_coarse_max=32)
+            g.pressure_assemble()
+            if order == "assembly_first":
+                g.set_options(amg_coarse_max=32)
+            st = g.pressure_solve()
+            assert st.converged and st.relres <= 1e-10 * 1.0001, (order, st.relres)
+            out[order] = (g.phi(po.PP).copy(), st.iterations)
+    assert out["options_first"][1] == out["assembly_first"][1]
+    assert_bits(out["assembly_first"][0], out["options_first"][0], "PP, rebuild after the assembly")
+
+
+def test_fp16_copy_switched_on_after_the_hierarchy():
+    """amg_half switched from 0 to 1 on a context whose hierarchy exists
+    rebuilds it with the FP16 smoothing copy: the next pressure solve is the
+    one of a context that had amg_half = 1 throughout and rebuilt at the same
+    point (amg_coarse_max changed), bit for bit, on the same system."""
+    print("amg_half 0 -> 1 m=16", flush=True)
+    o = po.Oracle(16, mode=po.MODE_SIMPLE, mom_rtol=1e-12, p_rtol=1e-12)
+    o.get_phi_gradients()
+    o.set_up_field()
+    o.calculate_UVW()
+    oracle_pressure_assemble(o)
+    b = o.cell("S_U").copy()
+    kw = dict(mode=po.MODE_SIMPLE, psolve=mc.PSOLVE_MIXED, p_rtol=1e-10)
+    out = {}
+    for case in ("half_later", "half_always"):
+        first = dict(amg_half=0) if case == "half_later" else dict(amg_half=1, amg_coarse_max=64)
+        with mc.Context(16, **dict(kw, **first)) as g:
+            push_state(o, g)
+            x, st = g.solve_pressure(b)
+            assert st.converged
+            g.set_options(amg_half=1, amg_coarse_max=128)
+            x, st = g.solve_pressure(b)
+            assert st.converged
+            out[case] = (x, st.iterations)
+    assert out["half_later"][1] == out["half_always"][1]
+    assert_bits(out["half_later"][0], out["half_always"][0], "pressure correction, FP16 copy switched on later")
+    o.close()
