@@ -175,20 +175,35 @@ static double dot(int64_t n, const double *a, const double *b)
 
 /* Unpreconditioned BiCGStab (van der Vorst 1992), x0 = 0 as set by
    VecZeroEntries(u) before KSPSolve (FlowSolver.inl:1348). Converged when
-   ||r|| <= rtol ||b||, PETSc's default test with x0 = 0. */
-int mco_bicgstab(int64_t n, const int64_t *rp, const int64_t *col, const double *A,
-                 const double *b, double *x, double rtol, int maxit, double *relres)
+   ||r|| <= rtol ||b||, PETSc's default test with x0 = 0.
+   warm != 0 (not in the reference; the library's simple-mode warm start):
+   x holds x0, r0 = b - A x0, and a guess whose residual is not below ||b||
+   is dropped for x0 = 0; the shadow residual is b and the test
+   ||r|| <= rtol ||b|| either way. */
+int mco_bicgstab_x0(int64_t n, const int64_t *rp, const int64_t *col, const double *A,
+                    const double *b, double *x, double rtol, int maxit, double *relres, int warm)
 {
     double *r = malloc(n * sizeof(double)), *rh = malloc(n * sizeof(double));
     double *p = calloc(n, sizeof(double)), *v = calloc(n, sizeof(double));
     double *s = malloc(n * sizeof(double)), *t = malloc(n * sizeof(double));
-    for (int64_t i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; rh[i] = b[i]; }
     const double bnorm = sqrt(dot(n, b, b));
     const double tol = rtol * bnorm;
+    double rn = bnorm;
+    int zero_start = 1;
+    if (warm && bnorm > 0.0) {
+        mco_spmv(n, rp, col, A, x, t);
+        for (int64_t i = 0; i < n; i++) r[i] = b[i] - t[i];
+        rn = sqrt(dot(n, r, r));
+        zero_start = !(rn < bnorm);
+        if (zero_start) rn = bnorm;
+    }
+    if (zero_start)
+        for (int64_t i = 0; i < n; i++) { x[i] = 0.0; r[i] = b[i]; }
+    for (int64_t i = 0; i < n; i++) rh[i] = b[i];
     double rho_old = 1.0, alpha = 1.0, omega = 1.0;
     int it = 0;
-    double rn = bnorm;
     if (bnorm == 0.0) { if (relres) *relres = 0.0; goto done; }
+    if (rn <= tol) { if (relres) *relres = rn / bnorm; goto done; }
     for (it = 1; it <= maxit; it++) {
         const double rho = dot(n, rh, r);
         if (rho == 0.0) break;
@@ -214,6 +229,12 @@ int mco_bicgstab(int64_t n, const int64_t *rp, const int64_t *col, const double 
 done:
     free(r); free(rh); free(p); free(v); free(s); free(t);
     return it;
+}
+
+int mco_bicgstab(int64_t n, const int64_t *rp, const int64_t *col, const double *A,
+                 const double *b, double *x, double rtol, int maxit, double *relres)
+{
+    return mco_bicgstab_x0(n, rp, col, A, b, x, rtol, maxit, relres, 0);
 }
 
 /* Jacobi-preconditioned CG for the singular, consistent pressure-correction
@@ -577,6 +598,7 @@ void mco_default_opts(mco_opts *o)
     o->solve_scalars = 1;
     o->p_solver = 0;
     o->limit_grad = 0;
+    o->warm_start = 0;
 }
 
 static void setup_sparse_matrix(mco_state *s, double URFactor, double *A_c, double *phi_c, double *S_c);
@@ -1075,7 +1097,9 @@ static void update_sparse_matrix(mco_state *s, double URFactor, double *A_c, dou
 static int solve_sparse_matrix(mco_state *s, double *phi_c)
 {
     double rr;
-    const int it = mco_bicgstab(s->N, s->row_ptr, s->col, s->A_vals, s->b, s->u, s->opts.mom_rtol, s->opts.mom_maxit, &rr);
+    if (s->opts.warm_start) memcpy(s->u, phi_c, s->N * sizeof(double));
+    const int it = mco_bicgstab_x0(s->N, s->row_ptr, s->col, s->A_vals, s->b, s->u, s->opts.mom_rtol,
+                                   s->opts.mom_maxit, &rr, s->opts.warm_start);
     memcpy(phi_c, s->u, s->N * sizeof(double));
     return it;
 }

@@ -73,6 +73,10 @@ typedef struct mco_opts {
     int    p_solver;         /* 0: Jacobi-PCG (checker), 1: aggregation-AMG PCG */
     int    limit_grad;       /* 1: Venkatakrishnan limiter after the gradients
                                 (FLOW_SOLVER_LIMIT_GRAD, utils.hpp:26; off there) */
+    int    warm_start;       /* 1: momentum and scalar BiCGStab start from the
+                                current field (the library's simple-mode option;
+                                the reference starts from zero,
+                                FlowSolver.inl:1348)                          */
 } mco_opts;
 
 typedef struct mco_stats {
@@ -160,6 +164,10 @@ void mco_timestep(mco_state *s);                       /* 3 passes */
 int  mco_bicgstab(int64_t n, const int64_t *row_ptr, const int64_t *col,
                   const double *vals, const double *b, double *x,
                   double rtol, int maxit, double *relres);
+/* the same with warm != 0: x holds the starting guess on entry */
+int  mco_bicgstab_x0(int64_t n, const int64_t *row_ptr, const int64_t *col,
+                     const double *vals, const double *b, double *x,
+                     double rtol, int maxit, double *relres, int warm);
 int  mco_pcg_singular(int64_t n, const int64_t *row_ptr, const int64_t *col,
                       const double *vals, const double *b, double *x,
                       double rtol, double atol, int maxit, double *relres, double *bnorm);

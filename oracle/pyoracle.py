@@ -42,7 +42,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("delta", C.c_double), ("mom_rtol", C.c_double), ("mom_maxit", C.c_int),
         ("p_rtol", C.c_double), ("p_maxit", C.c_int), ("solve_scalars", C.c_int), ("p_solver", C.c_int),
-        ("limit_grad", C.c_int),
+        ("limit_grad", C.c_int), ("warm_start", C.c_int),
     ]
 
 
@@ -125,6 +125,8 @@ def lib():
                                        _DP]
         L.mco_bicgstab.restype = C.c_int
         L.mco_bicgstab.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP, C.c_double, C.c_int, _DP]
+        L.mco_bicgstab_x0.restype = C.c_int
+        L.mco_bicgstab_x0.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP, C.c_double, C.c_int, _DP, C.c_int]
         L.mco_pcg_amg.restype = C.c_int
         L.mco_pcg_amg.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP, C.c_double, C.c_double, C.c_int, _DP, _DP]
         L.mco_spmv.argtypes = [C.c_int64, _IP, _IP, _DP, _DP, _DP]

@@ -37,7 +37,7 @@ def test_oracle_mesh_equals_reference_golden(m):
 
 
 @pytest.mark.skipif(not os.path.exists(po.REF_MESH_DUMP), reason="oracle/_ref not built (needs the reference tree)")
-@pytest.mark.parametrize("m", [4, 5, 8, 20])
+@pytest.mark.parametrize("m", [4, 5, 8, 20, 64])
 def test_oracle_mesh_equals_reference_generator(m):
     """Same, at larger sizes, running the reference's compiled generator."""
     with tempfile.TemporaryDirectory() as d:
@@ -249,6 +249,39 @@ def test_bicgstab_against_direct_solve():
     assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b) * 1.0001
     xd = spla.spsolve(A.tocsc(), b)
     np.testing.assert_allclose(x, xd, rtol=1e-8, atol=1e-12 * np.abs(xd).max())
+
+
+def test_bicgstab_warm_start():
+    """The oracle's warm-started BiCGStab (the library's simple-mode option):
+    from a perturbed solution it meets ||b - A x|| <= rtol ||b|| in fewer
+    iterations than from zero; from the exact solution it does no iteration
+    and returns the guess; from a guess worse than zero it is the zero start,
+    bit for bit (the guess is dropped)."""
+    import ctypes as C
+    o = _state_after_momentum(4)
+    rp, col = o.csr()
+    vals = o.vals("A_vals").copy()
+    b = o.cell("S_U").copy()
+    A = sp.csr_matrix((vals, col, rp), shape=(o.N, o.N))
+    xd = spla.spsolve(A.tocsc(), b)
+    L = po.lib()
+
+    def solve(x0, warm, rtol=1e-8):
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        rel = C.c_double(0.0)
+        its = L.mco_bicgstab_x0(o.N, _ptr(rp, np.int64), _ptr(col, np.int64), _ptr(vals), _ptr(b), _ptr(x),
+                                rtol, 1000, C.byref(rel), warm)
+        return x, its
+
+    rng = np.random.default_rng(4)
+    x_zero, it_zero = solve(np.zeros(o.N), 0)
+    x_warm, it_warm = solve(xd * (1.0 + 1e-4 * rng.standard_normal(o.N)), 1)
+    assert np.linalg.norm(b - A @ x_warm) <= 1e-8 * np.linalg.norm(b) * 1.0001
+    assert 0 < it_warm < it_zero, (it_warm, it_zero)
+    x_ex, it_ex = solve(xd, 1, rtol=1e-6)
+    assert it_ex == 0 and np.array_equal(x_ex, xd)
+    x_bad, it_bad = solve(1e6 * rng.standard_normal(o.N), 1)
+    assert it_bad == it_zero and same_bits(x_bad, x_zero)
 
 
 @pytest.mark.parametrize("solver", ["pcg_singular", "pcg_amg"])
