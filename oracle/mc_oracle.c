@@ -1378,11 +1378,40 @@ void mco_update_mass_flux(mco_state *s)
 /* ======================================================================
  * A14: pressure-correction loop (FlowSolver.inl:2317-2463)
  * ====================================================================== */
-void mco_calculate_pressure(mco_state *s)
+/* One loop's corrections from the solved PP (FlowSolver.inl:2391-2433):
+   PP boundary slots and gradient, face fluxes += ff0 (PP1 - PP0), and the
+   cell corrections P += 0.2 PP, U -= grad PP V / A_U -- in compat mode only
+   while first_pressure holds, which the first corrected cell clears (so the
+   reference corrects cell 0 of the first loop only). */
+void mco_apply_pressure_correction(mco_state *s)
 {
     const mco_mesh *M = &s->mesh;
     const int64_t N = s->N;
     double *PP = s->phi[MCO_PP], *P = s->phi[MCO_P];
+    mco_update_P_at_boundaries(s, MCO_PP);
+    mco_get_phi_gradient(s, MCO_PP, 1);
+    for (int64_t face = 0; face < s->F; face++) {
+        const uint64_t c0 = M->faces[2 * face], c1 = M->faces[2 * face + 1];
+        if (c1 >= (uint64_t)N) continue;
+        s->mflux[face] += (s->ff0[face] * (PP[c1] - PP[c0]));
+    }
+    for (int64_t cell = 0; cell < N; cell++) {
+        const double Ar = (s->A_U[cell] != 0.0) ? 1.0 / s->A_U[cell] : 0.0;
+        const double fact = s->cell_volumes[cell] * Ar;
+        if (s->first_pressure || s->opts.mode == MCO_MODE_SIMPLE) {
+            P[cell] += 0.2 * PP[cell];
+            s->phi[MCO_U][cell] -= s->grad[MCO_PP][cell].x * fact;
+            s->phi[MCO_V][cell] -= s->grad[MCO_PP][cell].y * fact;
+            s->phi[MCO_W][cell] -= s->grad[MCO_PP][cell].z * fact;
+            s->first_pressure = 0;
+        }
+    }
+}
+
+void mco_calculate_pressure(mco_state *s)
+{
+    const int64_t N = s->N;
+    double *PP = s->phi[MCO_PP];
     int Loop_num = 0, Loop_continue = 1;
     double Pressure_correction_max = 0, Pressure_correction_ref = 0;
     double first_bnorm = 0.0;
@@ -1402,24 +1431,7 @@ void mco_calculate_pressure(mco_state *s)
         for (int64_t i = 1; i < N; i++)
             if (fabs(PP[i]) > Pressure_correction_max) Pressure_correction_max = fabs(PP[i]);
         if (Loop_num == 1) Pressure_correction_ref = Pressure_correction_max;
-        mco_update_P_at_boundaries(s, MCO_PP);
-        mco_get_phi_gradient(s, MCO_PP, 1);
-        for (int64_t face = 0; face < s->F; face++) {
-            const uint64_t c0 = M->faces[2 * face], c1 = M->faces[2 * face + 1];
-            if (c1 >= (uint64_t)N) continue;
-            s->mflux[face] += (s->ff0[face] * (PP[c1] - PP[c0]));
-        }
-        for (int64_t cell = 0; cell < N; cell++) {
-            const double Ar = (s->A_U[cell] != 0.0) ? 1.0 / s->A_U[cell] : 0.0;
-            const double fact = s->cell_volumes[cell] * Ar;
-            if (s->first_pressure || s->opts.mode == MCO_MODE_SIMPLE) {
-                P[cell] += 0.2 * PP[cell];
-                s->phi[MCO_U][cell] -= s->grad[MCO_PP][cell].x * fact;
-                s->phi[MCO_V][cell] -= s->grad[MCO_PP][cell].y * fact;
-                s->phi[MCO_W][cell] -= s->grad[MCO_PP][cell].z * fact;
-                s->first_pressure = 0;
-            }
-        }
+        mco_apply_pressure_correction(s);
         for (int64_t i = 0; i < N; i++) s->S_U[i] = 0.0;
         mco_update_mass_flux(s);
         for (int64_t i = 0; i < N + 6; i++) PP[i] = 0.0;

@@ -152,6 +152,7 @@ void mco_solve_pressure_matrix(mco_state *s);
 void mco_update_P_at_boundaries(mco_state *s, int field);
 void mco_update_P(mco_state *s, int field);
 void mco_update_mass_flux(mco_state *s);
+void mco_apply_pressure_correction(mco_state *s);   /* one loop's corrections from phi[PP] */
 void mco_calculate_pressure(mco_state *s);
 void mco_flux_scalar(mco_state *s, int type, int field);
 void mco_solve_turbulence_models(mco_state *s, int type);

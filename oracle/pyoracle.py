@@ -103,7 +103,7 @@ def lib():
         for name in ["mco_get_phi_gradients", "mco_limit_phi_gradients", "mco_precomp_AU", "mco_set_up_field", "mco_calculate_flux_UVW",
                      "mco_momentum_assemble", "mco_calculate_UVW", "mco_calculate_mass_flux",
                      "mco_setup_pressure_matrix", "mco_solve_pressure_matrix", "mco_update_mass_flux",
-                     "mco_calculate_pressure", "mco_timestep"]:
+                     "mco_calculate_pressure", "mco_timestep", "mco_apply_pressure_correction"]:
             getattr(L, name).argtypes = [SP]
         L.mco_get_phi_gradient.argtypes = [SP, C.c_int, C.c_int]
         L.mco_repair_nonneg.argtypes = [SP, C.c_int]
