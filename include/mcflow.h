@@ -299,6 +299,19 @@ int  mc_particle_terms_reset(mc_ctx *ctx);
    energy, fuel) added to cell cells[i], in receive order (host arrays).
    Result: mc_array_get(MC_ARR_PARTICLE_TERMS).                               */
 int  mc_particle_terms_add(mc_ctx *ctx, const uint64_t *cells, const double *terms, int64_t n);
+/* mc_particle_terms_add without any device allocation.  No counterpart in
+   the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight per-timestep coupling loops, harnesses that
+   forbid hipMalloc).  Same arguments and the same result, bit for bit, as
+   mc_particle_terms_add; its device buffers (staged rows, sort keys and
+   indices, the sort's temporary storage) live in a fixed 64 MiB array that
+   each GPU holds until the process ends, which bounds n to about 850,000 rows
+   (n x 64 bytes plus the sort's storage): larger n is refused with MC_ERR_ARG
+   and a message giving the bytes needed.  mc_particle_terms_reset must have
+   run once on the context (it allocates the terms array; MC_ERR_STATE
+   otherwise).  Every call on one GPU shares that array: calls that overlap
+   on one GPU (contexts on different streams or threads) race.              */
+int  mc_particle_terms_add_noalloc(mc_ctx *ctx, const uint64_t *cells, const double *terms, int64_t n);
 /* FlowSolver.inl:297-335 + interpolate_to_nodes (:428-493): values at the
    requested nodes from the interpolation cell set `cells` (duplicates count
    once; each node sums its cells in ascending order).  node_fields[5 k ..]:

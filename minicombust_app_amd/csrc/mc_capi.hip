@@ -255,6 +255,15 @@ static int create_common(const mc_options *opt, mc_ctx **out, mc_ctx **made)
     }
     set_constants(c);
     *made = x;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        c->err = "hipGetDevice failed";
+        *out = x;
+        return MC_ERR_HIP;
+    }
+    if (int rc = noalloc_scratch_init(c)) {
+        *out = x;
+        return rc;
+    }
     return MC_OK;
 }
 

@@ -13,6 +13,7 @@
 // contraction, like the reference's CPU arithmetic.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
 #include <vector>
 
 #include "mc_internal.hpp"
@@ -151,6 +152,95 @@ int particle_terms_add(Ctx *c, const uint64_t *cells, const double *terms, int64
     return MC_OK;
 }
 
+// ---- allocation-free variant of particle_terms_add -------------------------
+// Fixed device scratch of mc_particle_terms_add_noalloc: the staged cell ids
+// and term rows, the sort's keys and indices (in and out) and its temporary
+// storage.  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes `scratch` itself to a kernel or a
+// copy (in host code the name is only a stand-in for the device symbol): it
+// uses the address hipGetSymbolAddress returned for the context's device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+constexpr int SCRATCH_DEVICES = 64;
+static void *g_scratch_dev[SCRATCH_DEVICES];   // per device, resolved once
+
+int noalloc_scratch_init(Ctx *c)
+{
+    if (c->device < 0 || c->device >= SCRATCH_DEVICES) {
+        c->err = "device id out of range for the fixed scratch table";
+        return MC_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void *p = nullptr;
+        // the context's device is current here (context creation)
+        MC_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[c->device] = p;
+    }
+    c->noalloc_scratch = g_scratch_dev[c->device];
+    return MC_OK;
+}
+
+__global__ void k_keys_of_cells(int n, const uint64_t *cells, unsigned *key, int *idx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (unsigned)cells[i];
+    idx[i] = i;
+}
+
+// particle_terms_add with every device buffer in the fixed scratch: no
+// device allocation, no allocating library call.  Calls that overlap on one
+// GPU (two contexts on two streams) share the array and race.
+int particle_terms_add_noalloc(Ctx *c, const uint64_t *cells, const double *terms, int64_t n)
+{
+    if (n < 0 || n > (int64_t)INT32_MAX / 8 || (n > 0 && (!cells || !terms))) {
+        c->err = "particle terms: bad arguments";
+        return MC_ERR_ARG;
+    }
+    if (!c->pterms) {
+        c->err = "particle terms not allocated: call mc_particle_terms_reset once first (this entry point "
+                 "allocates nothing)";
+        return MC_ERR_STATE;
+    }
+    if (!c->noalloc_scratch) { c->err = "fixed scratch not resolved"; return MC_ERR_STATE; }
+    if (n == 0) return MC_OK;
+    for (int64_t i = 0; i < n; i++)
+        if (cells[i] >= (uint64_t)c->N) { c->err = "particle terms: cell index out of range"; return MC_ERR_ARG; }
+    int bits = 1;
+    while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)c->N) bits++;
+    size_t tb = 0;   // the size query allocates nothing
+    MC_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned *)nullptr, (unsigned *)nullptr,
+                                                 (const int *)nullptr, (int *)nullptr, (int)n, 0, bits, c->stream));
+    const size_t b8 = up16((size_t)n * 8), b4 = up16((size_t)n * 4), b40 = up16((size_t)n * 40);
+    const size_t need = b8 + 4 * b4 + b40 + up16(tb);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        char msg[200];
+        snprintf(msg, sizeof msg, "particle terms (no allocation): %lld rows need %zu bytes of scratch, above the "
+                 "fixed %zu", (long long)n, need, NOALLOC_SCRATCH_BYTES);
+        c->err = msg;
+        return MC_ERR_ARG;
+    }
+    char *p = (char *)c->noalloc_scratch;
+    uint64_t *c_dev = (uint64_t *)p;
+    unsigned *k_in = (unsigned *)(p + b8), *k_out = (unsigned *)(p + b8 + b4);
+    int *i_in = (int *)(p + b8 + 2 * b4), *i_out = (int *)(p + b8 + 3 * b4);
+    double *t_dev = (double *)(p + b8 + 4 * b4);
+    void *tmp = p + b8 + 4 * b4 + b40;
+    MC_HIP(c, hipMemcpyAsync(c_dev, cells, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    MC_HIP(c, hipMemcpyAsync(t_dev, terms, (size_t)n * 40, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_keys_of_cells, dim3(div_up(n, 256)), dim3(256), 0, c->stream, (int)n, c_dev, k_in, i_in);
+    MC_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, k_in, k_out, i_in, i_out, (int)n, 0, bits, c->stream));
+    hipLaunchKernelGGL(k_terms_add, dim3(div_up(n, 256)), dim3(256), 0, c->stream, (int)n, k_out, i_out, t_dev,
+                       c->pterms);
+    MC_LAUNCH_CHECK(c);
+    // the caller's host arrays must outlive the copies
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    return MC_OK;
+}
+
 // point -> cells in ascending cell order, built once from the cells' vertex lists
 static int ensure_point_cells(Ctx *c)
 {
@@ -238,6 +328,12 @@ int mc_particle_terms_add(mc_ctx *x, const uint64_t *cells, const double *terms,
 {
     if (!x) return MC_ERR_ARG;
     return mc::particle_terms_add(&x->c, cells, terms, n);
+}
+
+int mc_particle_terms_add_noalloc(mc_ctx *x, const uint64_t *cells, const double *terms, int64_t n)
+{
+    if (!x) return MC_ERR_ARG;
+    return mc::particle_terms_add_noalloc(&x->c, cells, terms, n);
 }
 
 int mc_interpolate_to_nodes(mc_ctx *x, const uint64_t *cells, int64_t ncells, const uint64_t *nodes, int64_t nnodes,

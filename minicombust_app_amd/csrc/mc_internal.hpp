@@ -78,6 +78,10 @@ struct AmgLevel {
 struct Ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int device = -1;               // the HIP device current at creation
+    // device address, on this context's device, of the fixed scratch array
+    // of the allocation-free coupling entry point (mc_couple.hip)
+    void *noalloc_scratch = nullptr;
     // the coarsest AMG level's inverse is formed on `side` (one workgroup)
     // while `stream` runs the first V-cycle's descent; ev_dense marks it done
     hipStream_t side = nullptr;
@@ -428,6 +432,9 @@ struct KTimer {
 int profile_fold(Ctx *c);
 int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out);   // mc_linalg.hip
 void profile_release(Ctx *c);
+
+// resolves Ctx::noalloc_scratch (once per device, with that device current)
+int noalloc_scratch_init(Ctx *c);   // mc_couple.hip
 
 // stages (mc_mesh.hip)
 int mesh_box_generate(Ctx *c, int64_t m);

@@ -113,6 +113,7 @@ def lib():
         L.mc_repair_nonneg.argtypes = [P_, C.c_int]
         L.mc_particle_terms_reset.argtypes = [P_]
         L.mc_particle_terms_add.argtypes = [P_, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int64]
+        L.mc_particle_terms_add_noalloc.argtypes = [P_, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.c_int64]
         L.mc_interpolate_to_nodes.argtypes = [P_, C.POINTER(C.c_uint64), C.c_int64, C.POINTER(C.c_uint64), C.c_int64,
                                               C.POINTER(C.c_double)]
         L.mc_get_phi_gradient.argtypes = [P_, C.c_int, C.c_int]
@@ -239,6 +240,16 @@ class Context:
         self._check(self.L.mc_particle_terms_add(self.h, cells.ctypes.data_as(C.POINTER(C.c_uint64)),
                                                  terms.ctypes.data_as(C.POINTER(C.c_double)), cells.shape[0]),
                     "mc_particle_terms_add")
+
+    def particle_terms_add_noalloc(self, cells, terms):
+        """particle_terms_add through the entry point that allocates no device memory."""
+        cells = np.ascontiguousarray(cells, dtype=np.uint64)
+        terms = np.ascontiguousarray(terms, dtype=np.float64).reshape(-1, 5)
+        assert terms.shape[0] == cells.shape[0]
+        self._check(self.L.mc_particle_terms_add_noalloc(self.h, cells.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                         terms.ctypes.data_as(C.POINTER(C.c_double)),
+                                                         cells.shape[0]),
+                    "mc_particle_terms_add_noalloc")
 
     def interpolate_to_nodes(self, cells, nodes):
         """(len(nodes), 5) flow_aos rows (vel xyz, pressure, temp)."""
