@@ -9,12 +9,17 @@
 //   public compute_time                                                 (:277-282)
 //
 // Everything below the class goes through the C-ABI of include/mcflow.h (one
-// context = one GPU).  One flow rank owns the whole mesh, so the MPI_Config
-// argument has no counterpart; the mesh is either the reference's box
-// (modifier m: 2m x m x m cells, generated on the device) or host arrays in
-// the layout of the reference Mesh<T> (mc_mesh_desc).  As in the reference
-// (gpuErrchk, include/flow/gpu/gpu_kernels.cu:24-32), a failed call prints the
-// error and exits.
+// context = one GPU).  The reference constructor takes the reference's own
+// MPI_Config and Mesh<T> objects as they are (template parameters, so this
+// header needs none of the reference's headers); one flow rank owns the whole
+// mesh, so a mesh split over several flow ranks is refused.  Two more
+// constructors take the reference's box (modifier m: 2m x m x m cells,
+// generated on the device) or host arrays in the Mesh<T> layout
+// (mc_mesh_desc).  The alias minicombust::flow::FlowSolver at the end makes
+// the class the one src/minicombust.cpp names after its
+// `using namespace minicombust::flow`.  As in the reference (gpuErrchk,
+// include/flow/gpu/gpu_kernels.cu:24-32), a failed call prints the error and
+// exits.
 #pragma once
 
 #include <chrono>
@@ -35,6 +40,37 @@ class FlowSolver {
 
   public:
     double compute_time = 0.0;   // seconds inside timestep(), as the reference's
+
+    // FlowSolver(MPI_Config *, Mesh<T> *, double delta, FILE *)
+    // (include/flow/gpu/FlowSolver.hpp:274, include/flow/FlowSolver.hpp:147),
+    // as src/minicombust.cpp:174 calls it after load_mesh (:134).  The mesh
+    // members read are those of include/geometry/Mesh.hpp:70-107: sizes,
+    // points (vec<T>, three doubles), cells (8 vertex ids), cell_neighbours,
+    // faces (Face<uint64_t>: cell0, cell1), cell_faces, boundary_types.  The
+    // arrays are copied to the device here; the mesh object stays the caller's.
+    template <class MPIConfig, class MeshT>
+    FlowSolver(MPIConfig *mpi_config, MeshT *mesh, double delta, FILE *output_file, const mc_options *opts = nullptr)
+        : out_(output_file ? output_file : stdout)
+    {
+        static_assert(sizeof(*mesh->points) == 3 * sizeof(double), "Mesh<T>::points must be vec<double>");
+        static_assert(sizeof(*mesh->faces) == 2 * sizeof(uint64_t), "Mesh<T>::faces must be Face<uint64_t>");
+        if (mpi_config && mpi_config->particle_flow_world_size != 1)
+            fail("FlowSolver: one flow rank owns the whole mesh on one GPU (particle_flow_world_size must be 1)");
+        if (mesh->local_mesh_size != mesh->mesh_size || mesh->cell_size != 8 || mesh->faces_per_cell != 6)
+            fail("FlowSolver: the mesh must be the whole hexahedral mesh of one flow rank");
+        mc_mesh_desc d{};
+        d.n_cells = (int64_t)mesh->mesh_size;
+        d.n_faces = (int64_t)mesh->faces_size;
+        d.n_points = (int64_t)mesh->points_size;
+        d.points = reinterpret_cast<const double *>(mesh->points);
+        d.cells = mesh->cells;
+        d.cell_neighbours = mesh->cell_neighbours;
+        d.faces = reinterpret_cast<const uint64_t *>(mesh->faces);
+        d.cell_faces = mesh->cell_faces;
+        for (int s = 0; s < 6; s++) d.boundary_types[s] = (uint64_t)mesh->boundary_types[s];
+        mc_options o = options(delta, opts);
+        check(mc_ctx_create(&d, &o, &ctx_), "mc_ctx_create");
+    }
 
     // box mesh of the given modifier (src/minicombust.cpp:123-124)
     FlowSolver(uint64_t modifier, double delta, FILE *output_file, const mc_options *opts = nullptr)
@@ -122,6 +158,12 @@ class FlowSolver {
         return o;
     }
 
+    [[noreturn]] void fail(const char *msg)
+    {
+        fprintf(stderr, "%s\n", msg);
+        exit(EXIT_FAILURE);
+    }
+
     void check(int rc, const char *what)
     {
         if (rc != MC_OK) {
@@ -136,3 +178,12 @@ class FlowSolver {
 };
 
 }  // namespace minicombust_amd
+
+// The name src/minicombust.cpp uses (`using namespace minicombust::flow`,
+// `new FlowSolver<double>(&mpi_config, mesh, delta, stdout)` at :174).
+namespace minicombust {
+namespace flow {
+template <typename T>
+using FlowSolver = ::minicombust_amd::FlowSolver<T>;
+}  // namespace flow
+}  // namespace minicombust

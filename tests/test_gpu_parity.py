@@ -644,6 +644,47 @@ def test_cpp_driver_matches_the_library(tmp_path):
     assert drv == ours
 
 
+@pytest.mark.parametrize("mode", [po.MODE_COMPAT, po.MODE_SIMPLE])
+def test_reference_load_mesh_into_the_dropin(tmp_path, mode):
+    """The reference's own load_mesh output (Mesh<double>, built from the
+    reference's source into oracle/_ref/ref_flow_driver) handed unchanged to
+    the drop-in FlowSolver(&mpi_config, mesh, delta, stdout) under the name
+    src/minicombust.cpp uses: after 2 timesteps at modifier 8 the fields and
+    mass fluxes equal the generated-box context's bit for bit and the
+    oracle's within 1e-8."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                       "ref_flow_driver")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ref_flow_driver is built only where the reference sources are present")
+    m, steps = 8, 2
+    name = "simple" if mode == po.MODE_SIMPLE else "compat"
+    out_bin = tmp_path / "fields.bin"
+    print("reference load_mesh -> drop-in FlowSolver m=%d %s" % (m, name), flush=True)
+    out = subprocess.run([exe, str(m), str(steps), name, str(out_bin)], cwd=str(tmp_path), capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Flow Compute Time" in out.stdout
+    raw = out_bin.read_bytes()
+    N, F = np.frombuffer(raw[:16], dtype=np.int64)
+    vals = np.frombuffer(raw[16:], dtype=np.float64)
+    assert vals.size == 11 * N + F
+    kw = dict(mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=5000, warm_start=0)
+    o = po.Oracle(m, mode=mode, mom_rtol=1e-12, p_rtol=1e-12, p_maxit=100000)
+    with mc.Context(m, **kw) as g:
+        assert g.N == N and g.F == F
+        for _ in range(steps):
+            g.timestep()
+            o.timestep()
+        for k, f in enumerate(FIELDS_ALL):
+            drv = vals[k * N:(k + 1) * N]
+            assert_bits(drv, g.phi(f)[:N], "drop-in vs box field %s" % po.FIELD_NAMES[f])
+            assert_close(drv, o.phi(f)[:N], "drop-in vs oracle field %s" % po.FIELD_NAMES[f], 1e-8)
+        assert_bits(vals[11 * N:], g.get(mc.ARR_MFLUX), "drop-in vs box mass flux")
+    o.close()
+
+
 def test_error_paths_leave_the_context_usable():
     """Refusals come back as status codes with a message and change nothing:
     a box modifier out of range, options mc_set_options refuses (the previous
