@@ -7,8 +7,11 @@ FlowSolver.inl:3190-3267) on the box mesh of modifier m (2m x m x m hex cells;
 m = 128 -> 4,194,304 cells), in simple mode (pressure-correction system
 re-assembled and solved to 1e-8 relative residual every iteration), with every
 input already resident in HBM.  Each pressure-solve form is timed on its own
-context: FP64 PCG-AMG, and FP32 PCG-AMG inside FP64 iterative refinement.
-`value` is the iterations/s of the fastest form meeting the 1e-8 residual.
+context: FP64 PCG-AMG; FP32 PCG-AMG inside FP64 iterative refinement (the
+fine-level smoothing sweeps on an FP16 copy of D^-1 A); and that mixed form
+with the learned pressure-correction guess.  `value` is the iterations/s of
+the learned + mixed form, the form the metric names (`speedup_vs_fp64` beside
+it); the other forms' rates are listed under `forms`.
 
 A profiled pass (before the timed one) brackets every kernel class with HIP
 events on the context's stream (no synchronisation) and divides its
@@ -16,9 +19,15 @@ algorithmic bytes by its time.  The dominant single-kernel class is then timed
 again with HIP events inside the timed region itself: `roofline` is that
 kernel's algorithmic bytes per launch over its average launch duration,
 against the 8 TB/s HBM3E peak; `traffic` is its HBM bytes per launch from the
-FETCH_SIZE/WRITE_SIZE counter summary in profiles/ (null when absent).  `cpu_baseline` is the CPU
-oracle (one core) timed on a bounded sample of the same workload, run in a
-child process while the GPU is measured.
+FETCH_SIZE/WRITE_SIZE counter summary in profiles/ (null when absent), and
+the rates of both byte counts are reported against the peak and against the
+on-box copy probe.  Algorithmic bytes count each array a kernel touches once
+per launch; a class whose algorithmic rate comes out above the HBM peak is
+refused (listed under `refused_classes`, never the roofline kernel): its
+byte count or its cache residency is wrong.  `cpu_baseline` is the CPU oracle
+(one core) timed on a bounded sample of the same workload (3 SIMPLE
+iterations, BiCGStab started as the GPU forms start it), run in a child
+process after the GPU forms are measured.
 """
 import argparse
 import json
@@ -73,8 +82,15 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
                 kern[name]["achieved_GBs"] = None
             else:
                 kern[name]["achieved_GBs"] = rate / 1e9 if rate else None
+    refused = {k: v["achieved_GBs"] for k, v in kern.items()
+               if v.get("achieved_GBs") and v["achieved_GBs"] > HBM_PEAK_GBS}
+    for k in refused:
+        print("# class %s refused: algorithmic rate %.0f GB/s above the HBM peak" % (k, refused[k]),
+              file=sys.stderr, flush=True)
+        kern[k]["refused"] = "algorithmic rate above the HBM peak"
     if roof_class is None and kern:
-        cands = [(v["ms_per_step"], k) for k, v in kern.items() if v["achieved_GBs"] and k in mc.KC_KERNELS]
+        cands = [(v["ms_per_step"], k) for k, v in kern.items()
+                 if v["achieved_GBs"] and k in mc.KC_KERNELS and k not in refused]
         roof_class = max(cands)[1] if cands else None
     # timed pass: only the roofline kernel's launches carry events
     ctx.sync()
@@ -108,10 +124,12 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
         "learned_guess_residual_ratio": (st["p_guess_ratio_sum"] / st["p_guesses"]) if st["p_guesses"] else None,
         "scalar_iters_per_step": st["scalar_iters"] / steps,
         "kernels": kern,
+        "refused_classes": refused or None,
         "roof_live": live,
         # final fields after the same iteration sequence in every form
         "fields": {name: ctx.phi(f)[:ctx.N].copy() for name, f in FIELDS},
         "memory": ctx.memory(),
+        "amg_half": bool(ctx.opts.amg_half) and psolve == mc.PSOLVE_MIXED,
     }
     if use_net and "mpnn_fwd" not in kern:
         # the measured policy can leave the guess off through the profiled steps:
@@ -182,15 +200,27 @@ def traffic_of(kernel, m):
     return None
 
 
-def roofline_of(live, m):
+def roofline_of(live, m, probe):
+    """achieved = algorithmic bytes per launch / average launch time; the
+    measured traffic's rate beside it, each against the HBM peak and the
+    copy probe's rate."""
     if not live:
         return None
-    gbs = live["bytes_per_launch"] / (live["avg_launch_ms"] * 1e-3) / 1e9
+    sec = live["avg_launch_ms"] * 1e-3
+    gbs = live["bytes_per_launch"] / sec / 1e9
+    traffic = traffic_of(live["kernel"], m)
+    tgbs = traffic / sec / 1e9 if traffic else None
+    pr = probe["GBs"] if probe else None
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_of(live["kernel"], m),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": live["kernel"], "kernel_class": live["kernel_class"],
             "bytes_per_launch": live["bytes_per_launch"], "avg_launch_ms": round(live["avg_launch_ms"], 5),
-            "launches_timed": live["launches"]}
+            "launches_timed": live["launches"],
+            "frac_of_copy_probe": round(gbs / pr, 4) if pr else None,
+            "traffic_over_algorithmic": round(traffic / live["bytes_per_launch"], 3) if traffic else None,
+            "traffic_GBs": round(tgbs, 1) if tgbs else None,
+            "traffic_frac": round(tgbs / HBM_PEAK_GBS, 4) if tgbs else None,
+            "traffic_frac_of_copy_probe": round(tgbs / pr, 4) if (tgbs and pr) else None}
 
 
 def copy_probe(mc, mb, reps):
@@ -200,8 +230,9 @@ def copy_probe(mc, mb, reps):
     nbytes = mb << 20
     ms = ctx.copy_probe(nbytes, reps)
     ctx.close()
+    gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9
     return {"kernel": "mc::k_copy_probe", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
-            "avg_launch_ms": round(ms, 5), "GBs": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)}
+            "avg_launch_ms": round(ms, 5), "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def main():
@@ -213,6 +244,7 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed,learned_mixed")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-m", type=int, default=128)
+    ap.add_argument("--cpu-iters", type=int, default=3, help="timed SIMPLE iterations of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-class", default=None, help="kernel class for the roofline (default: dominant)")
     ap.add_argument("--copy-probe-mb", type=int, default=1024, help="size of the HBM copy probe (MiB)")
@@ -224,11 +256,6 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-
-    cpu_proc = None
-    if not args.no_cpu_baseline:
-        cpu_proc = subprocess.Popen([sys.executable, "-m", "oracle.cpu_bench", "--m", str(args.cpu_sample_m)],
-                                    cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
 
     from minicombust_app_amd import mcflow as mc
     from minicombust_app_amd import learned
@@ -254,20 +281,32 @@ def main():
     warm = (ws[-1] > 0 or (ws[-1] < 0 and args.mode == "simple")) if ws else args.mode == "simple"
     probe = copy_probe(mc, args.copy_probe_mb, 10) if args.copy_probe_mb > 0 else None
 
+    # the metric's form: learned + mixed; without it (weights missing or the
+    # form not asked for) the fastest form that met the residual, named in config
     ok = {k: v for k, v in forms.items() if v["last_rel_residual"] <= 1e-8}
-    head = max(ok or forms, key=lambda k: forms[k]["iter_per_s"])
+    head = "learned_mixed" if "learned_mixed" in forms else max(ok or forms, key=lambda k: forms[k]["iter_per_s"])
     h = forms[head]
 
     cpu = None
-    if cpu_proc is not None:
+    if not args.no_cpu_baseline:
+        print("# cpu baseline: oracle at m=%d" % args.cpu_sample_m, file=sys.stderr, flush=True)
         try:
-            so, se = cpu_proc.communicate(timeout=900)
-            cpu = json.loads(so.strip().splitlines()[-1])
+            out = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", "--m", str(args.cpu_sample_m),
+                                  "--iters", str(args.cpu_iters)], cwd=ROOT, capture_output=True, text=True,
+                                 timeout=900)
+            cpu = json.loads(out.stdout.strip().splitlines()[-1])
         except Exception as e:  # report, never fake
             cpu = {"error": "cpu baseline failed: %s" % e}
 
     N = 2 * args.m ** 3
-    dtype = {"fp64": "f64", "mixed": "f64+f32", "learned_mixed": "f64+f32+bf16", "learned_fp64": "f64+bf16"}[head]
+    half = h["amg_half"]
+    dtype = {"fp64": "f64", "mixed": "f64+f32" + ("+f16" if half else ""),
+             "learned_mixed": "f64+f32" + ("+f16" if half else "") + "+bf16", "learned_fp64": "f64+bf16"}[head]
+    precision = {"fp64": "FP64 everywhere",
+                 "mixed": "FP64 refinement residual, update and coarsest solve; FP32 PCG, SpMV and AMG levels"
+                          + ("; fine-level smoothing sweeps read an FP16 copy of D^-1 A" if half else ""),
+                 "learned_fp64": "FP64 solve; net BF16 weights and MFMA operands, FP32 accumulation"}
+    precision["learned_mixed"] = precision["mixed"] + "; net BF16 weights and MFMA operands, FP32 accumulation"
     line = {
         "metric": METRIC,
         "value": round(h["iter_per_s"], 3),
@@ -284,7 +323,8 @@ def main():
         "config": {"workload": "box mesh modifier %d (%d hex cells), %s mode, pressure rel. residual 1e-8, "
                                "momentum/scalar BiCGStab rtol 1e-5 (||r|| <= rtol ||b||, %s)"
                                % (args.m, N, args.mode, "started from the current fields" if warm else "started from zero"),
-                   "modifier": args.m, "cells": N, "form": head, "option_overrides": args.opt or None},
+                   "modifier": args.m, "cells": N, "form": head, "precision": precision[head],
+                   "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
                       "learned_guesses_per_step": v["learned_guesses_per_step"],
@@ -297,7 +337,8 @@ def main():
         "speedup_vs_fp64": (round(h["iter_per_s"] / forms["fp64"]["iter_per_s"], 3) if "fp64" in forms else None),
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in h["kernels"].items()},
-        "roofline": roofline_of(h["roof_live"], args.m),
+        "refused_classes": h["refused_classes"],
+        "roofline": roofline_of(h["roof_live"], args.m, probe),
         "cpu_baseline": cpu,
         "hbm_copy_probe": probe,
     }

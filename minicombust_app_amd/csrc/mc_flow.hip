@@ -115,6 +115,20 @@ __device__ __forceinline__ CellFaces cell_faces_of(const Geo &g, int c)
     return r;
 }
 
+// ------------------------------------------------- algorithmic bytes --
+// Bytes of one launch with every array the kernel reads or writes counted
+// once (a neighbour's value read again is free): FP64 values, int32 indices,
+// per-cell arrays x N, arrays of all faces x F, of interior faces only x F_i,
+// of boundary faces only x F_b, matrix entries x nnz (the entries a row
+// stores: the PETSc pattern).  Each kernel's table lists its arrays.
+struct Sizes {
+    double N, F, Fi, Fb, nnz;
+};
+static Sizes sizes_of(const Ctx *c)
+{
+    return Sizes{(double)c->N, (double)c->F, (double)(c->F - c->Fb), (double)c->Fb, (double)c->nnz};
+}
+
 // ========================================================== A4 / A5 ==
 struct GradArgs {
     const double *phi[11];
@@ -256,11 +270,13 @@ __global__ void __launch_bounds__(256) k_limit(Geo g, GradArgs a)
     }
 }
 
-// LSQ bytes: 11 fields read (8N) and gradients written (24N), centres, face
-// centres of boundary faces, neighbour lists.
+// k_grad<NF>: per cell phi (8 NF) read, gradients (24 NF) written, centre
+// (24), neighbour list (24); per boundary face cell_faces and fc1 entries (8)
+// and the face centre (24).
 static double grad_bytes(Ctx *c, int nf)
 {
-    return 32.0 * nf * c->N + 48.0 * c->N + 8.0 * c->F + 24.0 * c->Fb;
+    const Sizes z = sizes_of(c);
+    return (32.0 * nf + 48.0) * z.N + 32.0 * z.Fb;
 }
 
 static GradArgs grad11_args(Ctx *c)
@@ -534,6 +550,18 @@ static int r0_buffers(Ctx *c)
     return MC_OK;
 }
 
+// k_momentum: per cell U, V, W (24), their gradients (72), grad P (24), TEM
+// (8), centre (24), volume and density (16), forder and fother (48) read;
+// A_U, S_U, S_V, S_W (32) and, warm-started, b - A x0 of the three systems
+// (24) written; per face mass flux, lambda, rlencos, normal (48) read; per
+// interior face ff0, ff1 (16) written; per boundary face centre and area
+// (32) read; the matrix entries (8 nnz) written.
+static double momentum_bytes(Ctx *c, bool r0)
+{
+    const Sizes z = sizes_of(c);
+    return (248.0 + (r0 ? 24.0 : 0.0)) * z.N + 48.0 * z.F + 16.0 * z.Fi + 32.0 * z.Fb + 8.0 * z.nnz;
+}
+
 int momentum_assemble(Ctx *c, bool r0)
 {
     MomArgs a = mom_args(c);
@@ -541,7 +569,7 @@ int momentum_assemble(Ctx *c, bool r0)
         MC_TRY(r0_buffers(c));
         for (int k = 0; k < 3; k++) a.r0[k] = c->r0buf[k];
     }
-    KTimer kt(c, MC_KC_MOM_ASM, 216.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
+    KTimer kt(c, MC_KC_MOM_ASM, momentum_bytes(c, a.r0[0] != nullptr));
     hipLaunchKernelGGL(k_momentum, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;
@@ -845,6 +873,26 @@ int set_up_field(Ctx *c)
     return MC_OK;
 }
 
+// The two launches of launch_mass, each array counted once per launch.
+// k_mass_faces: per cell gradients of U, V, W, P (96), centre (24), U, V, W,
+// P (32), density, A_U, volume (24), forder and fother (48) read; per
+// interior face lambda, centre, normal, area (64) read and mass flux, ff0,
+// ff1 (24) written.  k_mass_rows: per cell forder and fother (48) read, S_U,
+// A_V, Pressure_b (24) written; per face mass flux (8) and per interior face
+// ff0 (8) read; the CSR values (8 nnz) written; with the AMG fine level's
+// copies also the symmetric FP64 diagonals (8 (N + F_i)), the FP32 ones
+// (4 nnz), both inverse diagonals (12 N), and with the FP16 smoothing copy
+// the FP32 diagonal (4 N) and the FP16 off-diagonals (2 * 2 F_i).
+static double mass_bytes(Ctx *c, const MassArgs &a)
+{
+    const Sizes z = sizes_of(c);
+    double b = 224.0 * z.N + 88.0 * z.Fi;                  // k_mass_faces
+    b += 72.0 * z.N + 8.0 * z.F + 8.0 * z.Fi + 8.0 * z.nnz;  // k_mass_rows
+    if (a.pdia) b += 8.0 * (z.N + z.Fi) + 4.0 * z.nnz + 12.0 * z.N;
+    if (a.phoff) b += 4.0 * z.N + 4.0 * z.Fi;
+    return b;
+}
+
 int calculate_mass_flux_assemble(Ctx *c)
 {
     const bool into = c->first_pressure || c->opt.mode == MC_MODE_SIMPLE;
@@ -864,7 +912,7 @@ int calculate_mass_flux_assemble(Ctx *c)
         }
     }
     {
-        KTimer kt(c, MC_KC_P_ASM, 184.0 * c->N + 88.0 * c->F + 8.0 * c->nnz);
+        KTimer kt(c, MC_KC_P_ASM, mass_bytes(c, a));
         launch_mass(c, a);
     }
     MC_LAUNCH_CHECK(c);
@@ -1035,7 +1083,10 @@ int calculate_pressure(Ctx *c)
         MC_TRY(grad_single(c, PP, true));
         {
             {
-                KTimer kt(c, MC_KC_CORRECT, 56.0 * c->N + 88.0 * c->F);
+                // per cell centre, grad PP (48), PP (8), forder and fother (48)
+                // read; per interior face ff0, mass flux, centre, normal (64)
+                // read and mass flux, correction (16) written
+                KTimer kt(c, MC_KC_CORRECT, 104.0 * c->N + 80.0 * (double)(c->F - c->Fb));
                 hipLaunchKernelGGL(k_flux_correct, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c),
                                    c->phi[PP], c->gx[PP], c->gy[PP], c->gz[PP], c->ff0, c->mflux, c->fcorr);
             }
@@ -1483,8 +1534,17 @@ static int batch_solve(Ctx *c, bool tem)
     // kernel the extra accumulators cost more (168 VGPRs, +0.2 ms) than the pass
     const bool warm = warm_start(c);
     {
+        // per cell and field phi and gradient (32) read and S (8) written;
+        // per cell centre (24), volume and density (16), forder and fother
+        // (48) read, A (8) written; per face mass flux, lambda, rlencos,
+        // normal (48) read; per interior face ff0, ff1 (16) written; per
+        // boundary face centre (24) read; one matrix per equation system
+        // (8 nnz each) written
         const int nsc = SB + (tem ? 1 : 0);
-        KTimer kt(c, MC_KC_SCALAR_BATCH, 120.0 * nsc * c->N + 56.0 * c->F + 8.0 * c->nnz * (tem ? 2 : 1));
+        const Sizes z = sizes_of(c);
+        const double bytes = (40.0 * nsc + 96.0) * z.N + 48.0 * z.F + 16.0 * z.Fi + 24.0 * z.Fb +
+                             8.0 * z.nnz * (tem ? 2 : 1);
+        KTimer kt(c, MC_KC_SCALAR_BATCH, bytes);
         if (tem)
             hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
         else
@@ -1542,7 +1602,17 @@ int scalar_assemble(Ctx *c, int type, double *r0)
     case FUEL: case VARFU: a.inlet_phi = c->k.gas_ful; break;
     default: a.inlet_phi = 0.0; break;
     }
-    KTimer kt(c, MC_KC_SCALAR_ASM, 120.0 * c->N + 56.0 * c->F + 8.0 * c->nnz);
+    // per cell phi and gradient (32), centre (24), volume and density (16),
+    // forder and fother (48) read, A and S (16) written (r0: 8 more); TE also
+    // reads ED (8) and the velocity gradients (72) and writes TP (8); ED also
+    // reads TE, TP (16) and the wall flags (1); per face mass flux, lambda,
+    // rlencos, normal (48) read; per interior face ff0, ff1 (16) written; per
+    // boundary face centre (24) read; the matrix entries (8 nnz) written
+    const Sizes z = sizes_of(c);
+    double cell = 136.0 + (r0 ? 8.0 : 0.0);
+    if (type == TERBTE) cell += 88.0;
+    if (type == TERBED) cell += 17.0;
+    KTimer kt(c, MC_KC_SCALAR_ASM, cell * z.N + 48.0 * z.F + 16.0 * z.Fi + 24.0 * z.Fb + 8.0 * z.nnz);
     hipLaunchKernelGGL(k_scalar, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
     MC_LAUNCH_CHECK(c);
     return MC_OK;

@@ -103,11 +103,13 @@ __device__ __forceinline__ double dia_at(const Mat<T> &A, int d, int i)
     }
 }
 
-// bytes of one pass over a matrix: DIA values, or CSR values + columns + rows
+// bytes of one pass over a matrix: the stored diagonals (symmetric storage:
+// the main and upper ones), or CSR values + columns + row pointers
 template <typename T>
 static double mat_bytes(const Mat<T> &A, int64_t nnz, size_t vsize)
 {
-    return A.nd ? (double)vsize * A.nd * A.n : (double)(vsize + 4) * nnz + 4.0 * (A.n + 1);
+    if (A.nd) return (double)vsize * (A.sym ? A.nd / 2 + 1 : A.nd) * A.n;
+    return (double)(vsize + 4) * nnz + 4.0 * (A.n + 1);
 }
 
 
@@ -1764,8 +1766,9 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         T *dst = last ? x : oth;
         if (dst == cur) dst = tmp;
         const T *dotv = (last && ks) ? b : nullptr;
-        // A, dinv, b, x read, x' written
-        const double jbytes = (l == 0) ? mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n : 0.0;
+        // A (FP16 copy: its six off-diagonals), dinv, b, x read, x' written
+        const double jbytes = (l != 0 || !c->profile) ? 0.0
+                              : half ? 28.0 * L.n : mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n;
         {
             KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
             if constexpr (sizeof(T) == 4) {
@@ -1783,6 +1786,42 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     }
     if (cur != x) MC_HIP(c, hipMemcpyAsync(x, cur, (size_t)L.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
     return MC_OK;
+}
+
+// Algorithmic bytes of one V-cycle as vcycle() runs it on the built
+// hierarchy, each array counted once per pass.  Per level of n rows (T the
+// value size, M one pass over its matrix, D its inverse diagonal):
+//   first two sweeps from zero   M + D + 2Tn   (b read, x written)
+//   each further sweep           M + D + 3Tn   (b, x read, x' written)
+//   residual                     M + 3Tn       (+ the FP32 diagonal and D on the FP16 copy)
+//   restriction                  4(n + nc + 1) + T(n + nc)
+//   prolongation                 4n + T(nc + 2n)
+// On the FP16 copy M is its six off-diagonals (12n) and D the FP32 inverse
+// diagonal; the dense coarsest solve reads its inverse (8 n^2) and b, writes x.
+template <typename T>
+static double vcycle_bytes(Ctx *c)
+{
+    const int nu = std::max(1, c->opt.amg_nu);
+    const double t = (double)sizeof(T);
+    double b = 0.0;
+    for (size_t l = 0; l < c->amg.size(); l++) {
+        AmgLevel &L = c->amg[l];
+        const double n = L.n;
+        if (l + 1 == c->amg.size()) {
+            b += 8.0 * n * n + 2.0 * t * n;
+            break;
+        }
+        const bool half = sizeof(T) == 4 && L.hoff != nullptr && c->opt.amg_half;
+        const double M = half ? 12.0 * n : mat_bytes(level_mat<T>(L), L.nnz, sizeof(T));
+        const double D = half ? 4.0 * n : t * n;
+        b += (nu >= 2 ? M : 0.0) + D + 2.0 * t * n;
+        b += (double)std::max(0, nu - 2) * (M + D + 3.0 * t * n);
+        b += M + (half ? 8.0 * n : 0.0) + 3.0 * t * n;
+        b += 4.0 * (n + L.nc + 1) + t * (n + L.nc);
+        b += 4.0 * n + t * (L.nc + 2.0 * n);
+        b += (double)nu * (M + D + 3.0 * t * n);
+    }
+    return b;
 }
 
 // ------------------------------------------------------------ PCG --
@@ -1943,7 +1982,7 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
     for (;;) {
         for (int j = 0; j < batch && it < maxit; j++, it++) {
             {
-                KTimer kt(c, MC_KC_VCYCLE, 0.0);
+                KTimer kt(c, MC_KC_VCYCLE, c->profile ? vcycle_bytes<T>(c) : 0.0);
                 MC_TRY(vcycle<T>(c, 0, w.r, w.z, ks, ks));
             }
             {
@@ -2293,13 +2332,30 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
 }
 
 // ------------------------------------------------------ copy probe --
-// Plain streaming copy, 16 B per lane per access (grid-stride): the known-size
-// kernel for HBM bandwidth and counter calibration.
+// Streaming copy, the known-size kernel for HBM bandwidth and counter
+// calibration: each lane issues COPY_UNROLL independent 16-byte loads (one
+// per 256-lane stride of its block's span) before any of its stores, so
+// several loads per lane are in flight; every element is read once and
+// written once.
+constexpr int COPY_UNROLL = 8;
 __global__ void __launch_bounds__(256) k_copy_probe(int64_t n16, const double2 *__restrict__ src,
                                                    double2 *__restrict__ dst)
 {
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    const int64_t span = (int64_t)256 * COPY_UNROLL;
+    const int64_t stride = (int64_t)gridDim.x * span;
+    for (int64_t base = (int64_t)blockIdx.x * span + threadIdx.x; base < n16; base += stride) {
+        double2 v[COPY_UNROLL];
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n16) v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < COPY_UNROLL; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n16) dst[i] = v[u];
+        }
+    }
 }
 
 int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out)
@@ -2314,7 +2370,7 @@ int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out)
     int rc = MC_OK;
     hipEvent_t a = nullptr, b = nullptr;
     const int64_t n16 = bytes / 16;
-    const int blocks = (int)std::min<int64_t>(div_up(n16, 256), 256 * 64);
+    const int blocks = (int)std::min<int64_t>(div_up(n16, 256 * COPY_UNROLL), 256 * 16);
     if (hipMemsetAsync(src, 0, (size_t)bytes, c->stream) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
         hipEventCreate(&b) != hipSuccess) {
         rc = MC_ERR_HIP;
