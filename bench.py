@@ -45,11 +45,18 @@ FIELDS = [("U", 0), ("V", 1), ("W", 2), ("P", 3), ("TE", 5), ("ED", 6), ("TEM", 
 HBM_PEAK_GBS = 8000.0
 
 
+# Loop-1 pressure solves of the learned form start from the Galerkin projection
+# onto W = [net guess, last 2 loop-1 solutions] (deflation space); the
+# recycled form is the same without the net (what the net adds on its own).
+RECYCLE = 2
+
+
 def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=None, mode="simple"):
     psolve = mc.PSOLVE_MIXED if "mixed" in form else mc.PSOLVE_FP64
     use_net = form.startswith("learned")
+    recycle = RECYCLE if (use_net or form.startswith("recycled")) else 0
     opts = dict(mode=mc.MODE_SIMPLE if mode == "simple" else mc.MODE_COMPAT, psolve=psolve, p_rtol=1e-8,
-                mom_rtol=1e-5, learned=1 if use_net else 0)
+                mom_rtol=1e-5, learned=1 if use_net else 0, p_recycle=recycle)
     opts.update(extra or {})
     ctx = mc.Context(m, **opts)
     if use_net:
@@ -122,6 +129,9 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
         "momentum_iters_per_step": st["mom_iters"] / steps,
         "learned_guesses_per_step": st["p_guesses"] / steps,
         "learned_guess_residual_ratio": (st["p_guess_ratio_sum"] / st["p_guesses"]) if st["p_guesses"] else None,
+        "deflation_starts_per_step": st["p_defl_starts"] / steps,
+        "deflation_start_residual_ratio": (st["p_defl_ratio_sum"] / st["p_defl_starts"]) if st["p_defl_starts"] else None,
+        "deflation_vectors_per_start": (st["p_defl_vectors"] / st["p_defl_starts"]) if st["p_defl_starts"] else None,
         "scalar_iters_per_step": st["scalar_iters"] / steps,
         "kernels": kern,
         "refused_classes": refused or None,
@@ -241,7 +251,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--m", type=int, default=128)
-    ap.add_argument("--forms", default="fp64,mixed,learned_mixed")
+    ap.add_argument("--forms", default="fp64,mixed,recycled_mixed,learned_mixed")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-m", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=3, help="timed SIMPLE iterations of the CPU baseline")
@@ -301,12 +311,15 @@ def main():
     N = 2 * args.m ** 3
     half = h["amg_half"]
     dtype = {"fp64": "f64", "mixed": "f64+f32" + ("+f16" if half else ""),
+             "recycled_mixed": "f64+f32" + ("+f16" if half else ""),
              "learned_mixed": "f64+f32" + ("+f16" if half else "") + "+bf16", "learned_fp64": "f64+bf16"}[head]
     precision = {"fp64": "FP64 everywhere",
                  "mixed": "FP64 refinement residual, update and coarsest solve; FP32 PCG, SpMV and AMG levels"
                           + ("; fine-level smoothing sweeps read an FP16 copy of D^-1 A" if half else ""),
                  "learned_fp64": "FP64 solve; net BF16 weights and MFMA operands, FP32 accumulation"}
-    precision["learned_mixed"] = precision["mixed"] + "; net BF16 weights and MFMA operands, FP32 accumulation"
+    precision["recycled_mixed"] = precision["mixed"] + "; FP64 deflation-space start"
+    precision["learned_mixed"] = (precision["mixed"] + "; FP64 deflation-space start; net BF16 weights and MFMA "
+                                  "operands, FP32 accumulation")
     line = {
         "metric": METRIC,
         "value": round(h["iter_per_s"], 3),
@@ -324,11 +337,19 @@ def main():
                                "momentum/scalar BiCGStab rtol 1e-5 (||r|| <= rtol ||b||, %s)"
                                % (args.m, N, args.mode, "started from the current fields" if warm else "started from zero"),
                    "modifier": args.m, "cells": N, "form": head, "precision": precision[head],
+                   "pressure_start": {"fp64": "zero", "mixed": "zero",
+                                      "recycled_mixed": "Galerkin projection onto the last %d loop-1 solutions"
+                                                        % RECYCLE,
+                                      "learned_mixed": "Galerkin projection onto [net guess (while it pays), last %d "
+                                                       "loop-1 solutions]" % RECYCLE}.get(head),
                    "option_overrides": args.opt or None},
         "forms": {k: {"iter_per_s": round(v["iter_per_s"], 3), "pressure_ms_per_step": round(v["pressure_ms_per_step"], 3),
                       "pressure_iters_per_step": v["pressure_iters_per_step"],
                       "learned_guesses_per_step": v["learned_guesses_per_step"],
                       "learned_guess_residual_ratio": v["learned_guess_residual_ratio"],
+                      "deflation_starts_per_step": v["deflation_starts_per_step"],
+                      "deflation_start_residual_ratio": v["deflation_start_residual_ratio"],
+                      "deflation_vectors_per_start": v["deflation_vectors_per_start"],
                       "last_rel_residual": v["last_rel_residual"]} for k, v in forms.items()},
         "forms_skipped": skipped or None,
         "field_rel_error_vs_fp64": field_errors(forms),

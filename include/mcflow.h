@@ -107,6 +107,12 @@ typedef struct mc_options {
     int    amg_tail;       /* 1: the V-cycle from the first level of at most 2048
                               rows down runs in one workgroup (same results, fewer
                               launches); 0: one launch per pass                  */
+    int    p_recycle;      /* 0-4: the last p_recycle loop-1 pressure solutions join
+                              the deflation space W of the next loop-1 solve, which
+                              starts from the Galerkin projection x0 = W y,
+                              (W^T A W) y = W^T b (with the learned guess in W when
+                              `learned` uses it); 0: no recycling (the reference's
+                              x0 = 0 unless the learned guess is used)           */
 } mc_options;
 
 typedef struct mc_solve_status {
@@ -133,6 +139,11 @@ typedef struct mc_stats {
     int64_t host_polls;       /* host waits on solver convergence flags          */
     int64_t p_guesses;        /* pressure solves started from the learned guess  */
     double  p_guess_ratio_sum;/* sum over those of ||b - A x0|| / ||b - mean b||  */
+    int64_t p_defl_starts;    /* loop-1 pressure solves started from the deflation
+                                 space's Galerkin projection (learned guess and/or
+                                 recycled solutions), the start accepted           */
+    double  p_defl_ratio_sum; /* sum over those of ||b - A x0|| / ||b - mean b||  */
+    int64_t p_defl_vectors;   /* sum over those of the vectors W held            */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>
@@ -161,8 +172,11 @@ enum {
     MC_ARR_A_VALS, MC_ARR_PA_VALS, MC_ARR_PB,                        /* f64        */
     MC_ARR_FACES = 72, MC_ARR_CELL_FACES, MC_ARR_CELL_NEIGHBOURS,    /* u64        */
     MC_ARR_CELLS, MC_ARR_POINTS, MC_ARR_CELLS_PER_POINT,             /* u64/f64/u8 */
-    MC_ARR_PARTICLE_TERMS = 80                                       /* N x 5 f64: particle_aos
+    MC_ARR_PARTICLE_TERMS = 80,                                      /* N x 5 f64: particle_aos
                                                                         (momentum xyz, energy, fuel) */
+    MC_ARR_PDEFL = 84                                                /* + k (0-3): the k-th newest
+                                                                        recycled loop-1 pressure
+                                                                        solution, N f64, read-only */
 };
 
 /* Kernel classes timed with HIP events when profiling is on.  Every class but

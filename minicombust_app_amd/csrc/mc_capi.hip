@@ -205,6 +205,7 @@ int mc_options_default(mc_options *o)
     o->warm_start = -1;
     o->amg_half = 1;
     o->amg_tail = 1;
+    o->p_recycle = 0;
     return MC_OK;
 }
 
@@ -229,6 +230,7 @@ static int validate(const mc_options *o, std::string &err)
     if (o->cell_ychunk < 0 || o->cell_ychunk > 4096) { err = "cell_ychunk must be in [0, 4096]"; return MC_ERR_ARG; }
     if (o->warm_start < -1 || o->warm_start > 1) { err = "warm_start must be -1, 0 or 1"; return MC_ERR_ARG; }
     if (o->amg_half != 0 && o->amg_half != 1) { err = "amg_half must be 0 or 1"; return MC_ERR_ARG; }
+    if (o->p_recycle < 0 || o->p_recycle > DEFL_MAX) { err = "p_recycle must be in [0, 4]"; return MC_ERR_ARG; }
     return MC_OK;
 }
 
@@ -354,6 +356,8 @@ int mc_reset_fields(mc_ctx *x)
     c->first_pressure = true;
     c->field_set_up = false;
     c->timestep_count = 0;
+    c->defl_count = 0;   // the recycled solutions belong to the previous run
+    c->defl_head = 0;
     return MC_OK;
 }
 
@@ -504,6 +508,15 @@ static int describe(Ctx *c, int id, ArrDesc &d)
     if (id >= MC_ARR_GRAD && id < MC_ARR_GRAD + MC_NFIELDS) {
         const int f = id - MC_ARR_GRAD;
         d.kind = 1; d.count = NS * 3; d.sx = c->gx[f]; d.sy = c->gy[f]; d.sz = c->gz[f];
+        return MC_OK;
+    }
+    if (id >= MC_ARR_PDEFL && id < MC_ARR_PDEFL + DEFL_MAX) {
+        const int k = id - MC_ARR_PDEFL;
+        if (k >= c->defl_count || c->defl_cap != c->opt.p_recycle) {
+            c->err = "no recycled pressure solution in that slot";
+            return MC_ERR_STATE;
+        }
+        d.kind = 5; d.ptr = c->defl_x[(c->defl_head - k + c->defl_cap) % c->defl_cap]; d.count = c->N;
         return MC_OK;
     }
     switch (id) {

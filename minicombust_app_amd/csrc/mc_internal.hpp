@@ -42,6 +42,7 @@ struct Consts {
 constexpr int DIA_MAX = 32;     // most diagonals a diagonal-storage matrix may have
 constexpr int RED_PARTIALS = 16 * 8192;   // doubles in the reduction partials buffer
 constexpr int DENSE_MAX = 128;  // largest coarsest AMG level (inverted in LDS)
+constexpr int DEFL_MAX = 4;     // recycled pressure solutions in the deflation space
 
 // Per-level storage of the aggregation-AMG hierarchy.
 struct AmgLevel {
@@ -183,12 +184,20 @@ struct Ctx {
     double *amg_dense = nullptr;   // [n*n] coarsest (regularised) inverse
     double *amg_work = nullptr;
 
+    // ---- deflation space of loop-1 pressure solves (mc_linalg.hip) ----
+    // the last p_recycle loop-1 solutions, mean-zero (a ring of defl_cap
+    // slots, defl_x[defl_head] the newest), and A W of the current system
+    // (allocated on first use)
+    double *defl_x[DEFL_MAX] = {};
+    double *defl_aw[DEFL_MAX + 1] = {};
+    int defl_count = 0, defl_head = 0, defl_cap = 0;
+
     // ---- learned initial guess ----
     void *mpnn = nullptr;
     // measured use policy (loop-1 pressure solves): running means of the
     // iterations with / without the guess, the guess's cost and the cost of
     // one Krylov iteration; the guess stays on while it pays for itself
-    double lg_it_on = -1.0, lg_it_off = -1.0, lg_cost_ms = -1.0, lg_iter_ms = -1.0;
+    double lg_it_on = -1.0, lg_it_off = -1.0, lg_cost_ms = -1.0, lg_iter_ms = -1.0, lg_base_ms = -1.0;
     int64_t lg_count = 0;
     int64_t lg_last = 0;       // solve count of the last re-check of the other choice
     int lg_period = 16;        // solves between re-checks (doubles while they agree)

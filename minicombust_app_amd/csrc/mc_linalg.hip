@@ -2166,90 +2166,294 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     return MC_OK;
 }
 
-// ------------------------------------------------------ learned guess --
-// x0 = alpha g with g the net's mean-zero guess and alpha = (g, b) / (g, A g):
-// the A-norm-optimal multiple of g (a one-vector Galerkin / deflation step).
-// Kept only if it lowers the residual below ||b - mean b||; otherwise x0 = 0.
-__global__ void __launch_bounds__(BLOCK) k_guess_dots(int n, const double *b, const double *gv, const double *Ag,
-                                                      double *partials, int chunk)
+// ------------------------------------------------ deflation-space start --
+// A loop-1 pressure solve starts from the Galerkin projection of its solution
+// onto the deflation space W = [g, x_1, ..., x_s]: g the learned net's guess
+// (when the learned form uses it for this solve), x_1 .. x_s the last
+// s = p_recycle loop-1 solutions (newest first), every vector mean-zero.
+// x0 = W y with (W^T A W) y = W^T b is the vector of span W closest to the
+// solution in the A-norm, so the error left to the PCG is A-orthogonal to W.
+// The K x K system (K <= 5) is solved by Cholesky on one thread; a vector
+// whose pivot falls to 1e-10 of its diagonal entry (numerically in the span
+// of the earlier ones: successive solutions are nearly parallel), or is not a
+// number (a net output with NaN / Inf fails every comparison), is dropped.
+// The start is kept only if ||b - A x0|| < ||b - mean b||, else x0 = 0, so the
+// solve never starts worse than the reference's zero start.  The PCG-AMG and
+// its residual test are unchanged: W can change the iteration count, never
+// the tolerance met.
+constexpr int DEFL_W = DEFL_MAX + 1;   // the learned guess and the recycled solutions
+
+struct DeflVecs {
+    const double *w[DEFL_W];
+    double *aw[DEFL_W];
+};
+
+// aw[k0 + k] = A w[k0 + k] for NR vectors per matrix read
+template <int ND, int NR>
+__global__ void __launch_bounds__(BLOCK) k_defl_aw(Mat<double> A, DeflVecs v, int k0, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-        acc[0] += b[i];
-        acc[1] += gv[i] * b[i];
-        acc[2] += gv[i] * Ag[i];
-    }
-    block_partials<3>(acc, partials, bb, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return v.w[k0 + k][j]; },
+                     [&](int row, const double (&ax)[NR]) {
+#pragma unroll
+                         for (int k = 0; k < NR; k++) v.aw[k0 + k][row] = ax[k];
+                     });
 }
-struct EpiGuess1 {
+
+template <int K>
+constexpr int defl_nv() { return K * (K + 3) / 2 + 1; }   // E upper triangle, W^T b, sum b
+
+// partials of (w_a, A w_c) for a <= c, (w_a, b), and sum b
+template <int K>
+__global__ void __launch_bounds__(BLOCK) k_defl_dots(int n, DeflVecs v, const double *b, double *partials, int chunk)
+{
+    constexpr int NV = defl_nv<K>();
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) acc[q] = 0.0;
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        double w[K], aw[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) { w[k] = v.w[k][i]; aw[k] = v.aw[k][i]; }
+        const double bi = b[i];
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < K; a++)
+#pragma unroll
+            for (int c = a; c < K; c++) acc[q++] += w[a] * aw[c];
+#pragma unroll
+        for (int a = 0; a < K; a++) acc[q++] += w[a] * bi;
+        acc[q] += bi;
+    }
+    block_partials<NV>(acc, partials, bb, gridDim.x);
+}
+
+// sc[0] = mean b, sc[1 .. K] = y, sc[K + 1] = vectors kept
+template <int K>
+struct EpiDefl {
     double *sc;
     double inv_n;
-    __device__ void operator()(const double (&v)[3]) const
+    __device__ void operator()(const double (&v)[defl_nv<K>()]) const
     {
-        sc[0] = v[0] * inv_n;                              // mean b
-        sc[1] = (v[2] > 0.0) ? v[1] / v[2] : 0.0;          // alpha
+        double E[K][K], f[K], L[K][K], z[K], y[K];
+        bool keep[K];
+        int q = 0;
+        for (int a = 0; a < K; a++)
+            for (int c = a; c < K; c++) { E[a][c] = v[q]; E[c][a] = v[q]; q++; }
+        for (int a = 0; a < K; a++) f[a] = v[q++];
+        sc[0] = v[q] * inv_n;
+        int kept = 0;
+        for (int j = 0; j < K; j++) {
+            for (int i = 0; i < K; i++) L[i][j] = 0.0;
+            double d = E[j][j];
+            for (int k = 0; k < j; k++)
+                if (keep[k]) d -= L[j][k] * L[j][k];
+            keep[j] = E[j][j] > 0.0 && d > 1e-10 * E[j][j];
+            if (!keep[j]) continue;
+            kept++;
+            L[j][j] = sqrt(d);
+            for (int i = j + 1; i < K; i++) {
+                double t = E[i][j];
+                for (int k = 0; k < j; k++)
+                    if (keep[k]) t -= L[i][k] * L[j][k];
+                L[i][j] = t / L[j][j];
+            }
+        }
+        for (int i = 0; i < K; i++) {
+            z[i] = 0.0;
+            if (!keep[i]) continue;
+            double t = f[i];
+            for (int k = 0; k < i; k++)
+                if (keep[k]) t -= L[i][k] * z[k];
+            z[i] = t / L[i][i];
+        }
+        for (int i = K - 1; i >= 0; i--) {
+            y[i] = 0.0;
+            if (!keep[i]) continue;
+            double t = z[i];
+            for (int k = i + 1; k < K; k++)
+                if (keep[k]) t -= L[k][i] * y[k];
+            y[i] = t / L[i][i];
+        }
+        for (int i = 0; i < K; i++) sc[1 + i] = y[i];
+        sc[K + 1] = kept;
     }
 };
-__global__ void __launch_bounds__(BLOCK) k_guess_res(int n, const double *b, const double *Ag, const double *sc,
-                                                     double *partials, int chunk)
+
+// partials of ||b - mean||^2 and ||b - mean - A W y||^2
+template <int K>
+__global__ void __launch_bounds__(BLOCK) k_defl_res(int n, DeflVecs v, const double *b, const double *sc,
+                                                    double *partials, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    double y[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) y[k] = sc[1 + k];
+    const double mu = sc[0];
     double acc[2] = {0.0, 0.0};
-    const double mu = sc[0], al = sc[1];
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-        const double bt = b[i] - mu, r = bt - al * Ag[i];
+        const double bt = b[i] - mu;
+        double r = bt;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (y[k] != 0.0) r -= y[k] * v.aw[k][i];   // a dropped vector may hold NaN
         acc[0] += bt * bt;
         acc[1] += r * r;
     }
     block_partials<2>(acc, partials, bb, gridDim.x);
 }
-struct EpiGuess2 {
+
+// sc[K + 2] = ||b - mean||, sc[K + 3] = ||r0||, sc[K + 4] = 1 if the start is kept
+struct EpiDeflAccept {
     double *sc;
+    int K;
     __device__ void operator()(const double (&v)[2]) const
     {
-        sc[2] = sqrt(v[0]);                                // ||b - mean||
-        sc[3] = sqrt(v[1]);                                // ||r0||
-        sc[4] = (sc[3] < sc[2]) ? 1.0 : 0.0;               // accept
+        const double bn = sqrt(v[0]), rn = sqrt(v[1]);
+        sc[K + 2] = bn;
+        sc[K + 3] = rn;
+        sc[K + 4] = (sc[K + 1] > 0.0 && rn < bn) ? 1.0 : 0.0;
     }
 };
-__global__ void k_guess_set(int n, const double *gv, const double *sc, double *x)
+
+// x = W y if the start is kept, else 0
+template <int K>
+__global__ void k_defl_set(int n, DeflVecs v, const double *sc, double *x)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (sc[4] > 0.0) ? sc[1] * gv[i] : 0.0;
+    if (i >= n) return;
+    double s = 0.0;
+    if (sc[K + 4] > 0.0) {
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (sc[1 + k] != 0.0) s += sc[1 + k] * v.w[k][i];
+    }
+    x[i] = s;
 }
 
-static int learned_guess(Ctx *c, const double *b, double *x, bool *used, double *bnorm)
+// A grid for the deflation reductions: at most 21 partials per block.
+static Grid defl_grid(int n)
+{
+    Grid g = grid_for(n);
+    while (g.blocks > RED_PARTIALS / defl_nv<DEFL_W>() && g.chunk < (1 << 28)) {
+        g.chunk *= 2;
+        g.blocks = div_up(n, g.chunk);
+    }
+    return g;
+}
+
+static int defl_buffers(Ctx *c, int nrec)
+{
+    MemScope ms(c, MC_MEM_WORK);
+    for (int k = 0; k < nrec; k++)
+        if (!c->defl_x[k]) MC_TRY(dalloc(c, &c->defl_x[k], (size_t)c->N));
+    for (int k = 0; k <= nrec; k++)
+        if (!c->defl_aw[k]) MC_TRY(dalloc(c, &c->defl_aw[k], (size_t)c->N));
+    return MC_OK;
+}
+
+template <int K>
+static void defl_launch(Ctx *c, const DeflVecs &v, const double *b, double *x, double *sc)
+{
+    const int n = c->N;
+    const Grid g = defl_grid(n);
+    hipLaunchKernelGGL(k_defl_dots<K>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, b, c->red, g.chunk);
+    launch_finish<defl_nv<K>()>(c->stream, c->red, g.blocks, EpiDefl<K>{sc, 1.0 / n});
+    hipLaunchKernelGGL(k_defl_res<K>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, b, (const double *)sc, c->red,
+                       g.chunk);
+    launch_finish<2>(c->stream, c->red, g.blocks, EpiDeflAccept{sc, K});
+    hipLaunchKernelGGL(k_defl_set<K>, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, v, (const double *)sc, x);
+}
+
+// The start of a loop-1 solve: x set (x0 = W y or 0), *used = the start is
+// kept, *bnorm = ||b - mean b||; *guess = the learned guess is in W.
+static int deflation_start(Ctx *c, const double *b, double *x, bool want_guess, bool *used, double *bnorm,
+                           bool *guess)
 {
     *used = false;
+    *guess = false;
     *bnorm = 0.0;
-    if (!c->mpnn) return MC_OK;
     const int n = c->N;
-    double *gv = c->wk[6], *Ag = c->wk[7];
-    double *sc = c->dscal + 640;
-    int ok = 0;
-    MC_TRY(mpnn_guess(c, b, gv, &ok));
-    if (!ok) return MC_OK;
-    MC_TRY(project_mean_zero(c, gv));
+    const int nrec = (c->defl_cap == c->opt.p_recycle) ? std::min(c->opt.p_recycle, c->defl_count) : 0;
+    if (c->opt.p_recycle > 0 || want_guess) MC_TRY(defl_buffers(c, std::max(c->opt.p_recycle, 0)));
+    DeflVecs v{};
+    int K = 0;
+    if (want_guess && c->mpnn) {
+        int ok = 0;
+        double *gv = c->wk[6];
+        MC_TRY(mpnn_guess(c, b, gv, &ok));
+        if (ok) {
+            MC_TRY(project_mean_zero(c, gv));
+            v.w[K] = gv;
+            v.aw[K] = c->defl_aw[K];
+            K++;
+            *guess = true;
+        }
+    }
+    for (int k = 0; k < nrec; k++, K++) {
+        v.w[K] = c->defl_x[(c->defl_head - k + c->defl_cap) % c->defl_cap];
+        v.aw[K] = c->defl_aw[K];
+    }
+    if (K == 0) return MC_OK;
     const Grid g = grid_for(n);
     const Mat<double> A = level_mat<double>(c->amg[0]);
-    MC_DISPATCH_ND(nd_code(A), (k_spmv<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, (const double *)gv, Ag, g.chunk);
-    hipLaunchKernelGGL(k_guess_dots, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, (const double *)gv,
-                       (const double *)Ag, c->red, g.chunk);
-    launch_finish<3>(c->stream, c->red, g.blocks, EpiGuess1{sc, 1.0 / n});
-    hipLaunchKernelGGL(k_guess_res, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, (const double *)Ag,
-                       (const double *)sc, c->red, g.chunk);
-    launch_finish<2>(c->stream, c->red, g.blocks, EpiGuess2{sc});
-    hipLaunchKernelGGL(k_guess_set, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, (const double *)gv,
-                       (const double *)sc, x);
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int nr = std::min(4, K - k0);
+        if (nr == 4)
+            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 4>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
+        else if (nr == 3)
+            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 3>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
+        else if (nr == 2)
+            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 2>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
+        else
+            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 1>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
+    }
+    double *sc = c->dscal + 640;
+    switch (K) {
+    case 1: defl_launch<1>(c, v, b, x, sc); break;
+    case 2: defl_launch<2>(c, v, b, x, sc); break;
+    case 3: defl_launch<3>(c, v, b, x, sc); break;
+    case 4: defl_launch<4>(c, v, b, x, sc); break;
+    default: defl_launch<5>(c, v, b, x, sc); break;
+    }
     MC_LAUNCH_CHECK(c);
-    MC_HIP(c, hipMemcpyAsync(c->hscal + 640, sc, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MC_HIP(c, hipMemcpyAsync(c->hscal + 640, sc, (K + 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
-    *used = c->hscal[644] > 0.0;
-    *bnorm = c->hscal[642];
-    if (*used && c->hscal[642] > 0.0) c->st.p_guess_ratio_sum += c->hscal[643] / c->hscal[642];
+    const double *h = c->hscal + 640;
+    *used = h[K + 4] > 0.0;
+    *bnorm = h[K + 2];
+    if (*used) {
+        const double ratio = h[K + 2] > 0.0 ? h[K + 3] / h[K + 2] : 0.0;
+        c->st.p_defl_starts += 1;
+        c->st.p_defl_ratio_sum += ratio;
+        c->st.p_defl_vectors += (int64_t)h[K + 1];
+        if (*guess) c->st.p_guess_ratio_sum += ratio;
+    } else {
+        *guess = false;
+    }
+    return MC_OK;
+}
+
+// A converged loop-1 solution (mean-zero) joins the recycled solutions: a
+// ring of p_recycle slots (restarted when p_recycle changes).
+static int defl_push(Ctx *c, const double *x)
+{
+    const int cap = c->opt.p_recycle;
+    if (cap <= 0) return MC_OK;
+    MC_TRY(defl_buffers(c, cap));
+    if (c->defl_cap != cap) {
+        c->defl_cap = cap;
+        c->defl_count = 0;
+        c->defl_head = cap - 1;
+    }
+    const int slot = (c->defl_head + 1) % cap;
+    MC_HIP(c, hipMemcpyAsync(c->defl_x[slot], x, (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
+    c->defl_head = slot;
+    c->defl_count = std::min(c->defl_count + 1, cap);
     return MC_OK;
 }
 
@@ -2269,51 +2473,56 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     // while its measured saving (iterations x time per iteration) exceeds its
     // cost, re-checking the other choice after 16 solves, then after twice as
     // many each time a re-check confirms the choice (at most 1024)
-    bool recheck = false;
+    bool recheck = false, guess = false;
     if (c->opt.learned && c->mpnn && atol == 0.0) {
-        bool want = true;
+        tried = true;
         if (c->opt.learned == 1) {
-            if (c->lg_it_on < 0.0) want = true;
-            else if (c->lg_it_off < 0.0) want = false;
+            if (c->lg_it_on < 0.0) tried = true;
+            else if (c->lg_it_off < 0.0) tried = false;
             else {
                 recheck = c->lg_count - c->lg_last >= c->lg_period;
-                want = recheck ? !c->lg_on : c->lg_on;
+                tried = recheck ? !c->lg_on : c->lg_on;
             }
         }
-        if (want) {
-            MC_TRY(learned_guess(c, b, x, &used, &bn0));
-            tried = true;
-        }
     }
+    // loop-1 solves (atol == 0) start from the deflation space when it holds
+    // a vector; later loops solve the small non-orthogonal remainder from zero
+    if (atol == 0.0 && (tried || c->opt.p_recycle > 0))
+        MC_TRY(deflation_start(c, b, x, tried, &used, &bn0, &guess));
     const auto tg = std::chrono::steady_clock::now();
     if (c->opt.psolve == MC_PSOLVE_MIXED)
         rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used, bn0);
     else
         rc = pcg_amg(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used);
-    s.guess_used = used ? 1 : 0;
+    s.guess_used = guess ? 1 : 0;
     const auto t1 = std::chrono::steady_clock::now();
     c->st.p_solve_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (rc) return rc;
     if (c->opt.learned && c->mpnn && atol == 0.0) {
+        // the guess's cost: the start's time with it over the start's time
+        // without it (the recycled vectors cost the same in both)
         const double solve_ms = std::chrono::duration<double, std::milli>(t1 - tg).count();
+        const double start_ms = std::chrono::duration<double, std::milli>(tg - t0).count();
         if (s.iterations > 0) c->lg_iter_ms = ema(c->lg_iter_ms, solve_ms / s.iterations);
         if (tried) {
-            c->lg_cost_ms = ema(c->lg_cost_ms, std::chrono::duration<double, std::milli>(tg - t0).count());
-            c->lg_it_on = ema(c->lg_it_on, used ? s.iterations : (c->lg_it_off < 0 ? s.iterations : c->lg_it_off));
+            c->lg_cost_ms = ema(c->lg_cost_ms, start_ms);
+            c->lg_it_on = ema(c->lg_it_on, guess ? s.iterations : (c->lg_it_off < 0 ? s.iterations : c->lg_it_off));
         } else {
+            c->lg_base_ms = ema(c->lg_base_ms, start_ms);
             c->lg_it_off = ema(c->lg_it_off, s.iterations);
         }
         const bool was_on = c->lg_on;
         if (c->lg_it_on >= 0.0 && c->lg_it_off >= 0.0)
-            c->lg_on = (c->lg_it_off - c->lg_it_on) * c->lg_iter_ms > c->lg_cost_ms;
+            c->lg_on = (c->lg_it_off - c->lg_it_on) * c->lg_iter_ms > c->lg_cost_ms - std::max(c->lg_base_ms, 0.0);
         if (recheck) {
             c->lg_last = c->lg_count;
             c->lg_period = (c->lg_on == was_on) ? std::min(2 * c->lg_period, 1024) : 16;
         }
         c->lg_count++;
     }
-    c->st.p_guesses += used ? 1 : 0;
+    c->st.p_guesses += guess ? 1 : 0;
     if (c->mpnn) MC_TRY(mpnn_collect_pair(c, b, x));
+    if (atol == 0.0 && s.converged) MC_TRY(defl_push(c, x));
     c->st.p_iters += s.iterations;
     c->st.p_solves += 1;
     c->st.p_outer += s.outer;

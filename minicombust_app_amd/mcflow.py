@@ -29,6 +29,7 @@ ARR_FF0, ARR_FF1, ARR_MFLUX = 40, 41, 42
 ARR_ROW_PTR, ARR_COL, ARR_A_VALS, ARR_PA_VALS, ARR_PB = range(64, 69)
 ARR_FACES, ARR_CELL_FACES, ARR_CELL_NEIGHBOURS, ARR_CELLS, ARR_POINTS, ARR_CELLS_PER_POINT = range(72, 78)
 ARR_PARTICLE_TERMS = 80
+ARR_PDEFL = 84   # + k: k-th newest recycled loop-1 pressure solution (read-only)
 
 KC_NAMES = ["grad11", "grad1", "mom_asm", "p_asm", "scalar_asm", "pcg_spmv64", "pcg_spmv32", "vcycle",
             "flux_correct", "mpnn_fwd", "jac64", "jac32", "bicg_spmv", "scalar_batch", "ir_resid", "mpnn_bwd"]
@@ -51,7 +52,7 @@ class Options(C.Structure):
         ("mixed_inner_rtol", C.c_double), ("check_every", C.c_int), ("inlet_scale", C.c_double),
         ("amg_cheb_ratio", C.c_double), ("cell_tile", C.c_int), ("limit_grad", C.c_int), ("amg_refresh", C.c_int),
         ("cell_ychunk", C.c_int), ("warm_start", C.c_int), ("amg_half", C.c_int),
-        ("amg_tail", C.c_int),
+        ("amg_tail", C.c_int), ("p_recycle", C.c_int),
     ]
 
 
@@ -67,7 +68,8 @@ class Stats(C.Structure):
         ("last_p_relres", C.c_double), ("p_solve_ms", C.c_double), ("last_p_max", C.c_double),
         ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
         ("clip_passes", C.c_int64), ("host_polls", C.c_int64), ("p_guesses", C.c_int64),
-        ("p_guess_ratio_sum", C.c_double),
+        ("p_guess_ratio_sum", C.c_double), ("p_defl_starts", C.c_int64), ("p_defl_ratio_sum", C.c_double),
+        ("p_defl_vectors", C.c_int64),
     ]
 
 
@@ -356,6 +358,8 @@ class Context:
             ARR_CELL_NEIGHBOURS: (6 * self.N, np.uint64), ARR_CELLS: (8 * self.N, np.uint64),
             ARR_POINTS: (3 * self.NP, np.float64), ARR_CELLS_PER_POINT: (self.NP, np.uint8),
             ARR_PARTICLE_TERMS: (5 * self.N, np.float64),
+            ARR_PDEFL: (self.N, np.float64), ARR_PDEFL + 1: (self.N, np.float64),
+            ARR_PDEFL + 2: (self.N, np.float64), ARR_PDEFL + 3: (self.N, np.float64),
         }
         return table[aid]
 
