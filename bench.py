@@ -66,6 +66,9 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
     for _ in range(warmup):
         ctx.simple_iteration(it % 3)
         it += 1
+    ctx.sync()
+    s0 = ctx.stats()   # the AMG hierarchy was built during the warm-up
+    amg_setup = {"builds": s0["amg_setups"], "ms": round(s0["amg_setup_ms"], 2)}
     # profiled pass: every kernel class bracketed by HIP events (no syncs)
     kern = {}
     if profile_steps > 0:
@@ -140,6 +143,7 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
         "fields": {name: ctx.phi(f)[:ctx.N].copy() for name, f in FIELDS},
         "memory": ctx.memory(),
         "amg_half": bool(ctx.opts.amg_half) and psolve == mc.PSOLVE_MIXED,
+        "amg_setup": amg_setup,
     }
     if use_net and "mpnn_fwd" not in kern:
         # the measured policy can leave the guess off through the profiled steps:
@@ -350,7 +354,8 @@ def main():
                       "deflation_starts_per_step": v["deflation_starts_per_step"],
                       "deflation_start_residual_ratio": v["deflation_start_residual_ratio"],
                       "deflation_vectors_per_start": v["deflation_vectors_per_start"],
-                      "last_rel_residual": v["last_rel_residual"]} for k, v in forms.items()},
+                      "last_rel_residual": v["last_rel_residual"], "amg_setup": v["amg_setup"]}
+                  for k, v in forms.items()},
         "forms_skipped": skipped or None,
         "field_rel_error_vs_fp64": field_errors(forms),
         "device_memory_GB": {k: round(v / 1e9, 3) for k, v in h["memory"].items()},

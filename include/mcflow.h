@@ -144,6 +144,8 @@ typedef struct mc_stats {
                                  recycled solutions), the start accepted           */
     double  p_defl_ratio_sum; /* sum over those of ||b - A x0|| / ||b - mean b||  */
     int64_t p_defl_vectors;   /* sum over those of the vectors W held            */
+    int64_t amg_setups;       /* AMG hierarchy builds (aggregation, Galerkin pattern) */
+    double  amg_setup_ms;     /* host wall time of those builds, first values included */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>
@@ -353,6 +355,28 @@ int  mc_vtk_write_arrays(const char *path, int64_t n, const double *centres, con
 int  mc_profile_enable(mc_ctx *ctx, int mask);
 int  mc_profile_read(mc_ctx *ctx, double *ms, int64_t *calls, double *bytes);
 int  mc_profile_reset(mc_ctx *ctx);
+/* Per-stage times of the SIMPLE iteration: the reference's fine timers
+   (vel_/pres_/sca_*_time, include/flow/FlowSolver.hpp:128-141, printed by
+   FlowSolver.inl:1768-1806 and its pressure and scalar counterparts when
+   FLOW_SOLVER_FINE_TIME is set).  mc_stage_enable(ctx, 1) brackets every
+   stage with HIP events on the context stream (no synchronisation);
+   mc_stage_read synchronises and returns the accumulated milliseconds and
+   bracket counts per stage (arrays of MC_ST_COUNT, either may be NULL);
+   mc_stage_reset clears them.                                                */
+enum { MC_ST_GRAD = 0,     /* 11-field gradients (get_phi_gradients, limiter)   */
+       MC_ST_VEL_FLUX,     /* momentum fluxes, sources and matrix               */
+       MC_ST_VEL_SOLVE,    /* U, V, W BiCGStab                                  */
+       MC_ST_PRES_FLUX,    /* Rhie-Chow mass flux and pressure-correction matrix */
+       MC_ST_PRES_SOLVE,   /* pressure-correction solves (every loop, AMG refresh) */
+       MC_ST_PRES_CORRECT, /* PP slots, grad PP, flux and cell corrections, P update */
+       MC_ST_TURB_FLUX,    /* TE and ED assembly                                */
+       MC_ST_TURB_SOLVE,   /* TE and ED BiCGStab, non-negativity repair         */
+       MC_ST_SCA_FLUX,     /* TEM, FUL, PRO, VARF, VARP assembly                */
+       MC_ST_SCA_SOLVE,    /* their BiCGStab solves                             */
+       MC_ST_COUNT };
+int  mc_stage_enable(mc_ctx *ctx, int on);
+int  mc_stage_read(mc_ctx *ctx, double *ms, int64_t *calls);
+int  mc_stage_reset(mc_ctx *ctx);
 /* Streaming copy of `bytes` (multiple of 16) between two fresh device buffers,
    `reps` launches on the context stream; *ms_out = average launch time.  The
    HBM bandwidth probe and the known-size kernel that scales the FETCH_SIZE /

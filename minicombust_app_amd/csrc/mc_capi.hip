@@ -91,9 +91,10 @@ KTimer::KTimer(Ctx *c_, int kc_, double bytes_) : c(c_)
     slot = (long)i;
 }
 
-KTimer::~KTimer()
+void KTimer::end()
 {
     if (slot >= 0) (void)hipEventRecord(c->ev_pool[2 * slot + 1], c->stream);
+    slot = -1;
 }
 
 int profile_fold(Ctx *c)
@@ -681,7 +682,36 @@ int mc_profile_enable(mc_ctx *x, int mask)
 {
     CTX_OR_FAIL
     MC_TRY(profile_fold(c));
-    c->profile = (unsigned)mask & ((1u << MC_KC_COUNT) - 1u);
+    const unsigned kc = (1u << MC_KC_COUNT) - 1u;
+    c->profile = (c->profile & ~kc) | ((unsigned)mask & kc);
+    return MC_OK;
+}
+
+int mc_stage_enable(mc_ctx *x, int on)
+{
+    CTX_OR_FAIL
+    MC_TRY(profile_fold(c));
+    const unsigned st = ((1u << MC_ST_COUNT) - 1u) << MC_KC_COUNT;
+    c->profile = on ? (c->profile | st) : (c->profile & ~st);
+    return MC_OK;
+}
+
+int mc_stage_read(mc_ctx *x, double *ms, int64_t *calls)
+{
+    CTX_OR_FAIL
+    MC_TRY(profile_fold(c));
+    for (int k = 0; k < MC_ST_COUNT; k++) {
+        if (ms) ms[k] = c->kt_ms[MC_KC_COUNT + k];
+        if (calls) calls[k] = c->kt_calls[MC_KC_COUNT + k];
+    }
+    return MC_OK;
+}
+
+int mc_stage_reset(mc_ctx *x)
+{
+    CTX_OR_FAIL
+    MC_TRY(profile_fold(c));
+    for (int k = MC_KC_COUNT; k < MC_KC_COUNT + MC_ST_COUNT; k++) { c->kt_ms[k] = 0; c->kt_calls[k] = 0; }
     return MC_OK;
 }
 

@@ -10,8 +10,6 @@
 // by the owner of the highest-numbered face of that side: the reference's
 // last writer.  Compiled without FP contraction so each expression rounds as
 // written in FlowSolver.inl.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -90,8 +88,8 @@ __device__ __forceinline__ void store_row(const Geo &g, int c, int nint, const i
 {
     if (dia) {
         const size_t N = (size_t)g.N;
-        dia[(size_t)dslot(g, 0) * N + c] = diag;
-        for (int k = 0; k < nint; k++) dia[(size_t)dslot(g, nb[k] - c) * N + c] = v[k];
+        st_out(&dia[(size_t)dslot(g, 0) * N + c], diag);
+        for (int k = 0; k < nint; k++) st_out(&dia[(size_t)dslot(g, nb[k] - c) * N + c], v[k]);
     } else {
         int pos[6], dpos;
         row_positions(c, nint, nb, pos, dpos);
@@ -238,7 +236,7 @@ __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
         double gy = (bx[k] * invdet * c10) + (by[k] * invdet * c11) + (bz[k] * invdet * c12);
         double gz = (bx[k] * invdet * c20) + (by[k] * invdet * c21) + (bz[k] * invdet * c22);
         if constexpr (LIMIT) venkat(ph[k], pmax[k], pmin[k], ds, gx, gy, gz);
-        a.gx[k][c] = gx; a.gy[k][c] = gy; a.gz[k][c] = gz;
+        st_out(&a.gx[k][c], gx); st_out(&a.gy[k][c], gy); st_out(&a.gz[k][c], gz);
     }
 }
 
@@ -432,7 +430,7 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
                 SV = SV - blend_v + fvde - fvdi;
                 SW = SW - blend_w + fwde - fwdi;
                 own_ff[nint] = ff1;
-                a.ff0[f] = ff0; a.ff1[f] = ff1;
+                st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
             } else {
                 SU = SU + blend_u - fude + fudi;
                 SV = SV + blend_v - fvde + fvdi;
@@ -512,12 +510,12 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     SU = SU + (1.0 - a.urf) * A * Uc;
     SV = SV + (1.0 - a.urf) * A * Vc;
     SW = SW + (1.0 - a.urf) * A * Wc;
-    a.AU[c] = A;
-    a.SU[c] = SU; a.SV[c] = SV; a.SW[c] = SW;
+    st_out(&a.AU[c], A);
+    st_out(&a.SU[c], SU); st_out(&a.SV[c], SV); st_out(&a.SW[c], SW);
     if (a.r0[0]) {
-        a.r0[0][c] = SU - (A * Uc + xU);
-        a.r0[1][c] = SV - (A * Vc + xV);
-        a.r0[2][c] = SW - (A * Wc + xW);
+        st_out(&a.r0[0][c], SU - (A * Uc + xU));
+        st_out(&a.r0[1][c], SV - (A * Vc + xV));
+        st_out(&a.r0[2][c], SW - (A * Wc + xW));
     }
     store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
@@ -765,8 +763,8 @@ __global__ void __launch_bounds__(256) k_mass_faces(Geo g, MassArgs a)
         const RCCell other = rc_cell(g, a, c1);
         double ApV;
         const double m = rhie_chow(g, f, own, other, ApV);
-        a.mflux[f] = m;
-        a.ff0[f] = -ApV; a.ff1[f] = -ApV;
+        st_out(&a.mflux[f], m);
+        st_out(&a.ff0[f], -ApV); st_out(&a.ff1[f], -ApV);
     }
 }
 
@@ -959,8 +957,8 @@ __global__ void __launch_bounds__(256) k_flux_correct(Geo g, const double *PP, c
         const double dpz = gP1.z * Xn.z - gP0.z * Xp.z;
         const double fcv = fact * (dpx + dpy + dpz) * 0.8;
         m += fcv;
-        mflux[f] = m;
-        fcorr[f] = fcv;
+        st_out(&mflux[f], m);
+        st_out(&fcorr[f], fcv);
     }
 }
 
@@ -1047,10 +1045,14 @@ __global__ void k_copy(int n, const double *a, double *b)
 int calculate_pressure(Ctx *c)
 {
     const int N = c->N;
-    MC_TRY(calculate_mass_flux_assemble(c));
+    {
+        StageTimer t(c, MC_ST_PRES_FLUX);
+        MC_TRY(calculate_mass_flux_assemble(c));
+    }
     const bool simple = c->opt.mode == MC_MODE_SIMPLE;
     // AMG values follow the matrix that will be solved
     if (c->first_pressure || simple) {
+        StageTimer t(c, MC_ST_PRES_SOLVE);
         if (!c->amg_ready) MC_TRY(amg_setup(c, c->PA_vals));
         else MC_TRY(amg_update_values(c, c->PA_vals));
     }
@@ -1062,6 +1064,7 @@ int calculate_pressure(Ctx *c)
         // solve (compat: Pressure_b / Pressure_A are frozen after the first
         // assembly, so every later solve reproduces the stored solution)
         const bool fresh = simple || c->first_pressure || c->u == nullptr;
+        StageTimer ts(c, MC_ST_PRES_SOLVE);
         if (simple) {
             MC_HIP(c, hipMemcpyAsync(c->Pb, c->SU, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
         }
@@ -1077,6 +1080,8 @@ int calculate_pressure(Ctx *c)
         } else {
             MC_HIP(c, hipMemcpyAsync(c->phi[PP], c->u, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
         }
+        ts.end();
+        StageTimer tc(c, MC_ST_PRES_CORRECT);
         MC_TRY(reduce_maxabs_pp(c, c->phi[PP], &pmax));
         if (loop == 1) pref = pmax;
         MC_TRY(pressure_boundary_slots(c, PP));
@@ -1102,6 +1107,7 @@ int calculate_pressure(Ctx *c)
     c->st.p_loops += loop;
     c->st.last_loops = loop;
     c->st.last_p_max = pref;
+    StageTimer tc(c, MC_ST_PRES_CORRECT);
     MC_TRY(pressure_boundary_slots(c, P));
     MC_TRY(grad_single(c, P, true));
     MC_TRY(update_P(c));
@@ -1185,7 +1191,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
             if (own0) {
                 S = S - blend + fde1 - fdi;
                 own_ff[nint] = ff1;
-                a.ff0[f] = ff0; a.ff1[f] = ff1;
+                st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
             } else {
                 S = S + blend - fde1 + fdi;
                 own_ff[nint] = ff0;
@@ -1266,7 +1272,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
                 A = A + rho * DisP * vol;
             }
         }
-        a.TP[c] = TP;
+        st_out(&a.TP[c], TP);
     } else if (ed) {
         const double TEc = a.TE[c];
         const double fact = a.ED[c] / (TEc + 0.000000000000000001) * vol;
@@ -1293,9 +1299,9 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     for (int k = 0; k < nint; k++) A -= own_ff[k];
     A *= (1. / a.urf);
     S = S + (1.0 - a.urf) * A * phic;
-    a.AV[c] = A;
-    a.SU[c] = S;
-    if (a.r0) a.r0[c] = S - (A * pc + xo);   // x0 = phi[c], also where phic is ED's wall value
+    st_out(&a.AV[c], A);
+    st_out(&a.SU[c], S);
+    if (a.r0) st_out(&a.r0[c], S - (A * pc + xo));   // x0 = phi[c], also where phic is ED's wall value
     store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
 }
 
@@ -1405,7 +1411,7 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
             }
             if (own0) {
                 own_ff[nint] = ff1;
-                a.ff0[f] = ff0; a.ff1[f] = ff1;
+                st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
             } else {
                 own_ff[nint] = ff0;
             }
@@ -1487,7 +1493,7 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
         for (int k = 0; k < nint; k++) At -= own_fft[k];
         At *= (1. / a.urf);
         St = St + (1.0 - a.urf) * At * pt;
-        a.tS[c] = St;
+        st_out(&a.tS[c], St);
     }
     double phic[SB];
 #pragma unroll
@@ -1501,9 +1507,9 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
 #pragma unroll
     for (int s = 0; s < SB; s++) {
         S[s] = S[s] + (1.0 - a.urf) * A * phic[s];
-        a.S[s][c] = S[s];
+        st_out(&a.S[s][c], S[s]);
     }
-    a.AV[c] = A;
+    st_out(&a.AV[c], A);
     store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
     if constexpr (TEM) store_row(g, c, nint, own_nb, own_fft, At, a.tvals, a.tdia);
 }
@@ -1544,6 +1550,7 @@ static int batch_solve(Ctx *c, bool tem)
         const Sizes z = sizes_of(c);
         const double bytes = (40.0 * nsc + 96.0) * z.N + 48.0 * z.F + 16.0 * z.Fi + 24.0 * z.Fb +
                              8.0 * z.nnz * (tem ? 2 : 1);
+        StageTimer t(c, MC_ST_SCA_FLUX);
         KTimer kt(c, MC_KC_SCALAR_BATCH, bytes);
         if (tem)
             hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
@@ -1551,6 +1558,7 @@ static int batch_solve(Ctx *c, bool tem)
             hipLaunchKernelGGL(k_scalar_batch<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
         MC_LAUNCH_CHECK(c);
     }
+    StageTimer ts(c, MC_ST_SCA_SOLVE);
     if (tem) {
         double *bt[1] = {c->SU}, *xt[1] = {c->phi[TEM]};
         mc_solve_status stt{};
@@ -1626,10 +1634,68 @@ int scalar_assemble(Ctx *c, int type, double *r0)
 // The negative cells are compacted once (ascending); passes relax them in
 // place until a pass changes nothing -- that state is the fixed point, hence
 // the reference's values exactly, whatever order the threads ran in.
-__global__ void __launch_bounds__(256) k_flag_neg(int n, const double *v, uint8_t *flag)
+//
+// Compaction of the negative cells in ascending order: block b owns
+// [b * span, (b + 1) * span), span = 256 * rounds; k_neg_count counts its
+// negatives, k_neg_scan turns the counts into exclusive offsets (one
+// workgroup, fixed order) and the total, k_neg_write writes each negative
+// cell at its block offset plus its rank in index order (wave ballots).
+__device__ __forceinline__ bool is_neg(const double *v, int i, int n) { return i < n && v[i] < 0.0; }
+
+__global__ void __launch_bounds__(256) k_neg_count(int n, const double *v, int rounds, int *bcount)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) flag[i] = v[i] < 0.0 ? 1 : 0;
+    __shared__ int ws[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = blockIdx.x * 256 * rounds;
+    int cnt = 0;
+    for (int r = 0; r < rounds; r++) {
+        const unsigned long long b = __ballot(is_neg(v, base + r * 256 + (int)threadIdx.x, n));
+        cnt += __popcll(b);
+    }
+    if (lane == 0) ws[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// in place: counts -> exclusive offsets; *total = the sum
+__global__ void __launch_bounds__(1024) k_neg_scan(int nb, int *bcount, int *total)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x, per = (nb + 1023) / 1024;
+    const int i0 = min(nb, t * per), i1 = min(nb, i0 + per);
+    int s = 0;
+    for (int i = i0; i < i1; i++) s += bcount[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < 1024; k++) { const int x = part[k]; part[k] = run; run += x; }
+        *total = run;
+    }
+    __syncthreads();
+    int run = part[t];
+    for (int i = i0; i < i1; i++) { const int x = bcount[i]; bcount[i] = run; run += x; }
+}
+
+__global__ void __launch_bounds__(256) k_neg_write(int n, const double *v, int rounds, const int *boff, int *list)
+{
+    __shared__ int ws[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = blockIdx.x * 256 * rounds;
+    int off = boff[blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < rounds; r++) {
+        const int i = base + r * 256 + (int)threadIdx.x;
+        const bool neg = is_neg(v, i, n);
+        const unsigned long long b = __ballot(neg);
+        if (lane == 0) ws[wave] = __popcll(b);
+        __syncthreads();
+        int pre = off;
+        for (int w = 0; w < wave; w++) pre += ws[w];
+        if (neg) list[pre + __popcll(b & below)] = i;
+        off += ws[0] + ws[1] + ws[2] + ws[3];
+        __syncthreads();
+    }
 }
 
 // One relaxation pass.  After the first (all != 0), a cell is evaluated only
@@ -1677,11 +1743,6 @@ int flow_workspace(Ctx *c)
     }
     for (int s = 0; s < 4; s++) MC_TRY(dalloc(c, &c->srhs[s], c->N));
     MC_TRY(dalloc(c, &c->neg_list, c->N));
-    size_t tb = 0;
-    MC_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tb, hipcub::CountingInputIterator<int>(0), (const uint8_t *)nullptr,
-                                            (int *)nullptr, (int *)nullptr, c->N, c->stream));
-    c->cub_tmp_bytes = tb;
-    MC_TRY(alloc(c, &c->cub_tmp, tb + 256));
     return MC_OK;
 }
 
@@ -1699,11 +1760,19 @@ int clip_nonneg(Ctx *c, double *phi, int slot)
     int *changed = (int *)(c->dscal + 704);   // CLIP_BATCH_MAX pass counters
     int *hchanged = (int *)(c->hscal + 704);
     uint8_t *flag = (uint8_t *)c->wk[13];
-    hipLaunchKernelGGL(k_flag_neg, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, phi, flag);
-    MC_LAUNCH_CHECK(c);
-    size_t tb = c->cub_tmp_bytes;
-    MC_HIP(c, hipcub::DeviceSelect::Flagged(c->cub_tmp, tb, hipcub::CountingInputIterator<int>(0), flag, c->neg_list,
-                                            dcount, N, c->stream));
+    {
+        // block counts in the reduction partials buffer (as ints)
+        const int cap = 2 * RED_PARTIALS;
+        int rounds = 4;
+        while (div_up(N, 256 * rounds) > cap) rounds *= 2;
+        const int nb = div_up(N, 256 * rounds);
+        int *bc = (int *)c->red;
+        hipLaunchKernelGGL(k_neg_count, dim3(nb), dim3(256), 0, c->stream, N, (const double *)phi, rounds, bc);
+        hipLaunchKernelGGL(k_neg_scan, dim3(1), dim3(1024), 0, c->stream, nb, bc, dcount);
+        hipLaunchKernelGGL(k_neg_write, dim3(nb), dim3(256), 0, c->stream, N, (const double *)phi, rounds,
+                           (const int *)bc, c->neg_list);
+        MC_LAUNCH_CHECK(c);
+    }
     MC_HIP(c, hipMemcpyAsync(hcount, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     MC_HIP(c, hipStreamSynchronize(c->stream));
     const int K = *hcount;
@@ -1743,9 +1812,14 @@ int scalar_solve(Ctx *c, int type)
     // TE and ED: the assembly forms b - A x0 for the warm start; the scalars
     // the batched kernel assembles take the solver's own pass, as there, so a
     // single solve and the batch give the same bits
-    const bool fused = warm && (type == TERBTE || type == TERBED);
+    const bool turb = type == TERBTE || type == TERBED;
+    const bool fused = warm && turb;
     if (fused) MC_TRY(r0_buffers(c));
-    MC_TRY(scalar_assemble(c, type, fused ? c->r0buf[0] : nullptr));
+    {
+        StageTimer t(c, turb ? MC_ST_TURB_FLUX : MC_ST_SCA_FLUX);
+        MC_TRY(scalar_assemble(c, type, fused ? c->r0buf[0] : nullptr));
+    }
+    StageTimer ts(c, turb ? MC_ST_TURB_SOLVE : MC_ST_SCA_SOLVE);
     double *b[1] = {c->SU};
     double *x[1] = {c->phi[field]};
     mc_solve_status st{};
@@ -1760,14 +1834,24 @@ int scalar_solve(Ctx *c, int type)
 // ============================================================= A17 ==
 int simple_iteration(Ctx *c, int inner)
 {
-    MC_TRY(grad_multi(c, c->opt.limit_grad != 0));   // + limit_phi_gradients, FlowSolver.inl:3197-3198
+    {
+        StageTimer t(c, MC_ST_GRAD);
+        MC_TRY(grad_multi(c, c->opt.limit_grad != 0));   // + limit_phi_gradients, FlowSolver.inl:3197-3198
+    }
     // FlowSolver.inl:3204-3208: the first pass of the first timestep
     if (c->timestep_count == 0 && inner == 0 && !c->field_set_up) {
+        StageTimer t(c, MC_ST_PRES_FLUX);
         MC_TRY(set_up_field(c));
         c->field_set_up = true;
     }
-    MC_TRY(momentum_assemble(c, true));
-    MC_TRY(momentum_solve(c, true));
+    {
+        StageTimer t(c, MC_ST_VEL_FLUX);
+        MC_TRY(momentum_assemble(c, true));
+    }
+    {
+        StageTimer t(c, MC_ST_VEL_SOLVE);
+        MC_TRY(momentum_solve(c, true));
+    }
     MC_TRY(calculate_pressure(c));
     if (c->opt.solve_scalars) {
         MC_TRY(scalar_solve(c, TERBTE));

@@ -161,8 +161,6 @@ struct Ctx {
     double *srhs[4] = {};          // right-hand sides of the batched scalar solve
     double *r0buf[3] = {};         // b - A x0 formed by an assembly for the warm-started solve after it
     int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
-    void *cub_tmp = nullptr;       // hipcub scratch for the compaction
-    size_t cub_tmp_bytes = 0;
     double *wk[24] = {};           // N-length work vectors
     float *wk32[8] = {};
     double *red = nullptr;         // reduction partials
@@ -217,9 +215,12 @@ struct Ctx {
     std::vector<int> ev_kc;              // class of each pending pair
     std::vector<double> ev_bytes;
     size_t ev_used = 0;                  // pairs recorded since the last fold
-    double kt_ms[MC_KC_COUNT] = {};      // accumulated kernel-class time
-    int64_t kt_calls[MC_KC_COUNT] = {};
-    double kt_bytes[MC_KC_COUNT] = {};   // accumulated algorithmic bytes
+    // accumulated time, brackets and algorithmic bytes per kernel class,
+    // then per stage (class MC_KC_COUNT + MC_ST_x, bit of the same number in
+    // `profile`)
+    double kt_ms[MC_KC_COUNT + MC_ST_COUNT] = {};
+    int64_t kt_calls[MC_KC_COUNT + MC_ST_COUNT] = {};
+    double kt_bytes[MC_KC_COUNT + MC_ST_COUNT] = {};
 };
 
 // ---------------------------------------------------------------- errors --
@@ -269,10 +270,9 @@ struct CellWalk {
     int yc = 1;                   // y rows per chunk of the block order
 };
 
-__device__ __forceinline__ int walk_cell(const CellWalk &w)
+// Cell of thread t (0..255) of brick b in CellWalk order, or -1.
+__device__ __forceinline__ int walk_cell_at(const CellWalk &w, int b, int t)
 {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int t = threadIdx.x;
     if (w.nx == 0) {
         const int c = b * 256 + t;
         return c < w.N ? c : -1;
@@ -292,9 +292,25 @@ __device__ __forceinline__ int walk_cell(const CellWalk &w)
     return x + w.nx * (y + w.ny * z);
 }
 
+__device__ __forceinline__ int walk_cell(const CellWalk &w)
+{
+    return walk_cell_at(w, xcd_block(blockIdx.x, gridDim.x), threadIdx.x);
+}
+
 // std::min / std::max semantics (FlowSolver.inl uses them on doubles).
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// Store of a kernel output that nothing later in the same launch reads: a
+// streaming (non-temporal) store, so the written lines do not displace from
+// L2 the neighbour values that other workgroups of the launch still gather.
+// Coalesced full-word stores only: scattered or sub-word streaming stores
+// (the pressure assembly's FP16 / FP32 copies) measured slower than plain ones.
+template <class T>
+__device__ __forceinline__ void st_out(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
@@ -436,7 +452,13 @@ struct KTimer {
     Ctx *c;
     long slot = -1;
     KTimer(Ctx *c_, int kc_, double bytes_);
-    ~KTimer();
+    ~KTimer() { end(); }
+    void end();   // the closing event now (the destructor then records none)
+};
+static_assert(MC_KC_COUNT + MC_ST_COUNT <= 32, "class and stage bits share one 32-bit mask");
+// a stage of the SIMPLE iteration (MC_ST_x), timed when stages are enabled
+struct StageTimer : KTimer {
+    StageTimer(Ctx *c_, int st) : KTimer(c_, MC_KC_COUNT + st, 0.0) {}
 };
 int profile_fold(Ctx *c);
 int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out);   // mc_linalg.hip

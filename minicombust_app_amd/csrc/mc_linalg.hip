@@ -1019,7 +1019,22 @@ __global__ void __launch_bounds__(1024) k_dense_inverse(int n, const int *rp, co
         }
 }
 
+static int amg_build(Ctx *c, const double *vals);
+
+// Aggregation-AMG hierarchy from the pressure matrix (host aggregation and
+// Galerkin pattern, then the first values on the device); the wall time up to
+// the first values' completion goes to the statistics
 int amg_setup(Ctx *c, const double *vals)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    MC_TRY(amg_build(c, vals));
+    MC_HIP(c, hipStreamSynchronize(c->stream));
+    c->st.amg_setups += 1;
+    c->st.amg_setup_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MC_OK;
+}
+
+static int amg_build(Ctx *c, const double *vals)
 {
     MemScope ms(c, MC_MEM_AMG);
     HostCSR A;
@@ -2426,6 +2441,9 @@ static int deflation_start(Ctx *c, const double *b, double *x, bool want_guess, 
     const double *h = c->hscal + 640;
     *used = h[K + 4] > 0.0;
     *bnorm = h[K + 2];
+    // the guess is w[0]; a dropped vector (NaN / Inf output, or in the span
+    // of the others) has coefficient exactly 0 and contributes nothing to x0
+    if (*guess && h[1] == 0.0) *guess = false;
     if (*used) {
         const double ratio = h[K + 2] > 0.0 ? h[K + 3] / h[K + 2] : 0.0;
         c->st.p_defl_starts += 1;
@@ -2542,29 +2560,17 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
 
 // ------------------------------------------------------ copy probe --
 // Streaming copy, the known-size kernel for HBM bandwidth and counter
-// calibration: each lane issues COPY_UNROLL independent 16-byte loads (one
-// per 256-lane stride of its block's span) before any of its stores, so
-// several loads per lane are in flight; every element is read once and
-// written once.
-constexpr int COPY_UNROLL = 8;
-__global__ void __launch_bounds__(256) k_copy_probe(int64_t n16, const double2 *__restrict__ src,
-                                                   double2 *__restrict__ dst)
+// calibration: one 16-byte element per lane, one lane per element (a grid of
+// n / 256 workgroups), streaming (non-temporal) loads and stores; every
+// element is read once and written once.  Of the forms measured on the
+// MI355X (several elements per lane at a 256-lane stride, grid-stride loops,
+// plain or streaming accesses) this one moves the most bytes per second.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_copy_probe(int64_t n16, const dvec2 *__restrict__ src,
+                                                   dvec2 *__restrict__ dst)
 {
-    const int64_t span = (int64_t)256 * COPY_UNROLL;
-    const int64_t stride = (int64_t)gridDim.x * span;
-    for (int64_t base = (int64_t)blockIdx.x * span + threadIdx.x; base < n16; base += stride) {
-        double2 v[COPY_UNROLL];
-#pragma unroll
-        for (int u = 0; u < COPY_UNROLL; u++) {
-            const int64_t i = base + (int64_t)u * 256;
-            if (i < n16) v[u] = src[i];
-        }
-#pragma unroll
-        for (int u = 0; u < COPY_UNROLL; u++) {
-            const int64_t i = base + (int64_t)u * 256;
-            if (i < n16) dst[i] = v[u];
-        }
-    }
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n16) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out)
@@ -2579,17 +2585,21 @@ int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out)
     int rc = MC_OK;
     hipEvent_t a = nullptr, b = nullptr;
     const int64_t n16 = bytes / 16;
-    const int blocks = (int)std::min<int64_t>(div_up(n16, 256 * COPY_UNROLL), 256 * 16);
+    if (div_up(n16, 256) > INT32_MAX) {
+        c->err = "copy probe: size beyond one launch";
+        return MC_ERR_ARG;
+    }
+    const int blocks = div_up(n16, 256);
     if (hipMemsetAsync(src, 0, (size_t)bytes, c->stream) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
         hipEventCreate(&b) != hipSuccess) {
         rc = MC_ERR_HIP;
     } else {
-        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const double2 *)src,
-                           (double2 *)dst);   // untimed first touch
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const dvec2 *)src,
+                           (dvec2 *)dst);   // untimed first touch
         (void)hipEventRecord(a, c->stream);
         for (int r = 0; r < reps; r++)
-            hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const double2 *)src,
-                               (double2 *)dst);
+            hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, n16, (const dvec2 *)src,
+                               (dvec2 *)dst);
         (void)hipEventRecord(b, c->stream);
         float ms = 0.f;
         if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = MC_ERR_HIP;

@@ -70,6 +70,9 @@ struct Mpnn {
     float *w = nullptr, *g = nullptr, *am = nullptr, *av = nullptr;   // master, grad, Adam moments
     int64_t adam_t = 0;
     bf16_t *wb = nullptr;         // BF16 copies
+    bf16_t *wf = nullptr;         // BF16 A fragments of the fused inference layers
+    bf16_t *hb[2] = {};           // inference activations (BF16, fragment order)
+    int cus = 0;                  // compute units of the device
     int *nbr = nullptr, *rev = nullptr, *pmap = nullptr;
     float *deg = nullptr;
     float *f = nullptr, *e = nullptr, *y = nullptr, *dy = nullptr;
@@ -111,33 +114,6 @@ __global__ void k_pack_rows(bf16_t *dst, int ld, int koff, const float *src, int
     if (i >= rows * cols) return;
     const int r = i / cols, k = i % cols;
     dst[r * ld + koff + k] = to_bf16(src[i]);
-}
-
-static void pack_weights(Ctx *c)
-{
-    Mpnn *m = M(c);
-    auto T = [&](bf16_t *dst, int kp, const float *src, int rows, int cols) {
-        hipLaunchKernelGGL(k_pack_t, dim3(div_up(cols * kp, 256)), dim3(256), 0, c->stream, dst, kp, src, rows, cols);
-    };
-    auto R = [&](bf16_t *dst, int ld, int koff, const float *src, int rows, int cols) {
-        hipLaunchKernelGGL(k_pack_rows, dim3(div_up(rows * cols, 256)), dim3(256), 0, c->stream, dst, ld, koff, src,
-                           rows, cols);
-    };
-    for (int l = 0; l < ML; l++) {
-        const float *Wl = m->w + OFF_L0 + l * LSZ;
-        bf16_t *P = m->wb + l * PSZ;
-        T(P + P_ABT, MH, Wl + L_WA, MH, MH);               // rows m < 64: Wa^T
-        T(P + P_ABT + MH * MH, MH, Wl + L_WB, MH, MH);     // rows m >= 64: Wb^T
-        T(P + P_W2T, MH, Wl + L_W2, MH, MH);
-        T(P + P_N1T, 2 * MH, Wl + L_WN1, 2 * MH, MH);
-        T(P + P_N2T, MH, Wl + L_WN2, MH, MH);
-        R(P + P_ABR, 2 * MH, 0, Wl + L_WA, MH, MH);        // [Wa | Wb] row-major (64 x 128)
-        R(P + P_ABR, 2 * MH, MH, Wl + L_WB, MH, MH);
-        R(P + P_W2R, MH, 0, Wl + L_W2, MH, MH);
-        R(P + P_N1R, MH, 0, Wl + L_WN1, 2 * MH, MH);
-        R(P + P_N2R, MH, 0, Wl + L_WN2, MH, MH);
-    }
-    T(m->wb + P_ENCT, 16, m->w + OFF_WENC, MFIN, MH);
 }
 
 // ----------------------------------------------------- MFMA products --
@@ -194,112 +170,6 @@ __global__ void __launch_bounds__(256) k_mm(int n, const float *X1, int ld1, con
             if (EPI & EPI_RELU) v = fmaxf(v, 0.f);
             if (EPI & EPI_MASK) v = (mask[(size_t)row * ldm + col] > 0.f) ? v : 0.f;
             Y[(size_t)row * ldy + col] = v;
-        }
-    }
-}
-
-// Inference: the node update of one layer in one pass,
-//   a = s W2 + deg b2,  u = relu([h, a] Wn1 + bn1),  h' = h + u Wn2 + bn2,
-// a wave holding its 16 rows' a and u in LDS between the products instead of
-// writing and re-reading them (the training pass keeps them for backward).
-// Fragments, operand rounding and epilogues are k_mm's, in the same order.
-__global__ void __launch_bounds__(256) k_node_fused(int n, const float *S, const float *H, const float *deg,
-                                                   const bf16_t *W2t, const float *b2, const bf16_t *N1t,
-                                                   const float *bn1, const bf16_t *N2t, const float *bn2, float *Hout)
-{
-    __shared__ float tA[4][16][MH + 4];
-    __shared__ float tU[4][16][MH + 4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // every wave reaches the barriers; rows past n compute on zeros and store nothing
-    const int row0 = (blockIdx.x * 4 + wave) * 16;
-    const int ar = row0 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
-    auto frag = [&](float4 xv) {
-        short4v a;
-        a[0] = (short)to_bf16(xv.x); a[1] = (short)to_bf16(xv.y);
-        a[2] = (short)to_bf16(xv.z); a[3] = (short)to_bf16(xv.w);
-        return a;
-    };
-    float4v acc[MH / 16];
-    // a = s W2 + deg b2
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < MH / 16; kk++) {
-        const int k = kk * 16 + kq;
-        const float4 xv = (ar < n) ? *(const float4 *)(S + (size_t)ar * MH + k) : make_float4(0.f, 0.f, 0.f, 0.f);
-        const short4v a = frag(xv);
-#pragma unroll
-        for (int j = 0; j < MH / 16; j++) {
-            const short4v b = *(const short4v *)(W2t + (size_t)(j * 16 + (lane & 15)) * MH + k);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) {
-        const int col = j * 16 + (lane & 15);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int row = row0 + kq + t;
-            float v = acc[j][t];
-            v += b2[col] * (row < n ? deg[row] : 0.f);
-            tA[wave][kq + t][col] = v;
-        }
-    }
-    __syncthreads();
-    // u = relu([h, a] Wn1 + bn1)
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < 2 * MH / 16; kk++) {
-        const int k = kk * 16 + kq;
-        float4 xv;
-        if (k < MH) xv = (ar < n) ? *(const float4 *)(H + (size_t)ar * MH + k) : make_float4(0.f, 0.f, 0.f, 0.f);
-        else xv = (ar < n) ? *(const float4 *)&tA[wave][lane & 15][k - MH] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const short4v a = frag(xv);
-#pragma unroll
-        for (int j = 0; j < MH / 16; j++) {
-            const short4v b = *(const short4v *)(N1t + (size_t)(j * 16 + (lane & 15)) * (2 * MH) + k);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) {
-        const int col = j * 16 + (lane & 15);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float v = acc[j][t];
-            v += bn1[col];
-            v = fmaxf(v, 0.f);
-            tU[wave][kq + t][col] = v;
-        }
-    }
-    __syncthreads();
-    // h' = h + u Wn2 + bn2
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) acc[j] = float4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kk = 0; kk < MH / 16; kk++) {
-        const int k = kk * 16 + kq;
-        const float4 xv = (ar < n) ? *(const float4 *)&tU[wave][lane & 15][k] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const short4v a = frag(xv);
-#pragma unroll
-        for (int j = 0; j < MH / 16; j++) {
-            const short4v b = *(const short4v *)(N2t + (size_t)(j * 16 + (lane & 15)) * MH + k);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < MH / 16; j++) {
-        const int col = j * 16 + (lane & 15);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int row = row0 + kq + t;
-            if (row >= n) continue;
-            float v = acc[j][t];
-            v = H[(size_t)row * MH + col] + v;
-            v += bn2[col];
-            Hout[(size_t)row * MH + col] = v;
         }
     }
 }
@@ -597,6 +467,248 @@ __global__ void k_adam(int np, float *w, const float *g, float *m, float *v, flo
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// ------------------------------------------------- fused inference --
+// Inference runs one kernel per message-passing layer on BF16 activations
+// (128 bytes a cell), with v_mfma_f32_16x16x32_bf16 (gfx950, K = 32) in the
+// transposed form C = W^T X: weights are the A operand (rows = output
+// channels), a tile of 16 cells the B operand (columns).  Lane l then holds,
+// for output row tile j, channels 16 j + 4 g + t (g = l / 16, t < 4) of cell
+// l % 16, and those 16 channels are exactly the ones it needs as its B
+// fragment for the next product when the K dimension is taken in the order
+// frag_ch below: every product chains into the next without moving data
+// between lanes.  Activations are stored per cell in that fragment order, so
+// a lane's B fragment (8 channels) is one 16-byte load.
+//
+// The edge term Q_j = h_j Wb is recomputed for each of a cell's six
+// neighbours from the neighbour's stored activations (six 16 x 64 x 64
+// products per tile) instead of being written and gathered: a layer reads
+// each cell's 128 bytes, the neighbours' (from L2), its graph row, and
+// writes 128 bytes.  Cells go in the assembly kernels' brick order, each
+// XCD's workgroups over one contiguous range of bricks, so a neighbour's
+// activations were read shortly before by the same L2.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// channel at fragment position (ks, g, t) of a 64-channel vector
+__host__ __device__ constexpr int frag_ch(int ks, int g, int t) { return 16 * (2 * ks + (t >> 2)) + 4 * g + (t & 3); }
+
+// packed A fragments of one layer (BF16 elements): W^T of Wa, Wb, W2 (K = 64),
+// Wn1 (K = 128: h part, then a part), Wn2; fragment (j, ks) of lane l at
+// ((j * KS + ks) * 64 + l) * 8
+constexpr int FR_WA = 0, FR_WB = 4096, FR_W2 = 8192, FR_N1 = 12288, FR_N2 = 20480, FR_SZ = 24576;
+
+// dst[((j * KS + ks) * 64 + l) * 8 + t] = bf16(W[64 (ks / 2) + frag_ch(ks % 2, l / 16, t)][16 j + l % 16])
+__global__ void k_pack_frag(bf16_t *dst, const float *W, int KS)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * KS * 64 * 8) return;
+    const int t = i & 7, l = (i >> 3) & 63, q = i >> 9, ks = q % KS, j = q / KS;
+    const int kin = 64 * (ks >> 1) + frag_ch(ks & 1, l >> 4, t);
+    dst[i] = to_bf16(W[kin * MH + 16 * j + (l & 15)]);
+}
+
+__device__ __forceinline__ bf16x8 ld_frag(const bf16_t *h, int cell, int ks, int g)
+{
+    return ((const bf16x8 *)(h + (size_t)cell * MH))[ks * 4 + g];
+}
+
+__device__ __forceinline__ bf16x8 zero_frag()
+{
+    bf16x8 z;
+#pragma unroll
+    for (int t = 0; t < 8; t++) z[t] = (__bf16)0.0f;
+    return z;
+}
+
+__device__ __forceinline__ __bf16 bf(float x)
+{
+    const bf16_t u = to_bf16(x);
+    __bf16 b;
+    __builtin_memcpy(&b, &u, 2);
+    return b;
+}
+__device__ __forceinline__ float fl(__bf16 b)
+{
+    bf16_t u;
+    __builtin_memcpy(&u, &b, 2);
+    return from_bf16(u);
+}
+
+// C layout (4 row tiles) -> the two B fragments of the next product
+__device__ __forceinline__ void to_frags(const float4v (&c)[4], bf16x8 (&b)[2])
+{
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int t = 0; t < 8; t++) b[ks][t] = bf(c[2 * ks + (t >> 2)][t & 3]);
+}
+
+// out[j] = sum_ks A(j, ks) B(ks), A fragments from LDS
+template <int KS>
+__device__ __forceinline__ void mm_lds(const bf16x8 *A, const bf16x8 (&B)[KS], float4v (&out)[4], int lane)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        out[j] = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++)
+            out[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[(j * KS + ks) * 64 + lane], B[ks], out[j], 0, 0, 0);
+    }
+}
+
+// h0 = relu(f Wenc + benc) in fragment order; one thread per (cell, 8 channels)
+__global__ void __launch_bounds__(256) k_mp_encode(int n, const float *f, const float *W, bf16_t *h)
+{
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const int i = idx >> 3, q = idx & 7, ks = q >> 2, g = q & 3;
+    if (i >= n) return;
+    const float4 fv = *(const float4 *)(f + (size_t)4 * i);
+    const float x[4] = {from_bf16(to_bf16(fv.x)), from_bf16(to_bf16(fv.y)), from_bf16(to_bf16(fv.z)),
+                        from_bf16(to_bf16(fv.w))};
+    bf16x8 o;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int c = frag_ch(ks, g, t);
+        float v = 0.f;
+#pragma unroll
+        for (int k = 0; k < MFIN; k++) v += x[k] * from_bf16(to_bf16(W[OFF_WENC + k * MH + c]));
+        v += W[OFF_BENC + c];
+        o[t] = bf(fmaxf(v, 0.f));
+    }
+    ((bf16x8 *)(h + (size_t)i * MH))[ks * 4 + g] = o;
+}
+
+// One message-passing layer for every cell (LAST: the decoder too, y only).
+// Persistent: workgroup b serves the bricks of XCD b % 8's contiguous range
+// (the grid is a multiple of 8);
+// its four waves take the brick's 16 tiles of 16 cells in turn.
+template <bool LAST>
+__global__ void __launch_bounds__(256, 3) k_mp_layer(CellWalk w, int nbricks, const bf16_t *hin, bf16_t *hout,
+                                                    const int *nbr, const float *e, const float *deg,
+                                                    const bf16_t *wf, const float *Wl, const float *Wall, float *y)
+{
+    __shared__ bf16x8 sW[FR_SZ / 8];
+    __shared__ float sB[6][MH];   // we, b1, b2, bn1, bn2, bf16(wdec)
+    for (int i = threadIdx.x; i < FR_SZ / 8; i += 256) sW[i] = ((const bf16x8 *)wf)[i];
+    for (int i = threadIdx.x; i < MH; i += 256) {
+        sB[0][i] = Wl[L_WE + i]; sB[1][i] = Wl[L_B1 + i]; sB[2][i] = Wl[L_B2 + i];
+        sB[3][i] = Wl[L_BN1 + i]; sB[4][i] = Wl[L_BN2 + i];
+        sB[5][i] = from_bf16(to_bf16(Wall[OFF_WDEC + i]));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    __syncthreads();
+    const int nxcd = 8, xcd = blockIdx.x % nxcd, local = blockIdx.x / nxcd, nloc = gridDim.x / nxcd;
+    const int per = (nbricks + nxcd - 1) / nxcd;
+    const int bend = min(nbricks, (xcd + 1) * per);
+    for (int bi = xcd * per + local; bi < bend; bi += nloc) {
+        for (int tile = wave; tile < 16; tile += 4) {
+            const int cell = walk_cell_at(w, bi, tile * 16 + r);
+            const bool valid = cell >= 0;
+            // an offset the compiler cannot see through: the weight fragments
+            // are loop-invariant LDS loads, and hoisted out of the tile loop
+            // they would hold 352 registers; read at each use instead
+            int z0 = 0;
+            asm volatile("" : "+s"(z0));
+            const bf16x8 *fW = sW + z0;
+            const float(*vB)[MH] = (const float(*)[MH])(&sB[0][0] + z0);
+            bf16x8 hb[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) hb[ks] = valid ? ld_frag(hin, cell, ks, g) : zero_frag();
+            // P = h Wa
+            float4v P[4];
+            mm_lds<2>(fW + FR_WA / 8, hb, P, lane);
+            // s = sum over neighbours of relu(P + Q_j + e we + b1); the six
+            // neighbours' gathers are all in flight before the first product
+            float4v S[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) S[j] = float4v{0.f, 0.f, 0.f, 0.f};
+            int nb[6];
+            float es[6];
+            bf16x8 hn[6][2];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                nb[q] = valid ? nbr[6 * cell + q] : -1;
+                es[q] = valid ? e[6 * cell + q] : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+#pragma unroll
+                for (int ks = 0; ks < 2; ks++) hn[q][ks] = nb[q] >= 0 ? ld_frag(hin, nb[q], ks, g) : zero_frag();
+            {
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    float4v Q[4];
+                    mm_lds<2>(fW + FR_WB / 8, hn[q], Q, lane);
+                    if (nb[q] >= 0) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+#pragma unroll
+                            for (int t = 0; t < 4; t++) {
+                                const int ch = 16 * j + 4 * g + t;
+                                const float z = ((P[j][t] + Q[j][t]) + es[q] * vB[0][ch]) + vB[1][ch];
+                                S[j][t] += fmaxf(z, 0.f);
+                            }
+                    }
+                }
+            }
+            // a = s W2 + deg b2
+            bf16x8 sb[2];
+            to_frags(S, sb);
+            float4v A[4];
+            mm_lds<2>(fW + FR_W2 / 8, sb, A, lane);
+            const float dg = valid ? deg[cell] : 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) A[j][t] += dg * vB[2][16 * j + 4 * g + t];
+            // u = relu([h, a] Wn1 + bn1)
+            bf16x8 ha[4];
+            ha[0] = hb[0]; ha[1] = hb[1];
+            {
+                bf16x8 ab[2];
+                to_frags(A, ab);
+                ha[2] = ab[0]; ha[3] = ab[1];
+            }
+            float4v U[4];
+            mm_lds<4>(fW + FR_N1 / 8, ha, U, lane);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) U[j][t] = fmaxf(U[j][t] + vB[3][16 * j + 4 * g + t], 0.f);
+            // h' = h + u Wn2 + bn2
+            bf16x8 ub[2];
+            to_frags(U, ub);
+            float4v O[4];
+            mm_lds<2>(fW + FR_N2 / 8, ub, O, lane);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const float hv = fl(hb[j >> 1][4 * (j & 1) + t]);
+                    O[j][t] = (hv + O[j][t]) + vB[4][16 * j + 4 * g + t];
+                }
+            if constexpr (LAST) {
+                // y = bf16(h') . bf16(wdec) + bdec, the four lane groups' partial sums added
+                float part = 0.f;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) part += from_bf16(to_bf16(O[j][t])) * vB[5][16 * j + 4 * g + t];
+                part += __shfl_xor(part, 16, 64);
+                part += __shfl_xor(part, 32, 64);
+                if (valid && g == 0) y[cell] = part + Wall[OFF_BDEC];
+            } else {
+                bf16x8 ho[2];
+                to_frags(O, ho);
+                if (valid) {
+#pragma unroll
+                    for (int ks = 0; ks < 2; ks++) ((bf16x8 *)(hout + (size_t)cell * MH))[ks * 4 + g] = ho[ks];
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------- graph set-up --
 __global__ void k_graph(int N, const int *nbr6, int *nbr, int *rev, float *deg)
 {
@@ -732,6 +844,45 @@ __global__ void k_guess_out(int N, const float *y, double *g)
     if (i < N) g[i] = (double)y[i];
 }
 
+static void pack_weights(Ctx *c)
+{
+    Mpnn *m = M(c);
+    auto T = [&](bf16_t *dst, int kp, const float *src, int rows, int cols) {
+        hipLaunchKernelGGL(k_pack_t, dim3(div_up(cols * kp, 256)), dim3(256), 0, c->stream, dst, kp, src, rows, cols);
+    };
+    auto R = [&](bf16_t *dst, int ld, int koff, const float *src, int rows, int cols) {
+        hipLaunchKernelGGL(k_pack_rows, dim3(div_up(rows * cols, 256)), dim3(256), 0, c->stream, dst, ld, koff, src,
+                           rows, cols);
+    };
+    for (int l = 0; l < ML; l++) {
+        const float *Wl = m->w + OFF_L0 + l * LSZ;
+        bf16_t *P = m->wb + l * PSZ;
+        T(P + P_ABT, MH, Wl + L_WA, MH, MH);               // rows m < 64: Wa^T
+        T(P + P_ABT + MH * MH, MH, Wl + L_WB, MH, MH);     // rows m >= 64: Wb^T
+        T(P + P_W2T, MH, Wl + L_W2, MH, MH);
+        T(P + P_N1T, 2 * MH, Wl + L_WN1, 2 * MH, MH);
+        T(P + P_N2T, MH, Wl + L_WN2, MH, MH);
+        R(P + P_ABR, 2 * MH, 0, Wl + L_WA, MH, MH);        // [Wa | Wb] row-major (64 x 128)
+        R(P + P_ABR, 2 * MH, MH, Wl + L_WB, MH, MH);
+        R(P + P_W2R, MH, 0, Wl + L_W2, MH, MH);
+        R(P + P_N1R, MH, 0, Wl + L_WN1, 2 * MH, MH);
+        R(P + P_N2R, MH, 0, Wl + L_WN2, MH, MH);
+    }
+    T(m->wb + P_ENCT, 16, m->w + OFF_WENC, MFIN, MH);
+    auto Fr = [&](bf16_t *dst, const float *src, int KS) {
+        hipLaunchKernelGGL(k_pack_frag, dim3(div_up(4 * KS * 64 * 8, 256)), dim3(256), 0, c->stream, dst, src, KS);
+    };
+    for (int l = 0; l < ML; l++) {
+        const float *Wl = m->w + OFF_L0 + l * LSZ;
+        bf16_t *P = m->wf + l * FR_SZ;
+        Fr(P + FR_WA, Wl + L_WA, 2);
+        Fr(P + FR_WB, Wl + L_WB, 2);
+        Fr(P + FR_W2, Wl + L_W2, 2);
+        Fr(P + FR_N1, Wl + L_WN1, 4);
+        Fr(P + FR_N2, Wl + L_WN2, 2);
+    }
+}
+
 // --------------------------------------------------------------- host --
 static int ensure(Ctx *c)
 {
@@ -749,10 +900,14 @@ static int ensure(Ctx *c)
     MC_TRY(dalloc(c, &m->pmap, 6 * (int64_t)N)); MC_TRY(dalloc(c, &m->deg, N));
     MC_TRY(dalloc(c, &m->f, 4 * (int64_t)N)); MC_TRY(dalloc(c, &m->e, 6 * (int64_t)N));
     MC_TRY(dalloc(c, &m->y, N)); MC_TRY(dalloc(c, &m->dy, N));
-    MC_TRY(dalloc(c, &m->H[0], (int64_t)N * MH)); MC_TRY(dalloc(c, &m->H[1], (int64_t)N * MH));
-    MC_TRY(dalloc(c, &m->PQ[0], (int64_t)N * 2 * MH)); MC_TRY(dalloc(c, &m->S[0], (int64_t)N * MH));
-    MC_TRY(dalloc(c, &m->A[0], (int64_t)N * MH)); MC_TRY(dalloc(c, &m->U[0], (int64_t)N * MH));
+    MC_TRY(dalloc(c, &m->wf, ML * FR_SZ));
+    MC_TRY(dalloc(c, &m->hb[0], (int64_t)N * MH)); MC_TRY(dalloc(c, &m->hb[1], (int64_t)N * MH));
     MC_TRY(dalloc(c, &m->sc, 16));
+    {
+        int dev = 0;
+        MC_HIP(c, hipGetDevice(&dev));
+        MC_HIP(c, hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     MC_HIP(c, hipMemsetAsync(m->am, 0, NPARAMS * sizeof(float), c->stream));
     MC_HIP(c, hipMemsetAsync(m->av, 0, NPARAMS * sizeof(float), c->stream));
     hipLaunchKernelGGL(k_graph, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->nbr, m->nbr, m->rev, m->deg);
@@ -767,8 +922,8 @@ static int ensure_train(Ctx *c)
     if (m->train_bufs) return MC_OK;
     MemScope ms(c, MC_MEM_MPNN);
     const int64_t N = m->n;
-    for (int l = 2; l <= ML; l++) MC_TRY(dalloc(c, &m->H[l], N * MH));
-    for (int l = 1; l < ML; l++) {
+    for (int l = 0; l <= ML; l++) MC_TRY(dalloc(c, &m->H[l], N * MH));
+    for (int l = 0; l < ML; l++) {
         MC_TRY(dalloc(c, &m->PQ[l], N * 2 * MH)); MC_TRY(dalloc(c, &m->S[l], N * MH));
         MC_TRY(dalloc(c, &m->A[l], N * MH)); MC_TRY(dalloc(c, &m->U[l], N * MH));
     }
@@ -781,8 +936,9 @@ static int ensure_train(Ctx *c)
     return MC_OK;
 }
 
-// forward pass on features f / edge features e; train = keep every layer
-static void forward(Ctx *c, const float *f, const float *e, bool train)
+// Training forward pass on features f / edge features e: FP32 activations,
+// every layer's intermediates kept for the backward pass.
+static void forward_train(Ctx *c, const float *f, const float *e)
 {
     Mpnn *m = M(c);
     const int n = m->n;
@@ -794,38 +950,65 @@ static void forward(Ctx *c, const float *f, const float *e, bool train)
     for (int l = 0; l < ML; l++) {
         const float *Wl = W + OFF_L0 + l * LSZ;
         const bf16_t *Pk = m->wb + l * PSZ;
-        float *hin = train ? m->H[l] : m->H[l & 1];
-        float *hout = train ? m->H[l + 1] : m->H[(l + 1) & 1];
-        float *PQ = train ? m->PQ[l] : m->PQ[0], *S = train ? m->S[l] : m->S[0];
-        float *A = train ? m->A[l] : m->A[0], *Uu = train ? m->U[l] : m->U[0];
+        float *hin = m->H[l], *hout = m->H[l + 1];
         hipLaunchKernelGGL((k_mm<MH, 0, MH, 2 * MH, 0>), gmm, blk, 0, c->stream, n, hin, MH, (const float *)nullptr, 0,
                            Pk + P_ABT, (const float *)nullptr, (const float *)nullptr, (const float *)nullptr, 0,
-                           (const float *)nullptr, 0, PQ, 2 * MH);
-        hipLaunchKernelGGL(k_edge_fwd, dim3(div_up((int64_t)n * 16, 256)), blk, 0, c->stream, n, PQ, m->nbr, e,
-                           Wl + L_WE, Wl + L_B1, S);
-        if (!train) {
-            hipLaunchKernelGGL(k_node_fused, gmm, blk, 0, c->stream, n, (const float *)S, (const float *)hin, m->deg,
-                               Pk + P_W2T, Wl + L_B2, Pk + P_N1T, Wl + L_BN1, Pk + P_N2T, Wl + L_BN2, hout);
-            continue;
-        }
-        hipLaunchKernelGGL((k_mm<MH, 0, MH, MH, EPI_BIAS | EPI_ROWSCALE>), gmm, blk, 0, c->stream, n, S, MH,
+                           (const float *)nullptr, 0, m->PQ[l], 2 * MH);
+        hipLaunchKernelGGL(k_edge_fwd, dim3(div_up((int64_t)n * 16, 256)), blk, 0, c->stream, n, m->PQ[l], m->nbr, e,
+                           Wl + L_WE, Wl + L_B1, m->S[l]);
+        hipLaunchKernelGGL((k_mm<MH, 0, MH, MH, EPI_BIAS | EPI_ROWSCALE>), gmm, blk, 0, c->stream, n, m->S[l], MH,
                            (const float *)nullptr, 0, Pk + P_W2T, Wl + L_B2, m->deg, (const float *)nullptr, 0,
-                           (const float *)nullptr, 0, A, MH);
-        hipLaunchKernelGGL((k_mm<MH, MH, 2 * MH, MH, EPI_BIAS | EPI_RELU>), gmm, blk, 0, c->stream, n, hin, MH, A, MH,
-                           Pk + P_N1T, Wl + L_BN1, (const float *)nullptr, (const float *)nullptr, 0,
-                           (const float *)nullptr, 0, Uu, MH);
-        hipLaunchKernelGGL((k_mm<MH, 0, MH, MH, EPI_RESID | EPI_BIAS>), gmm, blk, 0, c->stream, n, Uu, MH,
+                           (const float *)nullptr, 0, m->A[l], MH);
+        hipLaunchKernelGGL((k_mm<MH, MH, 2 * MH, MH, EPI_BIAS | EPI_RELU>), gmm, blk, 0, c->stream, n, hin, MH, m->A[l],
+                           MH, Pk + P_N1T, Wl + L_BN1, (const float *)nullptr, (const float *)nullptr, 0,
+                           (const float *)nullptr, 0, m->U[l], MH);
+        hipLaunchKernelGGL((k_mm<MH, 0, MH, MH, EPI_RESID | EPI_BIAS>), gmm, blk, 0, c->stream, n, m->U[l], MH,
                            (const float *)nullptr, 0, Pk + P_N2T, Wl + L_BN2, (const float *)nullptr, hin, MH,
                            (const float *)nullptr, 0, hout, MH);
     }
-    const float *hl = train ? m->H[ML] : m->H[ML & 1];
-    hipLaunchKernelGGL(k_decode, dim3(div_up(n, 64)), blk, 0, c->stream, n, hl, W, m->y);
+    hipLaunchKernelGGL(k_decode, dim3(div_up(n, 64)), blk, 0, c->stream, n, m->H[ML], W, m->y);
 }
 
-// MFMA work of one forward pass (flops)
+// Inference forward pass into m->y: the encoder, then one fused kernel per
+// layer on BF16 activations (the last with the decoder).
+static void forward_infer(Ctx *c, const float *f, const float *e)
+{
+    Mpnn *m = M(c);
+    const int n = m->n;
+    hipLaunchKernelGGL(k_mp_encode, dim3(div_up((int64_t)n * 8, 256)), dim3(256), 0, c->stream, n, f, m->w, m->hb[0]);
+    const CellWalk w = cell_walk(c);
+    const int nbricks = walk_blocks(w);
+    // three resident workgroups per compute unit (134 registers, 50 KB of
+    // LDS each), a multiple of the 8 XCDs
+    const int per_xcd = std::max(1, std::min(div_up(nbricks, 8), 3 * std::max(m->cus, 8) / 8));
+    const dim3 grid(8 * per_xcd), blk(256);
+    for (int l = 0; l < ML; l++) {
+        const bf16_t *hin = m->hb[l & 1];
+        bf16_t *hout = m->hb[(l + 1) & 1];
+        const bf16_t *wf = m->wf + l * FR_SZ;
+        const float *Wl = m->w + OFF_L0 + l * LSZ;
+        if (l < ML - 1)
+            hipLaunchKernelGGL(k_mp_layer<false>, grid, blk, 0, c->stream, w, nbricks, hin, hout, m->nbr, e, m->deg,
+                               wf, Wl, (const float *)m->w, (float *)nullptr);
+        else
+            hipLaunchKernelGGL(k_mp_layer<true>, grid, blk, 0, c->stream, w, nbricks, hin, (bf16_t *)nullptr, m->nbr, e,
+                               m->deg, wf, Wl, (const float *)m->w, m->y);
+    }
+}
+
+// MFMA work of one training forward pass (flops)
 static double forward_flops(int n)
 {
     return (double)n * (2.0 * 16 * MH + ML * 2.0 * (MH * 2 * MH + MH * MH + 2 * MH * MH + MH * MH));
+}
+
+// MFMA work of one fused inference pass (flops): per cell and layer the
+// products with Wa, W2, Wn2 (64 x 64), Wn1 (128 x 64) and Wb once for each of
+// the six neighbour slots (recomputed, not stored); the 4 -> 64 encoder runs
+// on the vector units and is not counted
+static double infer_flops(int n)
+{
+    return (double)n * ML * 2.0 * MH * MH * (1 + 6 + 1 + 2 + 1);
 }
 
 template <int K1, int K2, int KP, int MM>
@@ -936,8 +1119,8 @@ int mpnn_guess(Ctx *c, const double *b, double *g, int *ok)
     if (!m || !m->ready) return MC_OK;
     MC_TRY(features(c, b, m->f, m->e));
     {
-        KTimer kt(c, MC_KC_MPNN, forward_flops(c->N));
-        forward(c, m->f, m->e, false);
+        KTimer kt(c, MC_KC_MPNN, infer_flops(c->N));
+        forward_infer(c, m->f, m->e);
     }
     hipLaunchKernelGGL(k_guess_out, dim3(div_up(c->N, 256)), dim3(256), 0, c->stream, c->N, m->y, g);
     MC_LAUNCH_CHECK(c);
@@ -989,7 +1172,7 @@ static int grad_of_pair(Ctx *c, int64_t pair, double *loss_out)
     MC_TRY(ensure_train(c));
     {
         KTimer kt(c, MC_KC_MPNN, forward_flops(c->N));
-        forward(c, f, e, true);
+        forward_train(c, f, e);
     }
     loss(c, t);
     {
@@ -1125,8 +1308,8 @@ int mc_mpnn_forward_pair(mc_ctx *x, int64_t pair, float *y_host)
     const float *f, *e, *t;
     MC_TRY(pair_ptrs(c, pair, &f, &e, &t));
     {
-        KTimer kt(c, MC_KC_MPNN, forward_flops(c->N));
-        forward(c, f, e, false);
+        KTimer kt(c, MC_KC_MPNN, infer_flops(c->N));
+        forward_infer(c, f, e);
     }
     MC_LAUNCH_CHECK(c);
     MC_HIP(c, hipMemcpyAsync(y_host, m->y, (size_t)c->N * sizeof(float), hipMemcpyDeviceToHost, c->stream));

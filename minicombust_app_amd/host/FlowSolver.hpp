@@ -24,6 +24,8 @@
 
 #include <chrono>
 #include <filesystem>
+#include <initializer_list>
+#include <utility>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>
@@ -70,6 +72,7 @@ class FlowSolver {
         for (int s = 0; s < 6; s++) d.boundary_types[s] = (uint64_t)mesh->boundary_types[s];
         mc_options o = options(delta, opts);
         check(mc_ctx_create(&d, &o, &ctx_), "mc_ctx_create");
+        check(mc_stage_enable(ctx_, 1), "mc_stage_enable");
     }
 
     // box mesh of the given modifier (src/minicombust.cpp:123-124)
@@ -78,6 +81,7 @@ class FlowSolver {
     {
         mc_options o = options(delta, opts);
         check(mc_ctx_create_box((int64_t)modifier, &o, &ctx_), "mc_ctx_create_box");
+        check(mc_stage_enable(ctx_, 1), "mc_stage_enable");
     }
 
     // a host mesh in the reference Mesh<T> layout (include/geometry/Mesh.hpp:70-107)
@@ -86,6 +90,7 @@ class FlowSolver {
     {
         mc_options o = options(delta, opts);
         check(mc_ctx_create(&mesh, &o, &ctx_), "mc_ctx_create");
+        check(mc_stage_enable(ctx_, 1), "mc_stage_enable");
     }
 
     ~FlowSolver()
@@ -127,9 +132,41 @@ class FlowSolver {
         fprintf(out_, "\tScalar BiCGStab iters:     %" PRId64 "\n", s.scalar_iters);
         fprintf(out_, "\tPressure solves / iters:   %" PRId64 " / %" PRId64 "\n", s.p_solves, s.p_iters);
         fprintf(out_, "\tPressure solve time:       %.6fs\n", s.p_solve_ms * 1e-3);
+        fprintf(out_, "\tAMG setups / time:         %" PRId64 " / %.6fs\n", s.amg_setups, s.amg_setup_ms * 1e-3);
         fprintf(out_, "\tMax pressure rel. residual %.3e\n", s.p_relres_max);
         fprintf(out_, "\tFlow Compute Time:         %.6fs\n", compute_time);
         fprintf(out_, "\tProgram time:              %.6fs\n", runtime);
+        print_stage_times();
+    }
+
+    // the per-stage breakdown the reference prints with FLOW_SOLVER_FINE_TIME
+    // (velocity: FlowSolver.inl:1768-1806; pressure and scalars alike): GPU
+    // time of each stage from HIP events, seconds and share of its equation
+    // group (flux and matrix set-up are one fused pass here)
+    void print_stage_times()
+    {
+        double ms[MC_ST_COUNT] = {};
+        int64_t calls[MC_ST_COUNT] = {};
+        check(mc_stage_read(ctx_, ms, calls), "mc_stage_read");
+        auto group = [&](const char *title, std::initializer_list<std::pair<const char *, int>> rows) {
+            double tot = 0.0;
+            for (auto &r : rows) tot += ms[r.second];
+            fprintf(out_, "\n%s:\n", title);
+            for (auto &r : rows)
+                fprintf(out_, "%s: %.3f (%.2f %%)\n", r.first, ms[r.second] * 1e-3,
+                        tot > 0.0 ? 100.0 * ms[r.second] / tot : 0.0);
+            fprintf(out_, "Total time: %f\n", tot * 1e-3);
+        };
+        group("Gradient Timings", {{"Gradients (11 fields) time", MC_ST_GRAD}});
+        group("Velocity Field Solve Timings", {{"Compute Flux + Matrix Setup time", MC_ST_VEL_FLUX},
+                                               {"Matrix Solve time", MC_ST_VEL_SOLVE}});
+        group("Pressure Solve Timings", {{"Compute Flux + Matrix Setup time", MC_ST_PRES_FLUX},
+                                         {"Matrix Solve time", MC_ST_PRES_SOLVE},
+                                         {"Correction time", MC_ST_PRES_CORRECT}});
+        group("Turbulence (TE, ED) Solve Timings", {{"Compute Flux + Matrix Setup time", MC_ST_TURB_FLUX},
+                                                   {"Matrix Solve time", MC_ST_TURB_SOLVE}});
+        group("Scalar (TEM, FUL, PRO, VARF, VARP) Solve Timings",
+              {{"Compute Flux + Matrix Setup time", MC_ST_SCA_FLUX}, {"Matrix Solve time", MC_ST_SCA_SOLVE}});
     }
 
     // host copy of a field's N cell values (field ids: include/mcflow.h)

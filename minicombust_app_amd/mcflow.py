@@ -40,6 +40,10 @@ KC_KERNELS = {"grad11": "mc::k_grad<11, false>", "grad1": "mc::k_grad<1, false>"
               "pcg_spmv32": "mc::k_pcg_pq<7, float>", "flux_correct": "mc::k_flux_correct",
               "scalar_batch": "mc::k_scalar_batch<true>", "ir_resid": "mc::k_ir_resid<-7>"}
 KC_COUNT = len(KC_NAMES)
+# stages of the SIMPLE iteration (MC_ST_*)
+ST_NAMES = ["grad", "vel_flux", "vel_solve", "pres_flux", "pres_solve", "pres_correct", "turb_flux", "turb_solve",
+            "sca_flux", "sca_solve"]
+ST_COUNT = len(ST_NAMES)
 MEM_NAMES = ["mesh", "fields", "matrix", "amg", "work", "mpnn", "dataset"]
 
 
@@ -69,7 +73,7 @@ class Stats(C.Structure):
         ("last_loops", C.c_int), ("p_relres_max", C.c_double), ("clip_cells", C.c_int64),
         ("clip_passes", C.c_int64), ("host_polls", C.c_int64), ("p_guesses", C.c_int64),
         ("p_guess_ratio_sum", C.c_double), ("p_defl_starts", C.c_int64), ("p_defl_ratio_sum", C.c_double),
-        ("p_defl_vectors", C.c_int64),
+        ("p_defl_vectors", C.c_int64), ("amg_setups", C.c_int64), ("amg_setup_ms", C.c_double),
     ]
 
 
@@ -108,6 +112,7 @@ def lib():
         L.mc_set_options.argtypes = [P_, C.POINTER(Options)]
         for n in ["mc_timestep", "mc_get_phi_gradients", "mc_limit_phi_gradients", "mc_set_up_field", "mc_momentum_assemble",
                   "mc_pressure_assemble", "mc_calculate_pressure", "mc_update_P", "mc_sync", "mc_profile_reset",
+                  "mc_stage_reset",
                   "mc_scalar_batch_solve", "mc_reset_fields"]:
             getattr(L, n).argtypes = [P_]
         L.mc_simple_iteration.argtypes = [P_, C.c_int]
@@ -133,6 +138,8 @@ def lib():
         L.mc_set_state_flags.argtypes = [P_, C.c_int, C.c_int64]
         L.mc_profile_enable.argtypes = [P_, C.c_int]
         L.mc_profile_read.argtypes = [P_, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.mc_stage_enable.argtypes = [P_, C.c_int]
+        L.mc_stage_read.argtypes = [P_, C.c_void_p, C.c_void_p]
         L.mc_copy_probe.argtypes = [P_, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.mc_ctx_memory.argtypes = [P_, C.c_void_p]
         L.mc_mpnn_nparams.restype = C.c_int64
@@ -467,6 +474,19 @@ class Context:
 
     def profile_reset(self):
         self._check(self.L.mc_profile_reset(self.h), "mc_profile_reset")
+
+    def stage_enable(self, on=True):
+        """Time the SIMPLE iteration's stages (MC_ST_*) with HIP events."""
+        self._check(self.L.mc_stage_enable(self.h, 1 if on else 0), "mc_stage_enable")
+
+    def stage_reset(self):
+        self._check(self.L.mc_stage_reset(self.h), "mc_stage_reset")
+
+    def stage_read(self):
+        ms = np.zeros(ST_COUNT)
+        calls = np.zeros(ST_COUNT, dtype=np.int64)
+        self._check(self.L.mc_stage_read(self.h, ms.ctypes.data, calls.ctypes.data), "mc_stage_read")
+        return {ST_NAMES[i]: {"ms": float(ms[i]), "calls": int(calls[i])} for i in range(ST_COUNT)}
 
     def profile_read(self):
         ms = np.zeros(KC_COUNT)

@@ -24,7 +24,10 @@ the boundary); messages run both ways over every interior face.
     loss = mean((y - mean(y) - t)^2),  t = x* / rms(x*) (x* mean-zero)
 
 GEMM operands are rounded to bfloat16 (round to nearest even) with float32
-accumulation when `bf16=True`, as the MFMA kernels compute them.
+accumulation when `bf16_ops=True`, as the MFMA kernels compute them; with
+`bf16_act=True` the activations h are also stored in bfloat16 after the
+encoder and after every layer (the residual adds the stored value), as the
+fused inference kernels keep them.
 """
 import numpy as np
 
@@ -98,7 +101,7 @@ def features(b, diag, vol, q):
     return f, s_b, s_d
 
 
-def forward(flat, f, nbr, e, bf16_ops=False):
+def forward(flat, f, nbr, e, bf16_ops=False, bf16_act=False):
     """f: (N, 4) features; nbr: (N, 6) neighbour ids (-1 none); e: (N, 6) edge
     features.  Returns y (N,) and the cache the backward needs."""
     p = unpack(flat)
@@ -110,6 +113,8 @@ def forward(flat, f, nbr, e, bf16_ops=False):
     deg = mask.sum(axis=1).astype(dt)
     c = {"f": f, "mask": mask, "nb": nb, "deg": deg, "e": e}
     h = np.maximum(_mm(f, p["Wenc"].astype(dt), bf16_ops) + p["benc"].astype(dt), 0)
+    if bf16_act:
+        h = bf16(h)
     c["h0"] = h
     for l in range(LAYERS):
         P = _mm(h, p["Wa%d" % l].astype(dt), bf16_ops)
@@ -123,7 +128,7 @@ def forward(flat, f, nbr, e, bf16_ops=False):
         u = np.maximum(zu, 0)
         hn = h + _mm(u, p["Wn2%d" % l].astype(dt), bf16_ops) + p["bn2%d" % l].astype(dt)
         c[l] = {"h": h, "P": P, "Q": Q, "z": z, "s": s, "ha": ha, "zu": zu, "u": u}
-        h = hn
+        h = bf16(hn) if bf16_act else hn
     c["hL"] = h
     y = _mm(h, p["wdec"].astype(dt)[:, None], bf16_ops)[:, 0] + p["bdec"].astype(dt)[0]
     return y, c

@@ -55,9 +55,10 @@ def _c_layout():
 int main(void) {
   S(mc_options) S(mc_solve_status) S(mc_stats) S(mc_mesh_desc)
   O(mc_options, delta) O(mc_options, p_rtol) O(mc_options, amg_cscale) O(mc_options, check_every)
-  O(mc_stats, p_solve_ms) O(mc_stats, host_polls) O(mc_solve_status, bnorm)
+  O(mc_stats, p_solve_ms) O(mc_stats, host_polls) O(mc_stats, amg_setup_ms) O(mc_solve_status, bnorm)
   O(mc_mesh_desc, cell_faces) O(mc_mesh_desc, boundary_types)
   printf("MC_KC_COUNT %d\n", (int)MC_KC_COUNT);
+  printf("MC_ST_COUNT %d\n", (int)MC_ST_COUNT);
   return 0;
 }
 """
@@ -82,10 +83,12 @@ def test_ctypes_structs_match_the_header():
     assert mc.Options.check_every.offset == lay["mc_options.check_every"]
     assert mc.Stats.p_solve_ms.offset == lay["mc_stats.p_solve_ms"]
     assert mc.Stats.host_polls.offset == lay["mc_stats.host_polls"]
+    assert mc.Stats.amg_setup_ms.offset == lay["mc_stats.amg_setup_ms"]
     assert mc.SolveStatus.bnorm.offset == lay["mc_solve_status.bnorm"]
     assert mc.MeshDesc.cell_faces.offset == lay["mc_mesh_desc.cell_faces"]
     assert mc.MeshDesc.boundary_types.offset == lay["mc_mesh_desc.boundary_types"]
     assert mc.KC_COUNT == lay["MC_KC_COUNT"]
+    assert mc.ST_COUNT == lay["MC_ST_COUNT"]
 
 
 def test_options_default():

@@ -13,7 +13,13 @@ the comparison; what differs is the pressure solve (two preconditioners, each
 to a 1e-8 relative residual) and rounding.  A pressure-correction residual of
 1e-8 moves P, U, V, W by far less than 1e-8 of their maxima (P ~ 4000, U ~ 50);
 the fields are compared at 1e-7 of each field's maximum, a margin of 10x over
-what the solves alone allow, and the measured figure is printed.
+what the solves alone allow, and the measured figure is printed.  The
+turbulence and scalar solves stop at their 1e-5 residual after one to three
+iterations: when rounding (parallel against sequential dot products) moves
+one of their stopping tests across its threshold, the two sides take a
+different number of iterations and that field moves by up to the solve's own
+tolerance.  Those fields are held to 1e-7 when the two sides' scalar
+iteration totals agree and to 1e-5 otherwise (both totals are printed).
 """
 import numpy as np
 import pytest
@@ -52,6 +58,7 @@ def test_bench_configuration_m128_against_oracle():
         o.simple_iteration(it)
     ref_phi = {f: o.phi(f).copy() for f in FIELDS}
     ref_mflux = o.face("mflux").copy()
+    ref_its = (o.s.stats.mom_iters, o.s.stats.scalar_iters)
     o.close()
     for psolve in (mc.PSOLVE_FP64, mc.PSOLVE_MIXED):
         print("bench configuration m=%d: psolve %d" % (m, psolve), flush=True)
@@ -64,8 +71,12 @@ def test_bench_configuration_m128_against_oracle():
             errs = _rel_errors(g, ref_phi, ref_mflux)
             print("psolve %d: max |gpu - oracle| / max |oracle|: %s" % (
                 psolve, ", ".join("%s %.1e" % kv for kv in errs.items())), flush=True)
+            print("BiCGStab iterations (momentum, scalars): gpu (%d, %d), oracle (%d, %d)"
+                  % (st["mom_iters"], st["scalar_iters"], ref_its[0], ref_its[1]), flush=True)
+            scalar_tol = 1e-7 if st["scalar_iters"] == ref_its[1] else 1e-5
             for k, e in errs.items():
-                assert e <= 1e-7, (psolve, k, e)
+                tol = 1e-7 if k in ("U", "V", "W", "P", "mflux") else scalar_tol
+                assert e <= tol, (psolve, k, e)
 
 
 @pytest.mark.timeout(600)

@@ -39,7 +39,8 @@ def test_start_is_the_galerkin_projection(psolve):
     newest loop-1 solutions are read back; the next loop-1 system is
     assembled stage by stage and solved: the library's start residual ratio
     equals numpy's Galerkin projection onto those two vectors to 1e-6
-    (relative), and the solve meets 1e-8."""
+    (relative), the projection leaves under a tenth of the zero start's
+    residual, and the solve meets 1e-8."""
     m = 16
     print("deflation start m=%d psolve %d" % (m, psolve), flush=True)
     with mc.Context(m, psolve=psolve, p_recycle=2, **BENCH) as g:
@@ -63,7 +64,7 @@ def test_start_is_the_galerkin_projection(psolve):
         assert s1["p_defl_vectors"] - s0["p_defl_vectors"] == 2
         ratio = s1["p_defl_ratio_sum"] - s0["p_defl_ratio_sum"]
         print("start residual ratio: library %.6e, numpy %.6e" % (ratio, ratio_np), flush=True)
-        assert ratio_np < 1e-2
+        assert ratio_np < 0.1
         assert abs(ratio - ratio_np) <= 1e-6 * ratio_np + 1e-14
         assert st.converged and st.relres <= 1e-8 * 1.0001
         x = g.phi(po.PP)[:g.N]

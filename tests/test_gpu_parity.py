@@ -636,6 +636,8 @@ def test_cpp_driver_matches_the_library(tmp_path):
     out = subprocess.run([exe], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "Flow Compute Time" in out.stdout
+    for title in ("Velocity Field Solve Timings", "Pressure Solve Timings", "Scalar (TEM, FUL, PRO, VARF, VARP)"):
+        assert title in out.stdout, title
     drv = (tmp_path / "out" / "flow" / "minicombust_flow0_timestep2.vtk").read_bytes()
     with mc.Context(4) as g:   # the driver's defaults: compat mode, FP64 pressure solve
         g.timestep()

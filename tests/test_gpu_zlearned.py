@@ -45,9 +45,18 @@ def test_forward_matches_oracle(m):
     nbr = neighbours(ctx)
     assert np.all((e != 0) == (nbr >= 0))
     y_gpu = ctx.mpnn_forward_pair(0)
+    # the inference kernels keep activations in BF16: against the oracle
+    # doing the same, the remaining differences are FP32 summation order and
+    # the BF16 roundings it flips (one BF16 step is 2^-8 of a value)
+    y_act, _ = mr.forward(w.astype(np.float64), f, nbr, e, bf16_ops=True, bf16_act=True)
+    scale = np.max(np.abs(y_act))
+    err = np.abs(y_gpu - y_act)
+    print("m=%d: inference vs oracle (BF16 activations): max %.2e, mean %.2e of max |y| %.3e"
+          % (m, err.max() / scale, err.mean() / scale, scale), flush=True)
+    assert err.max() <= 1e-2 * scale and err.mean() <= 1e-3 * scale
+    # and against the FP32-activation forward the training pass uses
     y_ref, _ = mr.forward(w.astype(np.float64), f, nbr, e, bf16_ops=True)
-    scale = np.max(np.abs(y_ref))
-    assert np.max(np.abs(y_gpu - y_ref)) <= 2e-2 * scale, (np.max(np.abs(y_gpu - y_ref)), scale)
+    assert np.max(np.abs(y_gpu - y_ref)) <= 2e-2 * np.max(np.abs(y_ref)), np.max(np.abs(y_gpu - y_ref))
     ctx.close()
 
 
