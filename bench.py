@@ -15,7 +15,9 @@ it); the other forms' rates are listed under `forms`.
 
 A profiled pass (before the timed one) brackets every kernel class with HIP
 events on the context's stream (no synchronisation) and divides its
-algorithmic bytes by its time.  The dominant single-kernel class is then timed
+algorithmic bytes by its time; the same pass times the iteration's stages
+(gradients, velocity, pressure, turbulence and scalar assembly and solves:
+`stage_ms_per_step`).  The dominant single-kernel class is then timed
 again with HIP events inside the timed region itself: `roofline` is that
 kernel's algorithmic bytes per launch over its average launch duration,
 against the 8 TB/s HBM3E peak; `traffic` is its HBM bytes per launch from the
@@ -46,9 +48,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 # Loop-1 pressure solves of the learned form start from the Galerkin projection
-# onto W = [net guess, last 2 loop-1 solutions] (deflation space); the
+# onto W = [net guess, last 4 loop-1 solutions] (deflation space); the
 # recycled form is the same without the net (what the net adds on its own).
-RECYCLE = 2
+RECYCLE = 4
 
 
 def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=None, mode="simple"):
@@ -71,14 +73,19 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
     amg_setup = {"builds": s0["amg_setups"], "ms": round(s0["amg_setup_ms"], 2)}
     # profiled pass: every kernel class bracketed by HIP events (no syncs)
     kern = {}
+    stages = None
     if profile_steps > 0:
         ctx.sync()
         ctx.profile(True)
         ctx.profile_reset()
+        ctx.stage_enable(True)
+        ctx.stage_reset()
         for _ in range(profile_steps):
             ctx.simple_iteration(it % 3)
             it += 1
         prof = ctx.profile_read()
+        stages = {k: round(v["ms"] / profile_steps, 4) for k, v in ctx.stage_read().items()}
+        ctx.stage_enable(False)
         ctx.profile(False)
         for name, v in prof.items():
             if v["calls"] == 0:
@@ -144,6 +151,7 @@ def run_form(mc, m, form, steps, warmup, profile_steps, roof_class=None, extra=N
         "memory": ctx.memory(),
         "amg_half": bool(ctx.opts.amg_half) and psolve == mc.PSOLVE_MIXED,
         "amg_setup": amg_setup,
+        "stage_ms_per_step": stages,
     }
     if use_net and "mpnn_fwd" not in kern:
         # the measured policy can leave the guess off through the profiled steps:
@@ -354,7 +362,8 @@ def main():
                       "deflation_starts_per_step": v["deflation_starts_per_step"],
                       "deflation_start_residual_ratio": v["deflation_start_residual_ratio"],
                       "deflation_vectors_per_start": v["deflation_vectors_per_start"],
-                      "last_rel_residual": v["last_rel_residual"], "amg_setup": v["amg_setup"]}
+                      "last_rel_residual": v["last_rel_residual"], "amg_setup": v["amg_setup"],
+                      "stage_ms_per_step": v["stage_ms_per_step"]}
                   for k, v in forms.items()},
         "forms_skipped": skipped or None,
         "field_rel_error_vs_fp64": field_errors(forms),

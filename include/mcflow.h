@@ -113,6 +113,12 @@ typedef struct mc_options {
                               (W^T A W) y = W^T b (with the learned guess in W when
                               `learned` uses it); 0: no recycling (the reference's
                               x0 = 0 unless the learned guess is used)           */
+    int    pcg_graph;      /* 1: each pressure PCG iteration (V-cycle, product and
+                              reductions) is captured once as a HIP graph and
+                              replayed (same kernels and order, so the same bits;
+                              the GPU runs its ~30 launches back to back); 0: one
+                              launch at a time.  Off while PCG kernel classes are
+                              being profiled.                                     */
 } mc_options;
 
 typedef struct mc_solve_status {

@@ -72,7 +72,7 @@ static constexpr size_t EV_PAIRS_MAX = 4096;   // pending pairs before a fold
 
 KTimer::KTimer(Ctx *c_, int kc_, double bytes_) : c(c_)
 {
-    if (!(c->profile & (1u << kc_))) return;
+    if (kc_ < 0 || !(c->profile & (1u << kc_))) return;   // kc_ < 0: not timed
     if (c->ev_used == EV_PAIRS_MAX && profile_fold(c) != MC_OK) return;
     const size_t i = c->ev_used;
     if (2 * i + 2 > c->ev_pool.size()) {
@@ -207,6 +207,7 @@ int mc_options_default(mc_options *o)
     o->amg_half = 1;
     o->amg_tail = 1;
     o->p_recycle = 0;
+    o->pcg_graph = 1;
     return MC_OK;
 }
 
@@ -232,6 +233,7 @@ static int validate(const mc_options *o, std::string &err)
     if (o->warm_start < -1 || o->warm_start > 1) { err = "warm_start must be -1, 0 or 1"; return MC_ERR_ARG; }
     if (o->amg_half != 0 && o->amg_half != 1) { err = "amg_half must be 0 or 1"; return MC_ERR_ARG; }
     if (o->p_recycle < 0 || o->p_recycle > DEFL_MAX) { err = "p_recycle must be in [0, 4]"; return MC_ERR_ARG; }
+    if (o->pcg_graph != 0 && o->pcg_graph != 1) { err = "pcg_graph must be 0 or 1"; return MC_ERR_ARG; }
     return MC_OK;
 }
 
@@ -300,6 +302,7 @@ int mc_ctx_destroy(mc_ctx *x)
     if (c->side) (void)hipStreamSynchronize(c->side);
     mpnn_destroy(c);
     profile_release(c);
+    graph_release(c);
     dfree_all(c);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -333,6 +336,7 @@ int mc_set_options(mc_ctx *x, const mc_options *opt)
     c->opt = *opt;
     set_constants(c);
     if (amg_change) c->amg_ready = false;
+    c->graph_gen++;   // smoother weights and level forms are baked into the PCG graphs
     return MC_OK;
 }
 

@@ -182,6 +182,8 @@ __global__ void __launch_bounds__(256) k_grad(Geo g, GradArgs a)
     int nbs[6];   // loaded ahead: each side's gathers then wait on one round trip
 #pragma unroll
     for (int s = 0; s < 6; s++) nbs[s] = g.nbr[6 * c + s];
+    // one side per trip: unrolling by 2 or 6 (more loads in flight, two waves
+    // per SIMD instead of three) measured 4% / 9% slower
     for (int s = 0; s < 6; s++) {
         const int nb = nbs[s];
         dv3 dX;

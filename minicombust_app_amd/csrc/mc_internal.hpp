@@ -88,6 +88,16 @@ struct Ctx {
     hipStream_t side = nullptr;
     hipEvent_t ev_coarse = nullptr, ev_dense = nullptr;
     bool dense_pending = false;
+    // one PCG iteration captured as a HIP graph, per precision (0 FP32, 1 FP64)
+    // and parity of the p buffers; valid while `gen` equals graph_gen (bumped
+    // by a hierarchy build and by mc_set_options) and the vectors are the same
+    struct PcgGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t gen = 0;
+        const void *key[8] = {};
+    };
+    PcgGraph pcg_graphs[2][2];
+    uint64_t graph_gen = 1;
     mc_options opt{};
     Consts k{};
     std::string err;
@@ -463,6 +473,7 @@ struct StageTimer : KTimer {
 int profile_fold(Ctx *c);
 int copy_probe(Ctx *c, int64_t bytes, int reps, double *ms_out);   // mc_linalg.hip
 void profile_release(Ctx *c);
+void graph_release(Ctx *c);   // mc_linalg.hip: the PCG iteration graphs
 
 // resolves Ctx::noalloc_scratch (once per device, with that device current)
 int noalloc_scratch_init(Ctx *c);   // mc_couple.hip

@@ -1055,6 +1055,7 @@ static int amg_build(Ctx *c, const double *vals)
         MC_TRY(dfree_category(c, MC_MEM_AMG));
     }
     const int coarse_max = std::max(8, std::min(c->opt.amg_coarse_max, DENSE_MAX));
+    c->graph_gen++;   // the PCG graphs hold the old levels' buffers
     HostCSR cur = std::move(A);
     bool first = true;
     for (;;) {
@@ -1785,7 +1786,7 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         const double jbytes = (l != 0 || !c->profile) ? 0.0
                               : half ? 28.0 * L.n : mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n;
         {
-            KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : MC_KC_COUNT, jbytes);
+            KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : -1, jbytes);
             if constexpr (sizeof(T) == 4) {
                 if (half)
                     hipLaunchKernelGGL(k_jac_h, dim3(g.blocks), dim3(BLOCK), 0, c->stream, H, b, (const float *)cur,
@@ -1961,6 +1962,80 @@ struct EpiPcgXr {
     }
 };
 
+// One PCG iteration: z = M^-1 r (V-cycle, whose last sweep also forms rho and
+// beta), p = z + beta p_old with q = A p (pa: p_old, pb: p), alpha, then x, r
+// and ||r||; every step returns at once when the solve is done.
+template <typename T>
+static int pcg_iteration(Ctx *c, const Mat<T> &A, const Grid &g, PcgVecs<T> w, T *pa, T *pb, KState *ks,
+                         double vbytes)
+{
+    const int n = c->N;
+    {
+        KTimer kt(c, MC_KC_VCYCLE, c->profile ? vcycle_bytes<T>(c) : 0.0);
+        MC_TRY(vcycle<T>(c, 0, w.r, w.z, ks, ks));
+    }
+    {
+        KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32, vbytes);
+        MC_DISPATCH_ND(nd_code(A), (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb, w.q,
+                       ks, c->red, g.chunk);
+    }
+    launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgAlpha{ks});
+    hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w.x, w.r, pb, w.q, ks, c->red,
+                       g.chunk);
+    launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgXr{ks});
+    return MC_OK;
+}
+
+void graph_release(Ctx *c)
+{
+    for (auto &pr : c->pcg_graphs)
+        for (auto &G : pr) {
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            G = Ctx::PcgGraph{};
+        }
+}
+
+// The iteration above as a HIP graph: captured on first use (and again after
+// the hierarchy, the options or the vectors changed), then replayed, so the
+// GPU runs the V-cycle's ~30 small launches back to back instead of at the
+// host's launch rate.  The capture holds kernel launches only (no memset or
+// event nodes, no other stream: the coarsest inverse is joined first).
+template <typename T>
+static int pcg_graph(Ctx *c, const Mat<T> &A, const Grid &g, PcgVecs<T> w, T *pa, T *pb, KState *ks, double vbytes,
+                     hipGraphExec_t *out)
+{
+    Ctx::PcgGraph &G = c->pcg_graphs[sizeof(T) == 8 ? 1 : 0][pa == w.p ? 0 : 1];
+    const void *key[8] = {w.x, w.r, w.z, w.p, w.q, w.p2, ks, pa};
+    bool hit = G.exec && G.gen == c->graph_gen;
+    for (int k = 0; hit && k < 8; k++) hit = G.key[k] == key[k];
+    if (!hit) {
+        if (G.exec) (void)hipGraphExecDestroy(G.exec);
+        G = Ctx::PcgGraph{};
+        MC_TRY(dense_join(c));
+        MC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = pcg_iteration<T>(c, A, g, w, pa, pb, ks, vbytes);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != MC_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc != MC_OK) return rc;
+            c->err = std::string("PCG graph capture: ") + hipGetErrorString(e);
+            return MC_ERR_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            G.exec = nullptr;
+            c->err = std::string("PCG graph instantiation: ") + hipGetErrorString(ei);
+            return MC_ERR_HIP;
+        }
+        G.gen = c->graph_gen;
+        for (int k = 0; k < 8; k++) G.key[k] = key[k];
+    }
+    *out = G.exec;
+    return MC_OK;
+}
+
 template <typename T>
 static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol, double atol, int maxit, KState *ks,
                     KState *hs, int *iters_out, bool x0 = false)
@@ -1994,21 +2069,21 @@ static int pcg_core(Ctx *c, const T *b, bool project, PcgVecs<T> w, double rtol,
     T *pa = w.p, *pb = w.p2;
     int it = 0, batch = 2;
     double rn_prev = -1.0;
+    // the kernel classes an iteration would time: profiling them launches
+    // kernel by kernel
+    const unsigned timed = (1u << MC_KC_VCYCLE) | (1u << MC_KC_SPMV64) | (1u << MC_KC_SPMV32) |
+                           (1u << MC_KC_JAC64) | (1u << MC_KC_JAC32);
+    const bool graphs = c->opt.pcg_graph && !(c->profile & timed) && c->stream != nullptr;   // no capture on the null stream
     for (;;) {
         for (int j = 0; j < batch && it < maxit; j++, it++) {
-            {
-                KTimer kt(c, MC_KC_VCYCLE, c->profile ? vcycle_bytes<T>(c) : 0.0);
-                MC_TRY(vcycle<T>(c, 0, w.r, w.z, ks, ks));
+            if (graphs) {
+                hipGraphExec_t ge = nullptr;
+                MC_TRY(pcg_graph<T>(c, A, g, w, pa, pb, ks, vbytes, &ge));
+                MC_TRY(dense_join(c));
+                MC_HIP(c, hipGraphLaunch(ge, c->stream));
+            } else {
+                MC_TRY(pcg_iteration<T>(c, A, g, w, pa, pb, ks, vbytes));
             }
-            {
-                KTimer kt(c, sizeof(T) == 8 ? MC_KC_SPMV64 : MC_KC_SPMV32, vbytes);
-                MC_DISPATCH_ND(nd_code(A), (k_pcg_pq<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, w.z, pa, pb,
-                               w.q, ks, c->red, g.chunk);
-            }
-            launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgAlpha{ks});
-            hipLaunchKernelGGL(k_pcg_xr<T>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, w.x, w.r, pb, w.q, ks,
-                               c->red, g.chunk);
-            launch_finish<1>(c->stream, c->red, g.blocks, EpiPcgXr{ks});
             std::swap(pa, pb);
         }
         MC_LAUNCH_CHECK(c);

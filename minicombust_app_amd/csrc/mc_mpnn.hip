@@ -555,47 +555,49 @@ __device__ __forceinline__ void mm_lds(const bf16x8 *A, const bf16x8 (&B)[KS], f
     }
 }
 
-// h0 = relu(f Wenc + benc) in fragment order; one thread per (cell, 8 channels)
-__global__ void __launch_bounds__(256) k_mp_encode(int n, const float *f, const float *W, bf16_t *h)
+// Wenc^T as the A fragments of one 16x16x32 product per output row tile j:
+// dst[(j * 64 + l) * 8 + t] = bf16(Wenc[t][16 j + l % 16]) for l < 16, t < 4
+// (the features are the first 4 of the 32 K rows), else 0
+__global__ void k_pack_enc(bf16_t *dst, const float *W)
 {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    const int i = idx >> 3, q = idx & 7, ks = q >> 2, g = q & 3;
-    if (i >= n) return;
-    const float4 fv = *(const float4 *)(f + (size_t)4 * i);
-    const float x[4] = {from_bf16(to_bf16(fv.x)), from_bf16(to_bf16(fv.y)), from_bf16(to_bf16(fv.z)),
-                        from_bf16(to_bf16(fv.w))};
-    bf16x8 o;
-#pragma unroll
-    for (int t = 0; t < 8; t++) {
-        const int c = frag_ch(ks, g, t);
-        float v = 0.f;
-#pragma unroll
-        for (int k = 0; k < MFIN; k++) v += x[k] * from_bf16(to_bf16(W[OFF_WENC + k * MH + c]));
-        v += W[OFF_BENC + c];
-        o[t] = bf(fmaxf(v, 0.f));
-    }
-    ((bf16x8 *)(h + (size_t)i * MH))[ks * 4 + g] = o;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4 * 64 * 8) return;
+    const int t = i & 7, l = (i >> 3) & 63, j = i >> 9;
+    dst[i] = (l < 16 && t < MFIN) ? to_bf16(W[OFF_WENC + t * MH + 16 * j + (l & 15)]) : (bf16_t)0;
 }
 
-// One message-passing layer for every cell (LAST: the decoder too, y only).
+// One message-passing layer for every cell.  FIRST: the layer's input is the
+// encoder h0 = relu(f Wenc + benc), formed here from the 4 features of the
+// cell and of each neighbour (one product per row tile, K padded to 32)
+// instead of being stored and gathered.  LAST: the decoder too, y only.
 // Persistent: workgroup b serves the bricks of XCD b % 8's contiguous range
-// (the grid is a multiple of 8);
-// its four waves take the brick's 16 tiles of 16 cells in turn.
-template <bool LAST>
-__global__ void __launch_bounds__(256, 3) k_mp_layer(CellWalk w, int nbricks, const bf16_t *hin, bf16_t *hout,
-                                                    const int *nbr, const float *e, const float *deg,
-                                                    const bf16_t *wf, const float *Wl, const float *Wall, float *y)
+// (the grid is a multiple of 8); its four waves take the brick's 16 tiles of
+// 16 cells in turn.  Three resident workgroups per compute unit (at most 170
+// registers, 50 KB of LDS each); holding Wb's fragments in registers instead
+// (two workgroups) measured 25% slower.
+constexpr int MP_LB = 3;
+template <bool FIRST, bool LAST>
+__global__ void __launch_bounds__(256, MP_LB) k_mp_layer(CellWalk w, int nbricks, const bf16_t *hin, const float *fin,
+                                                    bf16_t *hout, const int *nbr, const float *e, const float *deg,
+                                                    const bf16_t *wf, const bf16_t *wenc, const float *Wl,
+                                                    const float *Wall, float *y)
 {
     __shared__ bf16x8 sW[FR_SZ / 8];
-    __shared__ float sB[6][MH];   // we, b1, b2, bn1, bn2, bf16(wdec)
+    __shared__ float sB[7][MH];   // we, b1, b2, bn1, bn2, bf16(wdec), benc
     for (int i = threadIdx.x; i < FR_SZ / 8; i += 256) sW[i] = ((const bf16x8 *)wf)[i];
     for (int i = threadIdx.x; i < MH; i += 256) {
         sB[0][i] = Wl[L_WE + i]; sB[1][i] = Wl[L_B1 + i]; sB[2][i] = Wl[L_B2 + i];
         sB[3][i] = Wl[L_BN1 + i]; sB[4][i] = Wl[L_BN2 + i];
         sB[5][i] = from_bf16(to_bf16(Wall[OFF_WDEC + i]));
+        sB[6][i] = Wall[OFF_BENC + i];
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
+    bf16x8 wen[4];   // FIRST: Wenc^T fragments
+    if constexpr (FIRST) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) wen[j] = ((const bf16x8 *)wenc)[j * 64 + lane];
+    }
     __syncthreads();
     const int nxcd = 8, xcd = blockIdx.x % nxcd, local = blockIdx.x / nxcd, nloc = gridDim.x / nxcd;
     const int per = (nbricks + nxcd - 1) / nxcd;
@@ -611,9 +613,39 @@ __global__ void __launch_bounds__(256, 3) k_mp_layer(CellWalk w, int nbricks, co
             asm volatile("" : "+s"(z0));
             const bf16x8 *fW = sW + z0;
             const float(*vB)[MH] = (const float(*)[MH])(&sB[0][0] + z0);
-            bf16x8 hb[2];
+            // the layer input of cell cc as B fragments (zeros where cc < 0):
+            // raw() loads what the cell stores (FIRST: its features as the K
+            // rows 0..3 of lane group 0), input() turns it into h
+            auto raw = [&](int cc, bf16x8 (&o)[2]) {
+                if constexpr (FIRST) {
+                    o[0] = zero_frag();
+                    if (cc >= 0 && g == 0) {
+                        const float4 v = *(const float4 *)(fin + (size_t)4 * cc);
+                        o[0][0] = bf(v.x); o[0][1] = bf(v.y); o[0][2] = bf(v.z); o[0][3] = bf(v.w);
+                    }
+                } else {
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++) hb[ks] = valid ? ld_frag(hin, cell, ks, g) : zero_frag();
+                    for (int ks = 0; ks < 2; ks++) o[ks] = cc >= 0 ? ld_frag(hin, cc, ks, g) : zero_frag();
+                }
+            };
+            auto input = [&](int cc, bf16x8 (&o)[2]) {
+                if constexpr (FIRST) {
+                    const bf16x8 fb = o[0];
+                    float4v H[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        H[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wen[j], fb, float4v{0.f, 0.f, 0.f, 0.f}, 0,
+                                                                       0, 0);
+#pragma unroll
+                        for (int t = 0; t < 4; t++)
+                            H[j][t] = cc >= 0 ? fmaxf(H[j][t] + vB[6][16 * j + 4 * g + t], 0.f) : 0.f;
+                    }
+                    to_frags(H, o);
+                }
+            };
+            bf16x8 hb[2];
+            raw(cell, hb);
+            input(cell, hb);
             // P = h Wa
             float4v P[4];
             mm_lds<2>(fW + FR_WA / 8, hb, P, lane);
@@ -631,12 +663,11 @@ __global__ void __launch_bounds__(256, 3) k_mp_layer(CellWalk w, int nbricks, co
                 es[q] = valid ? e[6 * cell + q] : 0.f;
             }
 #pragma unroll
-            for (int q = 0; q < 6; q++)
-#pragma unroll
-                for (int ks = 0; ks < 2; ks++) hn[q][ks] = nb[q] >= 0 ? ld_frag(hin, nb[q], ks, g) : zero_frag();
+            for (int q = 0; q < 6; q++) raw(nb[q], hn[q]);
             {
 #pragma unroll
                 for (int q = 0; q < 6; q++) {
+                    input(nb[q], hn[q]);
                     float4v Q[4];
                     mm_lds<2>(fW + FR_WB / 8, hn[q], Q, lane);
                     if (nb[q] >= 0) {
@@ -702,7 +733,8 @@ __global__ void __launch_bounds__(256, 3) k_mp_layer(CellWalk w, int nbricks, co
                 to_frags(O, ho);
                 if (valid) {
 #pragma unroll
-                    for (int ks = 0; ks < 2; ks++) ((bf16x8 *)(hout + (size_t)cell * MH))[ks * 4 + g] = ho[ks];
+                    for (int ks = 0; ks < 2; ks++)   // streaming: read by the next launch only
+                        __builtin_nontemporal_store(ho[ks], (bf16x8 *)(hout + (size_t)cell * MH) + ks * 4 + g);
                 }
             }
         }
@@ -807,14 +839,16 @@ __global__ void k_features(int N, const double *b, const double *vals, const int
     if (i >= N) return;
     const double mu = sc[0], sb = sc[1], sd = sc[2], mv = sc[3], sq = sc[7];
     const double q = cell_outflow(i, mflux, forder, fother);
-    f[4 * i + 0] = (float)((b[i] - mu) / sb);
-    f[4 * i + 1] = (float)(vals[diag_idx[i]] / sd);
-    f[4 * i + 2] = (float)(vol[i] / mv);
-    f[4 * i + 3] = (float)(q / sq);
+    *(float4 *)(f + 4 * (size_t)i) = make_float4((float)((b[i] - mu) / sb), (float)(vals[diag_idx[i]] / sd),
+                                                 (float)(vol[i] / mv), (float)(q / sq));
+    float ev[6];
+#pragma unroll
     for (int s = 0; s < 6; s++) {
         const int p = pmap[6 * i + s];
-        e[6 * i + s] = (p >= 0) ? (float)(vals[p] / sd) : 0.f;
+        ev[s] = (p >= 0) ? (float)(vals[p] / sd) : 0.f;
     }
+    float2 *e2 = (float2 *)(e + 6 * (size_t)i);
+    e2[0] = make_float2(ev[0], ev[1]); e2[1] = make_float2(ev[2], ev[3]); e2[2] = make_float2(ev[4], ev[5]);
 }
 // The solve rescales the guess by its Galerkin factor, so the net learns the
 // solution's shape: the target is the mean-zero solution over its rms.
@@ -881,6 +915,7 @@ static void pack_weights(Ctx *c)
         Fr(P + FR_N1, Wl + L_WN1, 4);
         Fr(P + FR_N2, Wl + L_WN2, 2);
     }
+    hipLaunchKernelGGL(k_pack_enc, dim3(div_up(4 * 64 * 8, 256)), dim3(256), 0, c->stream, m->wf + ML * FR_SZ, m->w);
 }
 
 // --------------------------------------------------------------- host --
@@ -900,7 +935,7 @@ static int ensure(Ctx *c)
     MC_TRY(dalloc(c, &m->pmap, 6 * (int64_t)N)); MC_TRY(dalloc(c, &m->deg, N));
     MC_TRY(dalloc(c, &m->f, 4 * (int64_t)N)); MC_TRY(dalloc(c, &m->e, 6 * (int64_t)N));
     MC_TRY(dalloc(c, &m->y, N)); MC_TRY(dalloc(c, &m->dy, N));
-    MC_TRY(dalloc(c, &m->wf, ML * FR_SZ));
+    MC_TRY(dalloc(c, &m->wf, ML * FR_SZ + 4 * 64 * 8));   // the layers, then Wenc^T
     MC_TRY(dalloc(c, &m->hb[0], (int64_t)N * MH)); MC_TRY(dalloc(c, &m->hb[1], (int64_t)N * MH));
     MC_TRY(dalloc(c, &m->sc, 16));
     {
@@ -969,31 +1004,28 @@ static void forward_train(Ctx *c, const float *f, const float *e)
     hipLaunchKernelGGL(k_decode, dim3(div_up(n, 64)), blk, 0, c->stream, n, m->H[ML], W, m->y);
 }
 
-// Inference forward pass into m->y: the encoder, then one fused kernel per
-// layer on BF16 activations (the last with the decoder).
+// Inference forward pass into m->y: one fused kernel per layer on BF16
+// activations (the first with the encoder, the last with the decoder).
 static void forward_infer(Ctx *c, const float *f, const float *e)
 {
     Mpnn *m = M(c);
     const int n = m->n;
-    hipLaunchKernelGGL(k_mp_encode, dim3(div_up((int64_t)n * 8, 256)), dim3(256), 0, c->stream, n, f, m->w, m->hb[0]);
     const CellWalk w = cell_walk(c);
     const int nbricks = walk_blocks(w);
-    // three resident workgroups per compute unit (134 registers, 50 KB of
-    // LDS each), a multiple of the 8 XCDs
-    const int per_xcd = std::max(1, std::min(div_up(nbricks, 8), 3 * std::max(m->cus, 8) / 8));
+    // MP_LB resident workgroups per compute unit, a multiple of the 8 XCDs
+    const int per_xcd = std::max(1, std::min(div_up(nbricks, 8), MP_LB * std::max(m->cus, 8) / 8));
     const dim3 grid(8 * per_xcd), blk(256);
-    for (int l = 0; l < ML; l++) {
-        const bf16_t *hin = m->hb[l & 1];
-        bf16_t *hout = m->hb[(l + 1) & 1];
-        const bf16_t *wf = m->wf + l * FR_SZ;
-        const float *Wl = m->w + OFF_L0 + l * LSZ;
-        if (l < ML - 1)
-            hipLaunchKernelGGL(k_mp_layer<false>, grid, blk, 0, c->stream, w, nbricks, hin, hout, m->nbr, e, m->deg,
-                               wf, Wl, (const float *)m->w, (float *)nullptr);
-        else
-            hipLaunchKernelGGL(k_mp_layer<true>, grid, blk, 0, c->stream, w, nbricks, hin, (bf16_t *)nullptr, m->nbr, e,
-                               m->deg, wf, Wl, (const float *)m->w, m->y);
-    }
+    static_assert(ML == 3, "one first, one middle and one last layer");
+    const bf16_t *wenc = m->wf + ML * FR_SZ;
+    const float *W = m->w;
+    hipLaunchKernelGGL((k_mp_layer<true, false>), grid, blk, 0, c->stream, w, nbricks, (const bf16_t *)nullptr, f,
+                       m->hb[0], m->nbr, e, m->deg, (const bf16_t *)m->wf, wenc, W + OFF_L0, W, (float *)nullptr);
+    hipLaunchKernelGGL((k_mp_layer<false, false>), grid, blk, 0, c->stream, w, nbricks, (const bf16_t *)m->hb[0],
+                       (const float *)nullptr, m->hb[1], m->nbr, e, m->deg, (const bf16_t *)(m->wf + FR_SZ), wenc,
+                       W + OFF_L0 + LSZ, W, (float *)nullptr);
+    hipLaunchKernelGGL((k_mp_layer<false, true>), grid, blk, 0, c->stream, w, nbricks, (const bf16_t *)m->hb[1],
+                       (const float *)nullptr, (bf16_t *)nullptr, m->nbr, e, m->deg,
+                       (const bf16_t *)(m->wf + 2 * FR_SZ), wenc, W + OFF_L0 + 2 * LSZ, W, m->y);
 }
 
 // MFMA work of one training forward pass (flops)
@@ -1002,13 +1034,13 @@ static double forward_flops(int n)
     return (double)n * (2.0 * 16 * MH + ML * 2.0 * (MH * 2 * MH + MH * MH + 2 * MH * MH + MH * MH));
 }
 
-// MFMA work of one fused inference pass (flops): per cell and layer the
-// products with Wa, W2, Wn2 (64 x 64), Wn1 (128 x 64) and Wb once for each of
-// the six neighbour slots (recomputed, not stored); the 4 -> 64 encoder runs
-// on the vector units and is not counted
+// MFMA work of one fused inference pass (flops, as executed): per cell and
+// layer the products with Wa, W2, Wn2 (64 x 64), Wn1 (128 x 64) and Wb once
+// for each of the six neighbour slots (recomputed, not stored), and in the
+// first layer the encoder for the cell and its six neighbours (K padded to 32)
 static double infer_flops(int n)
 {
-    return (double)n * ML * 2.0 * MH * MH * (1 + 6 + 1 + 2 + 1);
+    return (double)n * (ML * 2.0 * MH * MH * (1 + 6 + 1 + 2 + 1) + 7 * 2.0 * MH * 32);
 }
 
 template <int K1, int K2, int KP, int MM>

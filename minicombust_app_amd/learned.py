@@ -21,6 +21,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights", "mpnn_pressure_guess.f32")
+# the training run that produced WEIGHTS: command, data, loss curve, held-out loss
+RECORD = os.path.join(HERE, "weights", "mpnn_pressure_guess.train.json")
 
 H, FIN, LAYERS = 64, 4, 3
 
@@ -102,6 +104,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=20240910)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--record", default=None, help="training record (JSON); default: next to --out")
     args = ap.parse_args()
     from minicombust_app_amd import mcflow as mc
 
@@ -113,14 +116,25 @@ def main():
     t0 = time.time()
     cases = generate(ctx, args.pairs, seed=args.seed, log=log)
     t1 = time.time()
+    n = ctx.mpnn_dataset_size()
+    n_train = max(1, int(round(n * 0.9)))
+    held0 = float(np.mean([ctx.mpnn_grad(p)[1] for p in range(n_train, n)])) if n > n_train else None
     losses, held = train(ctx, args.steps, lr=args.lr, log=log)
     t2 = time.time()
     w = ctx.mpnn_save()
     save_weights(w, args.out)
-    print(json.dumps({"m": args.m, "pairs": args.pairs, "cases": cases, "steps": args.steps,
-                      "generate_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
-                      "first_loss": losses[0][1], "last_loss": losses[-1][1], "heldout_loss": held,
-                      "weights": os.path.relpath(args.out, os.path.dirname(HERE))}))
+    rec = {"command": "python -m minicombust_app_amd.learned " + " ".join(sys.argv[1:]),
+           "m": args.m, "cells": 2 * args.m ** 3, "pairs": n, "train_pairs": n_train, "heldout_pairs": n - n_train,
+           "cases": cases, "inlet_scale": "1 + U(-0.2, 0.2), seed %d" % args.seed, "steps": args.steps,
+           "lr": args.lr, "schedule": "Adam, cosine decay, one pair per step",
+           "generate_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2),
+           "heldout_loss_initial": held0, "heldout_loss": held,
+           "loss_curve": [[s, round(l, 6)] for s, l in losses],
+           "weights": os.path.relpath(args.out, os.path.dirname(HERE)), "nparams": int(w.size)}
+    record = args.record or os.path.splitext(args.out)[0] + ".train.json"
+    with open(record, "w") as fh:
+        json.dump(rec, fh, indent=1)
+    print(json.dumps({k: v for k, v in rec.items() if k != "loss_curve"}))
     ctx.close()
 
 

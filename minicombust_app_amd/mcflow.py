@@ -56,7 +56,7 @@ class Options(C.Structure):
         ("mixed_inner_rtol", C.c_double), ("check_every", C.c_int), ("inlet_scale", C.c_double),
         ("amg_cheb_ratio", C.c_double), ("cell_tile", C.c_int), ("limit_grad", C.c_int), ("amg_refresh", C.c_int),
         ("cell_ychunk", C.c_int), ("warm_start", C.c_int), ("amg_half", C.c_int),
-        ("amg_tail", C.c_int), ("p_recycle", C.c_int),
+        ("amg_tail", C.c_int), ("p_recycle", C.c_int), ("pcg_graph", C.c_int),
     ]
 
 
