@@ -116,9 +116,11 @@ typedef struct mc_options {
     int    pcg_graph;      /* 1: each pressure PCG iteration (V-cycle, product and
                               reductions) is captured once as a HIP graph and
                               replayed (same kernels and order, so the same bits;
-                              the GPU runs its ~30 launches back to back); 0: one
-                              launch at a time.  Off while PCG kernel classes are
-                              being profiled.                                     */
+                              off while PCG kernel classes are being profiled);
+                              0 (default): one launch at a time.  On the MI355X
+                              the replay measured no faster: the ~8 us between
+                              dependent small kernels stays, and each replay adds
+                              its own launch gap.                                 */
 } mc_options;
 
 typedef struct mc_solve_status {
@@ -152,6 +154,9 @@ typedef struct mc_stats {
     int64_t p_defl_vectors;   /* sum over those of the vectors W held            */
     int64_t amg_setups;       /* AMG hierarchy builds (aggregation, Galerkin pattern) */
     double  amg_setup_ms;     /* host wall time of those builds, first values included */
+    int64_t p_loops_skipped;  /* correction loops after the first whose PP came out
+                                 zero everywhere (a zero right-hand side): their
+                                 passes, which would add only signed zeros, skipped */
 } mc_stats;
 
 /* Host view of a mesh, the arrays of the reference Mesh<T>

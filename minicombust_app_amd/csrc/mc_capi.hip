@@ -207,7 +207,7 @@ int mc_options_default(mc_options *o)
     o->amg_half = 1;
     o->amg_tail = 1;
     o->p_recycle = 0;
-    o->pcg_graph = 1;
+    o->pcg_graph = 0;
     return MC_OK;
 }
 

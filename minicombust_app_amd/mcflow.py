@@ -74,6 +74,7 @@ class Stats(C.Structure):
         ("clip_passes", C.c_int64), ("host_polls", C.c_int64), ("p_guesses", C.c_int64),
         ("p_guess_ratio_sum", C.c_double), ("p_defl_starts", C.c_int64), ("p_defl_ratio_sum", C.c_double),
         ("p_defl_vectors", C.c_int64), ("amg_setups", C.c_int64), ("amg_setup_ms", C.c_double),
+        ("p_loops_skipped", C.c_int64),
     ]
 
 

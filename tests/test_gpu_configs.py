@@ -71,8 +71,9 @@ def test_bench_configuration_m128_against_oracle():
             errs = _rel_errors(g, ref_phi, ref_mflux)
             print("psolve %d: max |gpu - oracle| / max |oracle|: %s" % (
                 psolve, ", ".join("%s %.1e" % kv for kv in errs.items())), flush=True)
-            print("BiCGStab iterations (momentum, scalars): gpu (%d, %d), oracle (%d, %d)"
-                  % (st["mom_iters"], st["scalar_iters"], ref_its[0], ref_its[1]), flush=True)
+            print("BiCGStab iterations (momentum, scalars): gpu (%d, %d), oracle (%d, %d); correction loops "
+                  "skipped with PP zero: %d" % (st["mom_iters"], st["scalar_iters"], ref_its[0], ref_its[1],
+                                                st["p_loops_skipped"]), flush=True)
             scalar_tol = 1e-7 if st["scalar_iters"] == ref_its[1] else 1e-5
             for k, e in errs.items():
                 tol = 1e-7 if k in ("U", "V", "W", "P", "mflux") else scalar_tol
@@ -134,8 +135,8 @@ def test_scale_up_m200():
             st = g.stats()
             mem = g.memory()
             total = sum(mem.values())
-            print("%s: p_relres_max %.2e, PCG iterations %d, device memory %.1f GB %s" % (
-                name, st["p_relres_max"], st["p_iters"], total / 1e9,
+            print("%s: p_relres_max %.2e, PCG iterations %d, AMG setup %.0f ms, device memory %.1f GB %s" % (
+                name, st["p_relres_max"], st["p_iters"], st["amg_setup_ms"], total / 1e9,
                 {k: round(v / 1e9, 2) for k, v in mem.items()}), flush=True)
             assert st["p_relres_max"] <= 1e-8 * 1.0001
             assert total <= budget, (name, total)
