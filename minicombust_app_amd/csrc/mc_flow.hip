@@ -1530,176 +1530,6 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
     if constexpr (TEM) store_row(g, c, nint, own_nb, own_fft, At, a.tvals, a.tdia);
 }
 
-// The same assembly with the fields taken one after another: a pass over the
-// six faces forms the shared matrix rows (and TEM's), then one pass per field
-// forms that field's sources, re-reading the face data and neighbour centres
-// from the caches instead of holding five fields' operands in registers.
-// Every field's additions are the single-field kernel's, in the same order.
-__device__ __forceinline__ double batch_field_source(const Geo &g, const CellFaces &cf, int c, const dv3 &cc,
-                                                     const double *mflux, double *phi, const double *gx,
-                                                     const double *gy, const double *gz, int type, double ev,
-                                                     double inlet_ev, double inlet_phi)
-{
-    const int N = g.N;
-    const double GammaBlend = 0.0;
-    const dv3 gc = ld3(gx, gy, gz, c);
-    const double pc = phi[c];
-    double S = 0.0;
-#ifndef MC_BFP_UNROLL
-#define MC_BFP_UNROLL 1
-#endif
-#pragma unroll MC_BFP_UNROLL
-    for (int k = 0; k < 6; k++) {
-        const int f = cf.f[k];
-        const int o = cf.o[k];
-        const bool own0 = o >= 0;
-        const int nb = o & 0x7fffffff;
-        const double mf = mflux[f];
-        if (nb < N) {
-            const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
-            double Visac = ev * lambda0 + ev * lambda1;
-            Visac -= ev;
-            Visac = scalar_visac(type, ev, Visac);
-            const dv3 cn = ld3(g.cx, g.cy, g.cz, nb);
-            const dv3 Xpn = own0 ? (cn - cc) : (cc - cn);
-            const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
-            const double VisFace = Visac * g.frlc[f];
-            const dv3 gn = ld3(gx, gy, gz, nb);
-            const dv3 dPhiXac = (own0 ? gc : gn) * lambda0 + (own0 ? gn : gc) * lambda1;
-            const double pn = phi[nb];
-            const double p0 = own0 ? pc : pn, p1 = own0 ? pn : pc;
-            const double PhiFace = (mf >= 0.0) ? p0 : p1;
-            const double fce = mf * PhiFace;
-            const double fde1 = Visac * dot(dPhiXac, nrm);
-            const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
-            const double fdi = VisFace * dot(dPhiXac, Xpn);
-            const double blend = GammaBlend * (fce - fci);
-            if (own0) S = S - blend + fde1 - fdi;
-            else S = S + blend - fde1 + fdi;
-        } else {
-            const int side = nb - N;
-            const int bt = g.btype[side];
-            const bool last = (f == g.last_face[side]);
-            const dv3 Xpn = ld3(g.fcx, g.fcy, g.fcz, f) - cc;
-            const dv3 nrm = ld3(g.fnx, g.fny, g.fnz, f);
-            if (bt == INLET) {
-                double Visac = inlet_ev;
-                Visac -= ev;
-                Visac = scalar_visac(type, ev, Visac);
-                const double VisFace = Visac * g.frlc[f];
-                const double fde = Visac * dot(gc, nrm);
-                const double fdi = VisFace * dot(gc, Xpn);
-                const double fb = -VisFace + dmin(mf, 0.0);
-                S = S - fb * inlet_phi + fde - fdi;
-                if (last) phi[N + side] = inlet_phi;
-            } else if (bt == OUTLET) {
-                double Visac = ev;
-                Visac -= ev;
-                Visac = scalar_visac(type, ev, Visac);
-                const double PhiFace = pc + dot(gc, Xpn);
-                const double VisFace = Visac * g.frlc[f];
-                const double fde = Visac * dot(gc, nrm);
-                const double fdi = VisFace * dot(gc, Xpn);
-                S = S + fde - fdi;
-                if (last) phi[N + side] = PhiFace;
-            } else if (bt == WALL) {
-                if (last) phi[N + side] = pc;
-            }
-        }
-    }
-    return S;
-}
-
-template <bool TEM>
-__global__ void __launch_bounds__(256) k_scalar_batch_fp(Geo g, ScalBatchArgs a)
-{
-    const int c = walk_cell(g.w);
-    if (c < 0) return;
-    const int N = g.N;
-    const double ev = a.ev;
-    const dv3 cc = ld3(g.cx, g.cy, g.cz, c);
-    const CellFaces cf = cell_faces_of(g, c);
-    // matrix rows: the passive scalars' (shared) and the temperature's
-    double A = 0.0, At = 0.0, own_ff[6], own_fft[6];
-    int own_nb[6];
-    int nint = 0;
-    for (int k = 0; k < 6; k++) {
-        const int f = cf.f[k];
-        const int o = cf.o[k];
-        const bool own0 = o >= 0;
-        const int nb = o & 0x7fffffff;
-        const double mf = a.mflux[f];
-        if (nb < N) {
-            const double lambda0 = g.flam[f], lambda1 = 1.0 - lambda0;
-            double Visac = ev * lambda0 + ev * lambda1;
-            Visac -= ev;
-            Visac = scalar_visac(a.type, ev, Visac);
-            const double VisFace = Visac * g.frlc[f];
-            const double ff0 = -VisFace - dmax(mf, 0.0);
-            const double ff1 = -VisFace + dmin(mf, 0.0);
-            if constexpr (TEM) {
-                double Vt = ev * lambda0 + ev * lambda1;
-                Vt -= ev;
-                Vt = scalar_visac(TEMP, ev, Vt);
-                const double VisFaceT = Vt * g.frlc[f];
-                own_fft[nint] = own0 ? (-VisFaceT + dmin(mf, 0.0)) : (-VisFaceT - dmax(mf, 0.0));
-            }
-            if (own0) {
-                own_ff[nint] = ff1;
-                st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
-            } else {
-                own_ff[nint] = ff0;
-            }
-            own_nb[nint] = nb;
-            nint++;
-        } else if (g.btype[nb - N] == INLET) {
-            if constexpr (TEM) {
-                double Vt = a.inlet_ev;
-                Vt -= ev;
-                Vt = scalar_visac(TEMP, ev, Vt);
-                const double VisFaceT = Vt * g.frlc[f];
-                const double fb = -VisFaceT + dmin(mf, 0.0);
-                At = At - fb;
-            }
-            double Visac = a.inlet_ev;
-            Visac -= ev;
-            Visac = scalar_visac(a.type, ev, Visac);
-            const double VisFace = Visac * g.frlc[f];
-            const double fb = -VisFace + dmin(mf, 0.0);
-            A = A - fb;
-        }
-    }
-    const double fv = g.rho[c] * g.vol[c] * a.rdelta;
-    if constexpr (TEM) {
-        At += fv;
-        for (int k = 0; k < nint; k++) At -= own_fft[k];
-        At *= (1. / a.urf);
-    }
-    A += fv;
-    for (int k = 0; k < nint; k++) A -= own_ff[k];
-    A *= (1. / a.urf);
-    st_out(&a.AV[c], A);
-    store_row(g, c, nint, own_nb, own_ff, A, a.vals, a.dia);
-    if constexpr (TEM) {
-        store_row(g, c, nint, own_nb, own_fft, At, a.tvals, a.tdia);
-        double St = batch_field_source(g, cf, c, cc, a.mflux, a.tphi, a.tgx, a.tgy, a.tgz, TEMP, ev, a.inlet_ev,
-                                       a.tinlet);
-        const double pt = a.tphi[c];
-        St += fv * pt;
-        St = St + (1.0 - a.urf) * At * pt;
-        st_out(&a.tS[c], St);
-    }
-#pragma unroll 1
-    for (int s = 0; s < SB; s++) {
-        double S = batch_field_source(g, cf, c, cc, a.mflux, a.phi[s], a.gx[s], a.gy[s], a.gz[s], a.type, ev,
-                                      a.inlet_ev, a.inlet_phi[s]);
-        const double p = a.phi[s][c];
-        S += fv * p;
-        S = S + (1.0 - a.urf) * A * p;
-        st_out(&a.S[s][c], S);
-    }
-}
-
 // FUL, PRO, VARF, VARP (FlowSolver.inl:3243-3260) as one 4-RHS solve; with
 // tem, the temperature equation (:3237-3242) is assembled in the same pass
 // into its own matrix and solved first, as the reference orders them (none of
@@ -1738,17 +1568,10 @@ static int batch_solve(Ctx *c, bool tem)
                              8.0 * z.nnz * (tem ? 2 : 1);
         StageTimer t(c, MC_ST_SCA_FLUX);
         KTimer kt(c, MC_KC_SCALAR_BATCH, bytes);
-#ifdef MC_BATCH_FP
-        if (tem)
-            hipLaunchKernelGGL(k_scalar_batch_fp<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
-        else
-            hipLaunchKernelGGL(k_scalar_batch_fp<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
-#else
         if (tem)
             hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
         else
             hipLaunchKernelGGL(k_scalar_batch<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
-#endif
         MC_LAUNCH_CHECK(c);
     }
     StageTimer ts(c, MC_ST_SCA_SOLVE);
