@@ -177,6 +177,11 @@ struct Ctx {
     double *dscal = nullptr;       // device scalars for the solvers
     double *hscal = nullptr;       // pinned host mirror
     int clip_batch[2] = {8, 8};    // TE / ED repair: passes to launch before the first check
+    // BiCGStab: iterations the last solve for the same unknowns took (its first
+    // batch before a host poll), keyed by the solution vector
+    static constexpr int BI_HIST = 8;
+    const void *bi_hist_key[BI_HIST] = {};
+    int bi_hist_its[BI_HIST] = {};
     // particle/flow coupling (allocated on first use)
     double *pterms = nullptr;      // [5N] particle_aos per cell: momentum xyz, energy, fuel
     int *p2c_ptr = nullptr, *p2c_cell = nullptr;   // point -> cells, ascending

@@ -680,7 +680,13 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
     MC_LAUNCH_CHECK(c);
     // A; r, p', v', r^ read and p, v written per RHS
     const double mbytes = mat_bytes(A, c->nnz, 8) + 48.0 * NR * n;
-    int it = 0, batch = 1;
+    // first batch: as many iterations as the last solve for these unknowns
+    // took (surplus iterations return at once on the device), so a solve that
+    // repeats its count needs one host poll
+    int hslot = -1;
+    for (int k = 0; k < Ctx::BI_HIST && hslot < 0; k++)
+        if (c->bi_hist_key[k] == x[0]) hslot = k;
+    int it = 0, batch = hslot >= 0 ? std::max(1, std::min(c->bi_hist_its[hslot], 8)) : 1;
     double rn_prev = -1.0;
     for (;;) {
         for (int j = 0; j < batch && it < maxit; j++, it++) {
@@ -706,6 +712,23 @@ static int bicgstab_nr(Ctx *c, const double *vals, double *const *b, double *con
             if (!hs->done[k]) rn = std::max(rn, hs->rnorm[k] / std::max(hs->tol[k], 1e-300));
         batch = predict_batch(rn_prev, rn, batch, 1.0, 8);
         rn_prev = rn;
+    }
+    {
+        int its = 0;
+        for (int k = 0; k < NR; k++) its = std::max(its, hs->iters[k]);
+        if (hslot < 0) {   // a free slot, else the oldest entry's
+            hslot = 0;
+            while (hslot < Ctx::BI_HIST - 1 && c->bi_hist_key[hslot]) hslot++;
+            if (c->bi_hist_key[hslot]) {
+                for (int k = 0; k + 1 < Ctx::BI_HIST; k++) {
+                    c->bi_hist_key[k] = c->bi_hist_key[k + 1];
+                    c->bi_hist_its[k] = c->bi_hist_its[k + 1];
+                }
+                hslot = Ctx::BI_HIST - 1;
+            }
+            c->bi_hist_key[hslot] = x[0];
+        }
+        c->bi_hist_its[hslot] = its;
     }
     if (st) {
         st->iterations = 0; st->converged = 1; st->relres = 0.0; st->outer = 0; st->guess_used = 0; st->bnorm = 0.0;
