@@ -105,22 +105,11 @@ __device__ __forceinline__ void store_row(const Geo &g, int c, int nint, const i
 struct CellFaces {
     int f[6], o[6];
 };
-// a value only its own cell reads (streaming load: evicted first from L2, so
-// it does not displace the neighbour data other workgroups still gather)
-template <class T>
-__device__ __forceinline__ T ld_own(const T *p)
-{
-#ifdef MC_NT_LOADS
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
 __device__ __forceinline__ CellFaces cell_faces_of(const Geo &g, int c)
 {
     CellFaces r;
 #pragma unroll
-    for (int k = 0; k < 6; k++) { r.f[k] = ld_own(g.forder + 6 * c + k); r.o[k] = ld_own(g.fother + 6 * c + k); }
+    for (int k = 0; k < 6; k++) { r.f[k] = g.forder[6 * c + k]; r.o[k] = g.fother[6 * c + k]; }
     return r;
 }
 
@@ -1180,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     // face ids and fother codes loaded ahead of the face loop (a plain loop
     // here: the unrolled load of cell_faces_of makes this kernel 2-9% slower)
     int fo[6], ot[6];
-    for (int k = 0; k < 6; k++) { fo[k] = ld_own(g.forder + 6 * c + k); ot[k] = ld_own(g.fother + 6 * c + k); }
+    for (int k = 0; k < 6; k++) { fo[k] = g.forder[6 * c + k]; ot[k] = g.fother[6 * c + k]; }
     // the cell's own values, read once: each face takes the other cell's
     // from memory and these for its own side
     const dv3 gc = ld3(a.gx, a.gy, a.gz, c), cc = ld3(g.cx, g.cy, g.cz, c);
@@ -1513,7 +1502,7 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
             }
         }
     }
-    const double fv = ld_own(g.rho + c) * ld_own(g.vol + c) * a.rdelta;
+    const double fv = g.rho[c] * g.vol[c] * a.rdelta;
     if constexpr (TEM) {
         St += fv * pt;
         At += fv;
