@@ -372,7 +372,6 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
     const int c = walk_cell(g.w);
     if (c < 0) return;
     const int N = g.N;
-    const double GammaBlend = 0.0;
     double SU = 0.0, SV = 0.0, SW = 0.0, A = 0.0;
     double xU = 0.0, xV = 0.0, xW = 0.0;   // off-diagonal part of A x0 (warm-started solve)
     double own_ff[6];
@@ -407,36 +406,30 @@ __global__ void __launch_bounds__(256) k_momentum(Geo g, MomArgs a)
             const double U0 = own0 ? Uc : Un, U1 = own0 ? Un : Uc;
             const double V0 = own0 ? Vc : Vn, V1 = own0 ? Vn : Vc;
             const double W0 = own0 ? Wc : Wn, W1 = own0 ? Wn : Wc;
-            double UFace, VFace, WFace;
-            if (mf >= 0.0) { UFace = U0; VFace = V0; WFace = W0; }
-            else { UFace = U1; VFace = V1; WFace = W1; }
-            const double fuce = mf * UFace, fvce = mf * VFace, fwce = mf * WFace;
             const double sx = g.fnx[f], sy = g.fny[f], sz = g.fnz[f];
             const double fude = Visac * ((dUdXac.x + dUdXac.x) * sx + (dUdXac.y + dVdXac.x) * sy + (dUdXac.z + dWdXac.x) * sz);
             const double fvde = Visac * ((dUdXac.y + dVdXac.x) * sx + (dVdXac.y + dVdXac.y) * sy + (dVdXac.z + dWdXac.y) * sz);
             const double fwde = Visac * ((dUdXac.z + dWdXac.x) * sx + (dWdXac.y + dVdXac.z) * sy + (dWdXac.z + dWdXac.z) * sz);
-            const double fmin = dmin(mf, 0.0), fmax = dmax(mf, 0.0);
-            const double fuci = fmin * U0 + fmax * U1;
-            const double fvci = fmin * V0 + fmax * V1;
-            const double fwci = fmin * W0 + fmax * W1;
             const double fudi = VisFace * dot(dUdXac, Xpn);
             const double fvdi = VisFace * dot(dVdXac, Xpn);
             const double fwdi = VisFace * dot(dWdXac, Xpn);
             const double ff0 = -VisFace - dmax(mf, 0.0);
             const double ff1 = -VisFace + dmin(mf, 0.0);
-            const double blend_u = GammaBlend * (fuce - fuci);
-            const double blend_v = GammaBlend * (fvce - fvci);
-            const double blend_w = GammaBlend * (fwce - fwci);
+            // The convective deferred correction GammaBlend * (fce - fci)
+            // (FlowSolver.inl:1471-1473) is a signed zero for finite operands
+            // (GammaBlend = 0, :1387); adding or subtracting a signed zero
+            // leaves a sum that starts at +0 unchanged (it never becomes -0),
+            // so the term is left out.
             if (own0) {
-                SU = SU - blend_u + fude - fudi;
-                SV = SV - blend_v + fvde - fvdi;
-                SW = SW - blend_w + fwde - fwdi;
+                SU = SU + fude - fudi;
+                SV = SV + fvde - fvdi;
+                SW = SW + fwde - fwdi;
                 own_ff[nint] = ff1;
                 st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
             } else {
-                SU = SU + blend_u - fude + fudi;
-                SV = SV + blend_v - fvde + fvdi;
-                SW = SW + blend_w - fwde + fwdi;
+                SU = SU - fude + fudi;
+                SV = SV - fvde + fvdi;
+                SW = SW - fwde + fwdi;
                 own_ff[nint] = ff0;
             }
             xU += own_ff[nint] * Un; xV += own_ff[nint] * Vn; xW += own_ff[nint] * Wn;
@@ -1159,7 +1152,6 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
     if (c < 0) return;
     const int N = g.N;
     const double ev = a.ev;
-    const double GammaBlend = 0.0;
     const bool wallcell = g.is_wall[c] != 0;
     const bool ed = a.type == TERBED;
     double S = 0.0, A = 0.0;
@@ -1190,26 +1182,25 @@ __global__ void __launch_bounds__(256) k_scalar(Geo g, ScalArgs a)
             const dv3 Xpn = own0 ? (cn - cc) : (cc - cn);
             const double VisFace = Visac * g.frlc[f];
             const double pn = a.phi[nb];
-            const double p0 = own0 ? pc : pn, p1 = own0 ? pn : pc;
-            const double PhiFace = (mf >= 0.0) ? p0 : p1;
-            const double fce = mf * PhiFace;
             const double fde1 = Visac * dot(dPhiXac, ld3(g.fnx, g.fny, g.fnz, f));
-            const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
             const double fdi = VisFace * dot(dPhiXac, Xpn);
             double ff0 = -VisFace - dmax(mf, 0.0);
             double ff1 = -VisFace + dmin(mf, 0.0);
-            const double blend = GammaBlend * (fce - fci);
+            // the convective deferred correction GammaBlend * (fce - fci)
+            // (FlowSolver.inl:2600-2604) is a signed zero for finite operands
+            // (GammaBlend = 0, :2523) and leaves S, which starts at +0 and so
+            // never becomes -0, unchanged: it is left out
             if (ed) {  // FlowSolver.inl:2855-2875: rows of wall cells lose their off-diagonals
                 const bool wn = g.is_wall[nb] != 0;
                 if (own0 ? wallcell : wn) ff1 = 0.0;
                 if (own0 ? wn : wallcell) ff0 = 0.0;
             }
             if (own0) {
-                S = S - blend + fde1 - fdi;
+                S = S + fde1 - fdi;
                 own_ff[nint] = ff1;
                 st_out(&a.ff0[f], ff0); st_out(&a.ff1[f], ff1);
             } else {
-                S = S + blend - fde1 + fdi;
+                S = S - fde1 + fdi;
                 own_ff[nint] = ff0;
             }
             xo += own_ff[nint] * pn;
@@ -1355,7 +1346,6 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
     if (c < 0) return;
     const int N = g.N;
     const double ev = a.ev;
-    const double GammaBlend = 0.0;
     double S[SB], A = 0.0;
 #pragma unroll
     for (int s = 0; s < SB; s++) S[s] = 0.0;
@@ -1390,6 +1380,12 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
             const double VisFace = Visac * g.frlc[f];
             const double ff0 = -VisFace - dmax(mf, 0.0);
             const double ff1 = -VisFace + dmin(mf, 0.0);
+            // The convective deferred correction GammaBlend * (fce - fci) is a
+            // signed zero for finite operands (GammaBlend = 0, FlowSolver.inl:
+            // 2523), and adding or subtracting a signed zero leaves a source
+            // sum that is never -0 unchanged (it starts at +0 and x - x = +0):
+            // the term, and the neighbour values that only it reads, are left
+            // out here (k_scalar keeps them; the batch equals it bit for bit).
             if constexpr (TEM) {
                 double Vt = ev * lambda0 + ev * lambda1;
                 Vt -= ev;
@@ -1397,33 +1393,21 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
                 const dv3 gn = ld3(a.tgx, a.tgy, a.tgz, nb);
                 const dv3 dPhiXac = (own0 ? gt : gn) * lambda0 + (own0 ? gn : gt) * lambda1;
                 const double VisFaceT = Vt * g.frlc[f];
-                const double pn = a.tphi[nb];
-                const double p0 = own0 ? pt : pn, p1 = own0 ? pn : pt;
-                const double PhiFace = (mf >= 0.0) ? p0 : p1;
-                const double fce = mf * PhiFace;
                 const double fde1 = Vt * dot(dPhiXac, nrm);
-                const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
                 const double fdi = VisFaceT * dot(dPhiXac, Xpn);
                 const double t0 = -VisFaceT - dmax(mf, 0.0);
                 const double t1 = -VisFaceT + dmin(mf, 0.0);
-                const double blend = GammaBlend * (fce - fci);
-                if (own0) { St = St - blend + fde1 - fdi; own_fft[nint] = t1; }
-                else { St = St + blend - fde1 + fdi; own_fft[nint] = t0; }
+                if (own0) { St = St + fde1 - fdi; own_fft[nint] = t1; }
+                else { St = St - fde1 + fdi; own_fft[nint] = t0; }
             }
 #pragma unroll
             for (int s = 0; s < SB; s++) {
                 const dv3 gn = ld3(a.gx[s], a.gy[s], a.gz[s], nb);
                 const dv3 dPhiXac = (own0 ? gs[s] : gn) * lambda0 + (own0 ? gn : gs[s]) * lambda1;
-                const double pn = a.phi[s][nb];
-                const double p0 = own0 ? ps[s] : pn, p1 = own0 ? pn : ps[s];
-                const double PhiFace = (mf >= 0.0) ? p0 : p1;
-                const double fce = mf * PhiFace;
                 const double fde1 = Visac * dot(dPhiXac, nrm);
-                const double fci = dmin(mf, 0.0) * p0 + dmax(mf, 0.0) * p1;
                 const double fdi = VisFace * dot(dPhiXac, Xpn);
-                const double blend = GammaBlend * (fce - fci);
-                if (own0) S[s] = S[s] - blend + fde1 - fdi;
-                else S[s] = S[s] + blend - fde1 + fdi;
+                if (own0) S[s] = S[s] + fde1 - fdi;
+                else S[s] = S[s] - fde1 + fdi;
             }
             if (own0) {
                 own_ff[nint] = ff1;
