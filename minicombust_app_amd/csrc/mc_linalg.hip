@@ -1404,6 +1404,72 @@ __global__ void __launch_bounds__(BLOCK) k_restrict(int nc, const int *aptr, con
     bc[I] = (T)s;
 }
 
+// k_resid then k_restrict in one launch, for the coarse levels in diagonal
+// storage (a launch there costs more than the pass): 8 lanes per coarse row,
+// lane m forms member m's residual as k_resid does (rounded to T), and the
+// first lane adds the members' residuals in member order as k_restrict does,
+// so bc holds the same bits as the two passes give.  Aggregates have at most
+// 8 members (the host's aggregation checks it).
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_resid_restrict_seq(Mat<T> A, int nc, const int *aptr, const int *amem,
+                                                              const T *b, const T *x, T *bc, const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    constexpr bool SYM = ND < 0;
+    constexpr int NA = SYM ? -ND : ND;
+    const int I = blockIdx.x * (BLOCK / 8) + (threadIdx.x >> 3);
+    const int lane8 = threadIdx.x & 7;
+    int t0 = 0, cnt = 0;
+    double r = 0.0;
+    if (I < nc) {
+        t0 = aptr[I];
+        cnt = aptr[I + 1] - t0;
+        if (lane8 < cnt) {
+            const int i = amem[t0 + lane8];
+            double a[NA > 0 ? NA : 1];
+            int j[NA > 0 ? NA : 1];
+#pragma unroll
+            for (int d = 0; d < NA; d++) {
+                a[d] = dia_at<SYM>(A, d, i);
+                j[d] = clampi(i + A.off[d], A.n);
+            }
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < NA; d++) acc += a[d] * (double)x[j[d]];
+            r = (double)(T)((double)b[i] - acc);
+        }
+    }
+    double s = 0.0;
+    const int base = threadIdx.x & ~7;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const double v = __shfl(r, base + m, 64);
+        if (m < cnt) s += v;
+    }
+    if (lane8 == 0 && I < nc) bc[I] = (T)s;
+}
+
+// The first post-smoothing sweep with the prolongation folded in: every x
+// value it reads is the prolonged x_j + cs xc[agg_j], formed by k_prolong's
+// expression, so the sweep's output equals k_prolong then k_jac bit for bit
+// and the prolonged vector is never stored.  For the coarse levels, where a
+// launch costs more than the gathered aggregate ids (on the finest level the
+// separate pass is faster).
+template <int ND, typename T>
+__global__ void __launch_bounds__(BLOCK) k_jac_prolong(Mat<T> A, const T *dinv, const T *b, const T *x, const int *agg,
+                                                       const T *xc, T cs, T *xout, T omega, int chunk,
+                                                       const KState *stop)
+{
+    if (stop && stop->all_done) return;
+    const int bb = xcd_block(blockIdx.x, gridDim.x);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
+    auto xp = [&](int j) -> T { return (T)(x[j] + cs * xc[agg[j]]); };
+    for_rows<ND, 1>(A, r0, r1, [&](int, int j) { return (double)xp(j); }, [&](int r, const double (&ax)[1]) {
+        const T y = xp(r);
+        xout[r] = y + omega * dinv[r] * (b[r] - (T)ax[0]);
+    });
+}
+
 // coarsest level: x = Ainv b in FP64.  One workgroup of 1024 threads: thread
 // t sums row t % 128 over the 16 columns of part t / 128 (consecutive lanes
 // read consecutive words of the transposed inverse), then the 8 part sums of
@@ -1779,7 +1845,13 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     }
     AmgLevel &Lc = c->amg[l + 1];
     T *bc = lv_b<T>(Lc), *xc = lv_x<T>(Lc);
-    if (A.nd) {
+    // below the finest level the residual and the restriction, and the
+    // prolongation and the first post-smoothing sweep, share a launch
+    const bool fuse = l >= 1 && !half;
+    if (fuse && A.nd) {
+        MC_DISPATCH_ND(nd_code(A), (k_resid_restrict_seq<ND_, T>), dim3(div_up(L.nc, BLOCK / 8)), dim3(BLOCK), 0,
+                       c->stream, A, L.nc, L.agg_ptr, L.agg_members, b, (const T *)cur, bc, stop);
+    } else if (A.nd) {
         T *r = (cur == x) ? tmp : x;   // free until post-smoothing
         if constexpr (sizeof(T) == 4) {
             if (half)
@@ -1797,8 +1869,9 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
     }
     MC_TRY(vcycle<T>(c, l + 1, bc, xc, nullptr, stop));
     const T cs = (T)c->opt.amg_cscale;
-    hipLaunchKernelGGL(k_prolong<T>, dim3(div_up(L.n, BLOCK)), dim3(BLOCK), 0, c->stream, L.n, L.agg, (const T *)xc,
-                       cs, cur, stop);
+    if (!fuse)
+        hipLaunchKernelGGL(k_prolong<T>, dim3(div_up(L.n, BLOCK)), dim3(BLOCK), 0, c->stream, L.n, L.agg,
+                           (const T *)xc, cs, cur, stop);
     // post-smoothing: the last sweep writes x (and the fused dot on level 0)
     for (int s = 0; s < nu; s++) {
         const bool last = s == nu - 1;
@@ -1808,7 +1881,10 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
         // A (FP16 copy: its six off-diagonals), dinv, b, x read, x' written
         const double jbytes = (l != 0 || !c->profile) ? 0.0
                               : half ? 28.0 * L.n : mat_bytes(A, L.nnz, sizeof(T)) + 4.0 * sizeof(T) * L.n;
-        {
+        if (fuse && s == 0) {
+            MC_DISPATCH_ND(nd_code(A), (k_jac_prolong<ND_, T>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, dinv, b,
+                           (const T *)cur, L.agg, (const T *)xc, cs, dst, (T)wts[nu - 1 - s], g.chunk, stop);
+        } else {
             KTimer kt(c, (l == 0) ? (sizeof(T) == 8 ? MC_KC_JAC64 : MC_KC_JAC32) : -1, jbytes);
             if constexpr (sizeof(T) == 4) {
                 if (half)
@@ -1835,8 +1911,13 @@ static int vcycle(Ctx *c, size_t l, const T *b, T *x, KState *ks, const KState *
 //   residual                     M + 3Tn       (+ the FP32 diagonal and D on the FP16 copy)
 //   restriction                  4(n + nc + 1) + T(n + nc)
 //   prolongation                 4n + T(nc + 2n)
-// On the FP16 copy M is its six off-diagonals (12n) and D the FP32 inverse
-// diagonal; the dense coarsest solve reads its inverse (8 n^2) and b, writes x.
+// Below the finest level the residual and restriction share a pass (r is
+// neither written nor read again: M + 2Tn + 4(n + nc + 1) + T nc) and the
+// prolongation is folded into the first post-smoothing sweep (that sweep
+// reads the aggregate ids and the coarse correction, 4n + T nc, and no
+// prolonged vector is written or read).  On the FP16 copy M is its six
+// off-diagonals (12n) and D the FP32 inverse diagonal; the dense coarsest
+// solve reads its inverse (8 n^2) and b, writes x.
 template <typename T>
 static double vcycle_bytes(Ctx *c)
 {
@@ -1855,9 +1936,14 @@ static double vcycle_bytes(Ctx *c)
         const double D = half ? 4.0 * n : t * n;
         b += (nu >= 2 ? M : 0.0) + D + 2.0 * t * n;
         b += (double)std::max(0, nu - 2) * (M + D + 3.0 * t * n);
-        b += M + (half ? 8.0 * n : 0.0) + 3.0 * t * n;
-        b += 4.0 * (n + L.nc + 1) + t * (n + L.nc);
-        b += 4.0 * n + t * (L.nc + 2.0 * n);
+        if (l >= 1 && !half) {
+            b += M + 2.0 * t * n + 4.0 * (n + L.nc + 1) + t * L.nc;
+            b += 4.0 * n + t * L.nc;
+        } else {
+            b += M + (half ? 8.0 * n : 0.0) + 3.0 * t * n;
+            b += 4.0 * (n + L.nc + 1) + t * (n + L.nc);
+            b += 4.0 * n + t * (L.nc + 2.0 * n);
+        }
         b += (double)nu * (M + D + 3.0 * t * n);
     }
     return b;
