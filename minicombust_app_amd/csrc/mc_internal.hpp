@@ -537,8 +537,15 @@ int dia_setup_fine(Ctx *c);
 // bnorm0 = ||b - mean b|| when starting from a guess
 int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
             double atol, mc_solve_status *st, bool x0 = false);
+// The first refinement residual formed by the deflation start: r = b - mean
+// - A x0 in wk[4], mean and ||r|| on the device, ||r|| on the host.
+struct IrStart {
+    const double *mean, *rn;
+    double rnorm;
+};
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit,
-                   double atol, mc_solve_status *st, bool x0 = false, double bnorm0 = 0.0);
+                   double atol, mc_solve_status *st, bool x0 = false, double bnorm0 = 0.0,
+                   const IrStart *start = nullptr);
 int reduce_maxabs_pp(Ctx *c, const double *v, double *out_host);
 int mean_project(Ctx *c, const double *b, double *out, int n);
 

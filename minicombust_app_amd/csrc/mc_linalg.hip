@@ -2306,19 +2306,21 @@ __global__ void k_zero(int n, double *x)
 }
 
 int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, double rtol, int maxit, double atol,
-                   mc_solve_status *st, bool x0, double bnorm0)
+                   mc_solve_status *st, bool x0, double bnorm0, const IrStart *start)
 {
     (void)vals;
     const int n = c->N;
     const Grid g = grid_for(n);
     KState *ks = (KState *)c->dscal;
     KState *hs = (KState *)c->hscal;
-    double *mean = c->dscal + 620, *rn = c->dscal + 621;
+    double *rn = c->dscal + 621;
+    const double *mean = c->dscal + 620;
     double *r = c->wk[4];
     float *r32 = c->wk32[0], *e32 = c->wk32[1];
     PcgVecs<float> w{e32, c->wk32[2], c->wk32[3], c->wk32[4], c->wk32[5], c->wk32[6]};
     const Mat<double> A = level_mat<double>(c->amg[0]);
-    sum_scaled(c, b, n, mean, 1.0 / (double)n);
+    if (start) mean = start->mean;   // the deflation start's mean and first residual
+    else sum_scaled(c, b, n, c->dscal + 620, 1.0 / (double)n);
     // diagonal storage: the first residual of a zero start skips A (same bits,
     // the same rows per thread); CSR rows are reduced in another order
     const bool fast0 = !x0 && nd_code(A) != 0;
@@ -2329,29 +2331,36 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
     const double rbytes = mat_bytes(A, c->nnz, 8) + 24.0 * n;   // A; b, x read, r written
     for (outer = 0; outer < 50; outer++) {
-        if (outer == 0 && fast0) {
-            hipLaunchKernelGGL(k_ir_resid0, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, x, r, c->red, g.chunk);
+        const double *rnd = rn;   // ||r|| on the device, the scale of this step's FP32 system
+        if (outer == 0 && start) {
+            rnd = start->rn;
+            rnorm = start->rnorm;
         } else {
-            KTimer kt(c, MC_KC_IR_RESID, rbytes);
-            MC_DISPATCH_ND(nd_code(A), (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x, r, c->red,
-                           g.chunk);
+            if (outer == 0 && fast0) {
+                hipLaunchKernelGGL(k_ir_resid0, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, b, mean, x, r, c->red,
+                                   g.chunk);
+            } else {
+                KTimer kt(c, MC_KC_IR_RESID, rbytes);
+                MC_DISPATCH_ND(nd_code(A), (k_ir_resid<ND_>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, b, mean, x,
+                               r, c->red, g.chunk);
+            }
+            launch_finish<1>(c->stream, c->red, g.blocks, EpiSqrt{rn});
+            MC_LAUNCH_CHECK(c);
+            MC_HIP(c, hipMemcpyAsync(c->hscal + 621, rn, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            MC_HIP(c, hipStreamSynchronize(c->stream));
+            c->st.host_polls++;
+            rnorm = c->hscal[621];
         }
-        launch_finish<1>(c->stream, c->red, g.blocks, EpiSqrt{rn});
-        MC_LAUNCH_CHECK(c);
-        MC_HIP(c, hipMemcpyAsync(c->hscal + 621, rn, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        MC_HIP(c, hipStreamSynchronize(c->stream));
-        c->st.host_polls++;
-        rnorm = c->hscal[621];
         if (bnorm < 0) bnorm = rnorm;
         if (rnorm <= std::max(rtol * bnorm, atol) || !(rnorm == rnorm)) { conv = rnorm == rnorm; break; }
         if (total >= maxit) break;
-        hipLaunchKernelGGL(k_to32_scaled, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, r, rn, r32);
+        hipLaunchKernelGGL(k_to32_scaled, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, r, rnd, r32);
         // the last refinement step needs only the remaining reduction
         const double need = std::max(rtol * bnorm, atol) / rnorm;
         int it = 0;
         MC_TRY(pcg_core<float>(c, r32, false, w, std::max(inner, 0.5 * need), 0.0, maxit - total, ks, hs, &it));
         total += it;
-        hipLaunchKernelGGL(k_ir_update, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, e32, rn);
+        hipLaunchKernelGGL(k_ir_update, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, e32, rnd);
         MC_LAUNCH_CHECK(c);
     }
     MC_TRY(project_mean_zero(c, x));
@@ -2387,46 +2396,40 @@ struct DeflVecs {
     double *aw[DEFL_W];
 };
 
-// aw[k0 + k] = A w[k0 + k] for NR vectors per matrix read
-template <int ND, int NR>
-__global__ void __launch_bounds__(BLOCK) k_defl_aw(Mat<double> A, DeflVecs v, int k0, int chunk)
-{
-    const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
-    for_rows<ND, NR>(A, r0, r1, [&](int k, int j) { return v.w[k0 + k][j]; },
-                     [&](int row, const double (&ax)[NR]) {
-#pragma unroll
-                         for (int k = 0; k < NR; k++) v.aw[k0 + k][row] = ax[k];
-                     });
-}
-
 template <int K>
 constexpr int defl_nv() { return K * (K + 3) / 2 + 1; }   // E upper triangle, W^T b, sum b
 
-// partials of (w_a, A w_c) for a <= c, (w_a, b), and sum b
-template <int K>
-__global__ void __launch_bounds__(BLOCK) k_defl_dots(int n, DeflVecs v, const double *b, double *partials, int chunk)
+// aw_k = A w_k for the K vectors in one pass over the matrix, with the block
+// partials of (w_a, aw_c) for a <= c, (w_a, b) and sum b formed from the
+// products' rows as they are made (each row read once, not again by a
+// separate reduction pass)
+template <int ND, int K>
+__global__ void __launch_bounds__(BLOCK) k_defl_aw_dots(Mat<double> A, DeflVecs v, const double *b, double *partials,
+                                                        int chunk)
 {
     constexpr int NV = defl_nv<K>();
     const int bb = xcd_block(blockIdx.x, gridDim.x);
-    const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
+    const int r0 = bb * chunk, r1 = min(A.n, r0 + chunk);
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) acc[q] = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
-        double w[K], aw[K];
+    for_rows<ND, K>(A, r0, r1, [&](int k, int j) { return v.w[k][j]; }, [&](int row, const double (&ax)[K]) {
+        double w[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) { w[k] = v.w[k][i]; aw[k] = v.aw[k][i]; }
-        const double bi = b[i];
+        for (int k = 0; k < K; k++) {
+            v.aw[k][row] = ax[k];
+            w[k] = v.w[k][row];
+        }
+        const double bi = b[row];
         int q = 0;
 #pragma unroll
         for (int a = 0; a < K; a++)
 #pragma unroll
-            for (int c = a; c < K; c++) acc[q++] += w[a] * aw[c];
+            for (int c = a; c < K; c++) acc[q++] += w[a] * ax[c];
 #pragma unroll
         for (int a = 0; a < K; a++) acc[q++] += w[a] * bi;
         acc[q] += bi;
-    }
+    });
     block_partials<NV>(acc, partials, bb, gridDim.x);
 }
 
@@ -2482,10 +2485,15 @@ struct EpiDefl {
     }
 };
 
-// partials of ||b - mean||^2 and ||b - mean - A W y||^2
+// x = W y and r = (b - mean) - A W y (from the stored products), with the
+// partials of ||b - mean||^2 and ||r||^2: the start, its residual (the first
+// residual of the refinement that follows) and the two norms of its
+// acceptance test in one pass.  A dropped vector has y_k = 0 and is skipped
+// (it may hold NaN).  x and r are written whatever the test decides: a
+// rejected start leaves the solve to set x = 0 itself.
 template <int K>
-__global__ void __launch_bounds__(BLOCK) k_defl_res(int n, DeflVecs v, const double *b, const double *sc,
-                                                    double *partials, int chunk)
+__global__ void __launch_bounds__(BLOCK) k_defl_start(int n, DeflVecs v, const double *b, const double *sc, double *x,
+                                                      double *r, double *partials, int chunk)
 {
     const int bb = xcd_block(blockIdx.x, gridDim.x);
     const int i0 = bb * chunk, i1 = min(n, i0 + chunk);
@@ -2496,12 +2504,17 @@ __global__ void __launch_bounds__(BLOCK) k_defl_res(int n, DeflVecs v, const dou
     double acc[2] = {0.0, 0.0};
     for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
         const double bt = b[i] - mu;
-        double r = bt;
+        double rr = bt, s = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++)
-            if (y[k] != 0.0) r -= y[k] * v.aw[k][i];   // a dropped vector may hold NaN
+            if (y[k] != 0.0) {
+                rr -= y[k] * v.aw[k][i];
+                s += y[k] * v.w[k][i];
+            }
+        x[i] = s;
+        r[i] = rr;
         acc[0] += bt * bt;
-        acc[1] += r * r;
+        acc[1] += rr * rr;
     }
     block_partials<2>(acc, partials, bb, gridDim.x);
 }
@@ -2518,21 +2531,6 @@ struct EpiDeflAccept {
         sc[K + 4] = (sc[K + 1] > 0.0 && rn < bn) ? 1.0 : 0.0;
     }
 };
-
-// x = W y if the start is kept, else 0
-template <int K>
-__global__ void k_defl_set(int n, DeflVecs v, const double *sc, double *x)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    if (sc[K + 4] > 0.0) {
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (sc[1 + k] != 0.0) s += sc[1 + k] * v.w[k][i];
-    }
-    x[i] = s;
-}
 
 // A grid for the deflation reductions: at most 21 partials per block.
 static Grid defl_grid(int n)
@@ -2556,22 +2554,25 @@ static int defl_buffers(Ctx *c, int nrec)
 }
 
 template <int K>
-static void defl_launch(Ctx *c, const DeflVecs &v, const double *b, double *x, double *sc)
+static void defl_launch(Ctx *c, const DeflVecs &v, const double *b, double *x, double *r, double *sc)
 {
     const int n = c->N;
     const Grid g = defl_grid(n);
-    hipLaunchKernelGGL(k_defl_dots<K>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, b, c->red, g.chunk);
+    const Mat<double> A = level_mat<double>(c->amg[0]);
+    MC_DISPATCH_ND(nd_code(A), (k_defl_aw_dots<ND_, K>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, b, c->red,
+                   g.chunk);
     launch_finish<defl_nv<K>()>(c->stream, c->red, g.blocks, EpiDefl<K>{sc, 1.0 / n});
-    hipLaunchKernelGGL(k_defl_res<K>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, b, (const double *)sc, c->red,
-                       g.chunk);
+    hipLaunchKernelGGL(k_defl_start<K>, dim3(g.blocks), dim3(BLOCK), 0, c->stream, n, v, b, (const double *)sc, x, r,
+                       c->red, g.chunk);
     launch_finish<2>(c->stream, c->red, g.blocks, EpiDeflAccept{sc, K});
-    hipLaunchKernelGGL(k_defl_set<K>, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, v, (const double *)sc, x);
 }
 
-// The start of a loop-1 solve: x set (x0 = W y or 0), *used = the start is
-// kept, *bnorm = ||b - mean b||; *guess = the learned guess is in W.
+// The start of a loop-1 solve: *used = the start is kept (x = W y, and *ir
+// the refinement's first residual: r = b - mean b - A x in wk[4], its norm
+// and the mean), *bnorm = ||b - mean b||; *guess = the learned guess is in W.
+// A start not kept leaves x to the solve, which starts from zero.
 static int deflation_start(Ctx *c, const double *b, double *x, bool want_guess, bool *used, double *bnorm,
-                           bool *guess)
+                           bool *guess, IrStart *ir)
 {
     *used = false;
     *guess = false;
@@ -2598,26 +2599,13 @@ static int deflation_start(Ctx *c, const double *b, double *x, bool want_guess, 
         v.aw[K] = c->defl_aw[K];
     }
     if (K == 0) return MC_OK;
-    const Grid g = grid_for(n);
-    const Mat<double> A = level_mat<double>(c->amg[0]);
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int nr = std::min(4, K - k0);
-        if (nr == 4)
-            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 4>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
-        else if (nr == 3)
-            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 3>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
-        else if (nr == 2)
-            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 2>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
-        else
-            MC_DISPATCH_ND(nd_code(A), (k_defl_aw<ND_, 1>), dim3(g.blocks), dim3(BLOCK), 0, c->stream, A, v, k0, g.chunk);
-    }
-    double *sc = c->dscal + 640;
+    double *sc = c->dscal + 640, *r = c->wk[4];
     switch (K) {
-    case 1: defl_launch<1>(c, v, b, x, sc); break;
-    case 2: defl_launch<2>(c, v, b, x, sc); break;
-    case 3: defl_launch<3>(c, v, b, x, sc); break;
-    case 4: defl_launch<4>(c, v, b, x, sc); break;
-    default: defl_launch<5>(c, v, b, x, sc); break;
+    case 1: defl_launch<1>(c, v, b, x, r, sc); break;
+    case 2: defl_launch<2>(c, v, b, x, r, sc); break;
+    case 3: defl_launch<3>(c, v, b, x, r, sc); break;
+    case 4: defl_launch<4>(c, v, b, x, r, sc); break;
+    default: defl_launch<5>(c, v, b, x, r, sc); break;
     }
     MC_LAUNCH_CHECK(c);
     MC_HIP(c, hipMemcpyAsync(c->hscal + 640, sc, (K + 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -2629,6 +2617,9 @@ static int deflation_start(Ctx *c, const double *b, double *x, bool want_guess, 
     // of the others) has coefficient exactly 0 and contributes nothing to x0
     if (*guess && h[1] == 0.0) *guess = false;
     if (*used) {
+        ir->mean = sc;
+        ir->rn = sc + K + 3;
+        ir->rnorm = h[K + 3];
         const double ratio = h[K + 2] > 0.0 ? h[K + 3] / h[K + 2] : 0.0;
         c->st.p_defl_starts += 1;
         c->st.p_defl_ratio_sum += ratio;
@@ -2676,6 +2667,7 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     // cost, re-checking the other choice after 16 solves, then after twice as
     // many each time a re-check confirms the choice (at most 1024)
     bool recheck = false, guess = false;
+    IrStart ir{};
     if (c->opt.learned && c->mpnn && atol == 0.0) {
         tried = true;
         if (c->opt.learned == 1) {
@@ -2690,10 +2682,11 @@ int pressure_solve(Ctx *c, const double *b, double *x, mc_solve_status *st, doub
     // loop-1 solves (atol == 0) start from the deflation space when it holds
     // a vector; later loops solve the small non-orthogonal remainder from zero
     if (atol == 0.0 && (tried || c->opt.p_recycle > 0))
-        MC_TRY(deflation_start(c, b, x, tried, &used, &bn0, &guess));
+        MC_TRY(deflation_start(c, b, x, tried, &used, &bn0, &guess, &ir));
     const auto tg = std::chrono::steady_clock::now();
     if (c->opt.psolve == MC_PSOLVE_MIXED)
-        rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used, bn0);
+        rc = mixed_ir_solve(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used, bn0,
+                            used ? &ir : nullptr);
     else
         rc = pcg_amg(c, c->PA_vals, b, x, c->opt.p_rtol, c->opt.p_maxit, atol, &s, used);
     s.guess_used = guess ? 1 : 0;

@@ -599,20 +599,21 @@ __global__ void __launch_bounds__(256) k_mp_encode(int n, const float *f, const 
 
 // One message-passing layer for every cell (LAST: the decoder too, y only).
 // Persistent: workgroup b serves the bricks of XCD b % 8's contiguous range
-// (the grid is a multiple of 8); its four waves take the brick's 16 tiles of
-// 16 cells in turn.  Three resident workgroups per compute unit (at most 170
-// registers, 50 KB of LDS each); holding Wb's fragments in registers instead
-// (two workgroups) measured 25% slower.
-constexpr int MP_LB = 3;
+// (the grid is a multiple of 8); its eight waves take the brick's 16 tiles of
+// 16 cells in turn.  Two resident workgroups of 512 threads per compute unit
+// (four waves per SIMD: at most 128 registers; 50 KB of LDS each); three of
+// 256 threads (three waves per SIMD) measured 2.5% slower, and holding Wb's
+// fragments in registers (two waves per SIMD) 25% slower.
+constexpr int MP_THREADS = 512, MP_LB = 2;
 template <bool LAST>
-__global__ void __launch_bounds__(256, MP_LB) k_mp_layer(CellWalk w, int nbricks, const bf16_t *hin, bf16_t *hout,
+__global__ void __launch_bounds__(MP_THREADS, MP_LB) k_mp_layer(CellWalk w, int nbricks, const bf16_t *hin, bf16_t *hout,
                                                     const int *nbr, const float *e, const float *deg,
                                                     const bf16_t *wf, const float *Wl, const float *Wall, float *y)
 {
     __shared__ bf16x8 sW[FR_SZ / 8];
     __shared__ float sB[6][MH];   // we, b1, b2, bn1, bn2, bf16(wdec)
-    for (int i = threadIdx.x; i < FR_SZ / 8; i += 256) sW[i] = ((const bf16x8 *)wf)[i];
-    for (int i = threadIdx.x; i < MH; i += 256) {
+    for (int i = threadIdx.x; i < FR_SZ / 8; i += MP_THREADS) sW[i] = ((const bf16x8 *)wf)[i];
+    for (int i = threadIdx.x; i < MH; i += MP_THREADS) {
         sB[0][i] = Wl[L_WE + i]; sB[1][i] = Wl[L_B1 + i]; sB[2][i] = Wl[L_B2 + i];
         sB[3][i] = Wl[L_BN1 + i]; sB[4][i] = Wl[L_BN2 + i];
         sB[5][i] = from_bf16(to_bf16(Wall[OFF_WDEC + i]));
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(256, MP_LB) k_mp_layer(CellWalk w, int nbricks
     const int per = (nbricks + nxcd - 1) / nxcd;
     const int bend = min(nbricks, (xcd + 1) * per);
     for (int bi = xcd * per + local; bi < bend; bi += nloc) {
-        for (int tile = wave; tile < 16; tile += 4) {
+        for (int tile = wave; tile < 16; tile += MP_THREADS / 64) {
             const int cell = walk_cell_at(w, bi, tile * 16 + r);
             const bool valid = cell >= 0;
             // an offset the compiler cannot see through: the weight fragments
@@ -662,17 +663,23 @@ __global__ void __launch_bounds__(256, MP_LB) k_mp_layer(CellWalk w, int nbricks
             }
 #pragma unroll
             for (int q = 0; q < 6; q++) input(nb[q], hn[q]);
-            {
+            // output row tile by row tile: the tile's two Wb fragments are
+            // read from LDS once for all six neighbours (not once per
+            // neighbour); each Q_j starts from P and the MFMAs add h_j Wb in
+            // the same order, and every element of S adds its six terms in
+            // neighbour order, so the bits are those of neighbour-major order
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const bf16x8 wb0 = fW[FR_WB / 8 + (2 * j) * 64 + lane];
+                const bf16x8 wb1 = fW[FR_WB / 8 + (2 * j + 1) * 64 + lane];
 #pragma unroll
                 for (int q = 0; q < 6; q++) {
-                    // Q_j starts from P: the MFMA adds h_j Wb to it
-                    float4v Q[4] = {P[0], P[1], P[2], P[3]};
-                    mm_lds<2>(fW + FR_WB / 8, hn[q], Q, lane);
+                    float4v Q = P[j];
+                    Q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb0, hn[q][0], Q, 0, 0, 0);
+                    Q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb1, hn[q][1], Q, 0, 0, 0);
                     if (nb[q] >= 0) {
 #pragma unroll
-                        for (int j = 0; j < 4; j++)
-#pragma unroll
-                            for (int t = 0; t < 4; t++) S[j][t] += fmaxf(Q[j][t] + es[q] * we[j][t], 0.f);
+                        for (int t = 0; t < 4; t++) S[j][t] += fmaxf(Q[t] + es[q] * we[j][t], 0.f);
                     }
                 }
             }
@@ -1009,7 +1016,7 @@ static void forward_infer(Ctx *c, const float *f, const float *e)
     const int nbricks = walk_blocks(w);
     // MP_LB resident workgroups per compute unit, a multiple of the 8 XCDs
     const int per_xcd = std::max(1, std::min(div_up(nbricks, 8), MP_LB * std::max(m->cus, 8) / 8));
-    const dim3 grid(8 * per_xcd), blk(256);
+    const dim3 grid(8 * per_xcd), blk(256), lblk(MP_THREADS);
     const float *W = m->w;
     hipLaunchKernelGGL(k_mp_encode, dim3(div_up(n, 64)), blk, 0, c->stream, n, f, (const bf16_t *)(m->wf + ML * FR_SZ),
                        W, m->hb[0]);
@@ -1018,10 +1025,10 @@ static void forward_infer(Ctx *c, const float *f, const float *e)
         const bf16_t *wf = m->wf + l * FR_SZ;
         const float *Wl = W + OFF_L0 + l * LSZ;
         if (l < ML - 1)
-            hipLaunchKernelGGL(k_mp_layer<false>, grid, blk, 0, c->stream, w, nbricks, hin, m->hb[(l + 1) & 1], m->nbr,
+            hipLaunchKernelGGL(k_mp_layer<false>, grid, lblk, 0, c->stream, w, nbricks, hin, m->hb[(l + 1) & 1], m->nbr,
                                e, m->deg, wf, Wl, W, (float *)nullptr);
         else
-            hipLaunchKernelGGL(k_mp_layer<true>, grid, blk, 0, c->stream, w, nbricks, hin, (bf16_t *)nullptr, m->nbr,
+            hipLaunchKernelGGL(k_mp_layer<true>, grid, lblk, 0, c->stream, w, nbricks, hin, (bf16_t *)nullptr, m->nbr,
                                e, m->deg, wf, Wl, W, m->y);
     }
 }
