@@ -74,7 +74,7 @@ typedef struct mc_options {
     double amg_omega;      /* Jacobi damping                                       */
     double amg_cscale;     /* coarse-correction scaling                            */
     int    amg_coarse_max; /* dense coarsest level size bound                      */
-    double mixed_inner_rtol; /* FP32 inner solve reduction per refinement step     */
+    double mixed_inner_rtol; /* FP32 inner solve reduction per refinement step (1e-5) */
     int    check_every;    /* Krylov iterations between host convergence checks    */
     double inlet_scale;    /* multiplies the reference gas velocity (Mesh.hpp:114,
                               inlet and initial field); 1 = the reference case    */

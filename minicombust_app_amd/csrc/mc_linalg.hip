@@ -2328,7 +2328,7 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
     double bnorm = x0 ? bnorm0 : -1.0, rnorm = 0.0;
     int total = 0, outer = 0;
     bool conv = false;
-    const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-3;
+    const double inner = c->opt.mixed_inner_rtol > 0 ? c->opt.mixed_inner_rtol : 1e-5;
     const double rbytes = mat_bytes(A, c->nnz, 8) + 24.0 * n;   // A; b, x read, r written
     for (outer = 0; outer < 50; outer++) {
         const double *rnd = rn;   // ||r|| on the device, the scale of this step's FP32 system
