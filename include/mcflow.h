@@ -70,7 +70,7 @@ typedef struct mc_options {
     int    mom_maxit;
     double p_rtol;         /* pressure-correction relative residual (1e-8)         */
     int    p_maxit;
-    int    amg_nu;         /* Jacobi sweeps before and after the coarse correction */
+    int    amg_nu;         /* Jacobi sweeps before and after the coarse correction (3) */
     double amg_omega;      /* Jacobi damping                                       */
     double amg_cscale;     /* coarse-correction scaling                            */
     int    amg_coarse_max; /* dense coarsest level size bound                      */
