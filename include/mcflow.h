@@ -75,7 +75,8 @@ typedef struct mc_options {
     double amg_cscale;     /* coarse-correction scaling                            */
     int    amg_coarse_max; /* dense coarsest level size bound                      */
     double mixed_inner_rtol; /* FP32 inner solve reduction per refinement step (1e-5) */
-    int    check_every;    /* Krylov iterations between host convergence checks    */
+    int    check_every;    /* ignored (kept for the layout): host convergence checks
+                              follow batches predicted from the convergence rate */
     double inlet_scale;    /* multiplies the reference gas velocity (Mesh.hpp:114,
                               inlet and initial field); 1 = the reference case    */
     double amg_cheb_ratio; /* > 1: the nu smoothing sweeps use Chebyshev weights on

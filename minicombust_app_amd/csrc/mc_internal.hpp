@@ -263,9 +263,6 @@ struct Ctx {
 // +-nx then sit in the same L2 as the cell that gathers them.
 __device__ __forceinline__ int xcd_block(int bid, int nblocks)
 {
-#ifdef MC_NO_XCD_REMAP
-    return bid;
-#endif
     constexpr int NXCD = 8;
     const int per = nblocks / NXCD, rem = nblocks % NXCD;
     const int x = bid % NXCD, local = bid / NXCD;
