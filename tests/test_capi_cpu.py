@@ -101,6 +101,11 @@ def test_options_default():
     assert o.mode == mc.MODE_COMPAT and o.psolve == mc.PSOLVE_FP64
     assert o.delta == 1e-5 and o.mom_rtol == 1e-5 and o.p_rtol == 1e-8
     assert 8 <= o.amg_coarse_max <= 128 and 1 <= o.amg_nu <= 8
+    # the solver settings the measurements were taken with: three
+    # Chebyshev-weighted sweeps on [0.5, 2], inner FP32 solves to 1e-5,
+    # coarse levels refreshed every 4th solve, FP16 smoothing copy
+    assert o.amg_nu == 3 and o.amg_cheb_ratio == 4.0 and o.mixed_inner_rtol == 1e-5
+    assert o.amg_refresh == 4 and o.amg_half == 1 and o.amg_tail == 1 and o.pcg_graph == 0
     assert L.mc_options_default(None) == mc.MC_ERR_ARG
 
 
