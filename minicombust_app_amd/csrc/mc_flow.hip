@@ -1037,6 +1037,12 @@ __global__ void k_copy(int n, const double *a, double *b)
 }
 
 // calculate_pressure (FlowSolver.inl:2317-2463)
+__global__ void k_zero4(int n, double *a, double *b, double *c, double *d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; d[i] = 0.0; }
+}
+
 int calculate_pressure(Ctx *c)
 {
     const int N = c->N;
@@ -1060,6 +1066,7 @@ int calculate_pressure(Ctx *c)
         // assembly, so every later solve reproduces the stored solution)
         const bool fresh = simple || c->first_pressure || c->u == nullptr;
         StageTimer ts(c, MC_ST_PRES_SOLVE);
+        c->p_zero = false;
         if (simple) {
             MC_HIP(c, hipMemcpyAsync(c->Pb, c->SU, (size_t)N * 8, hipMemcpyDeviceToDevice, c->stream));
         }
@@ -1077,7 +1084,8 @@ int calculate_pressure(Ctx *c)
         }
         ts.end();
         StageTimer tc(c, MC_ST_PRES_CORRECT);
-        MC_TRY(reduce_maxabs_pp(c, c->phi[PP], &pmax));
+        if (c->p_zero) pmax = 0.0;   // the solve left PP = +0 everywhere
+        else MC_TRY(reduce_maxabs_pp(c, c->phi[PP], &pmax));
         if (loop == 1) pref = pmax;
         if (loop > 1 && pmax == 0.0) {
             // PP is zero everywhere: this loop's right-hand side (the previous
@@ -1087,8 +1095,9 @@ int calculate_pressure(Ctx *c)
             // signed zeros to the fluxes, P and the velocities and leave PP
             // zero; their other outputs are written directly: grad PP and the
             // gathered correction S_U are zero
-            for (double *v : {c->gx[PP], c->gy[PP], c->gz[PP], c->SU})
-                MC_HIP(c, hipMemsetAsync(v, 0, (size_t)N * 8, c->stream));
+            hipLaunchKernelGGL(k_zero4, dim3(div_up(N, 256)), dim3(256), 0, c->stream, N, c->gx[PP], c->gy[PP],
+                               c->gz[PP], c->SU);
+            MC_LAUNCH_CHECK(c);
             c->st.p_loops_skipped += 1;
             c->first_pressure = false;
             break;
@@ -1527,7 +1536,7 @@ static int batch_solve(Ctx *c, bool tem)
     for (int s = 0; s < SB; s++) {
         a.phi[s] = c->phi[fields[s]];
         a.gx[s] = c->gx[fields[s]]; a.gy[s] = c->gy[fields[s]]; a.gz[s] = c->gz[fields[s]];
-        a.S[s] = c->srhs[s];
+        a.S[s] = (s == SB - 1) ? c->SU : c->srhs[s];   // S_phi.U ends holding VARP's, as in the reference
         a.inlet_phi[s] = (types[s] == FUEL || types[s] == VARFU) ? c->k.gas_ful : 0.0;
     }
     a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.vals = c->A_vals;
@@ -1535,7 +1544,7 @@ static int batch_solve(Ctx *c, bool tem)
     a.type = FUEL;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
     a.tphi = c->phi[TEM]; a.tgx = c->gx[TEM]; a.tgy = c->gy[TEM]; a.tgz = c->gz[TEM];
-    a.tS = c->SU; a.tvals = c->A_vals2; a.tdia = (double *)fine_dia_of(c, c->A_vals2); a.tinlet = c->k.gas_tem;
+    a.tS = c->srhs[SB]; a.tvals = c->A_vals2; a.tdia = (double *)fine_dia_of(c, c->A_vals2); a.tinlet = c->k.gas_tem;
     // the warm-started solves form b - A x0 in their own pass here: in this
     // kernel the extra accumulators cost more (168 VGPRs, +0.2 ms) than the pass
     const bool warm = warm_start(c);
@@ -1560,20 +1569,18 @@ static int batch_solve(Ctx *c, bool tem)
     }
     StageTimer ts(c, MC_ST_SCA_SOLVE);
     if (tem) {
-        double *bt[1] = {c->SU}, *xt[1] = {c->phi[TEM]};
+        double *bt[1] = {c->srhs[SB]}, *xt[1] = {c->phi[TEM]};
         mc_solve_status stt{};
         MC_TRY(bicgstab_solve(c, c->A_vals2, 1, bt, xt, c->opt.mom_rtol, c->opt.mom_maxit, &stt, warm));
         c->st.scalar_iters += stt.iterations;
         if (!stt.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
     }
     double *b[SB], *x[SB];
-    for (int s = 0; s < SB; s++) { b[s] = c->srhs[s]; x[s] = c->phi[fields[s]]; }
+    for (int s = 0; s < SB; s++) { b[s] = a.S[s]; x[s] = c->phi[fields[s]]; }
     mc_solve_status st{};
     MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
-    // S_phi.U holds the last equation's right-hand side, as after VARP
-    MC_HIP(c, hipMemcpyAsync(c->SU, c->srhs[SB - 1], (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
     return MC_OK;
 }
 
@@ -1741,7 +1748,7 @@ int flow_workspace(Ctx *c)
         MemScope ms(c, MC_MEM_MATRIX);
         MC_TRY(dalloc(c, &c->A_vals2, c->nnz));
     }
-    for (int s = 0; s < 4; s++) MC_TRY(dalloc(c, &c->srhs[s], c->N));
+    for (int s = 0; s < 5; s++) MC_TRY(dalloc(c, &c->srhs[s], c->N));
     MC_TRY(dalloc(c, &c->neg_list, c->N));
     return MC_OK;
 }

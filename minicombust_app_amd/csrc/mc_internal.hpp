@@ -163,12 +163,13 @@ struct Ctx {
     int64_t amg_updates = 0;       // pressure-matrix value updates since the hierarchy was built
     bool pa_dia_fresh = false;     // the pressure assembly also wrote the AMG fine level's diagonal copies
     bool first_pressure = true;
+    bool p_zero = false;          // the last pressure solve met its tolerance at x = 0 (x zeroed, no iteration)
     bool field_set_up = false;   // set_up_field has run (first pass of timestep 0)
     int64_t timestep_count = 0;
     int64_t global_iter = 0;
 
     // ---- solver workspace ----
-    double *srhs[4] = {};          // right-hand sides of the batched scalar solve
+    double *srhs[5] = {};          // right-hand sides of the batched scalar solve (4: TEM)
     double *r0buf[3] = {};         // b - A x0 formed by an assembly for the warm-started solve after it
     int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
     double *wk[24] = {};           // N-length work vectors

@@ -2232,7 +2232,9 @@ int pcg_amg(Ctx *c, const double *vals, const double *b, double *x, double rtol,
     PcgVecs<double> w{x, c->wk[0], c->wk[1], c->wk[2], c->wk[3], c->wk[5]};
     int iters = 0;
     MC_TRY(pcg_core<double>(c, b, true, w, rtol, atol, maxit, ks, hs, &iters, x0));
-    MC_TRY(project_mean_zero(c, x));
+    // met at the zero start: x is all +0, already mean-zero
+    c->p_zero = !x0 && iters == 0 && hs->conv[0];
+    if (!c->p_zero) MC_TRY(project_mean_zero(c, x));
     if (st) {
         st->iterations = iters;
         st->converged = hs->conv[0];
@@ -2363,7 +2365,9 @@ int mixed_ir_solve(Ctx *c, const double *vals, const double *b, double *x, doubl
         hipLaunchKernelGGL(k_ir_update, dim3(div_up(n, 256)), dim3(256), 0, c->stream, n, x, e32, rnd);
         MC_LAUNCH_CHECK(c);
     }
-    MC_TRY(project_mean_zero(c, x));
+    // met at the zero start: x is all +0, already mean-zero
+    c->p_zero = !x0 && conv && total == 0;
+    if (!c->p_zero) MC_TRY(project_mean_zero(c, x));
     if (st) {
         st->iterations = total;
         st->converged = conv ? 1 : 0;
