@@ -308,6 +308,8 @@ int mc_ctx_destroy(mc_ctx *x)
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     if (c->ev_coarse) (void)hipEventDestroy(c->ev_coarse);
     if (c->ev_dense) (void)hipEventDestroy(c->ev_dense);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
     delete x;
     return MC_OK;

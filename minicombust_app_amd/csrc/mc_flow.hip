@@ -1528,7 +1528,10 @@ __global__ void __launch_bounds__(256, 3) k_scalar_batch(Geo g, ScalBatchArgs a)
 // into its own matrix and solved first, as the reference orders them (none of
 // the four reads the temperature, and TEM's assembly reads nothing the TE /
 // ED solves change).
-static int batch_solve(Ctx *c, bool tem)
+// The batched pass's arguments.  side = true: it writes the alternate matrix,
+// diagonal and face-coefficient arrays and keeps VARP's right-hand side in
+// srhs[3] (S_U is then the ED system's), so it can run beside the ED solve.
+static ScalBatchArgs batch_args(Ctx *c, bool side)
 {
     static const int types[SB] = {FUEL, PROG, VARFU, VARPR};
     static const int fields[SB] = {FUL, PRO, VARF, VARP};
@@ -1536,37 +1539,47 @@ static int batch_solve(Ctx *c, bool tem)
     for (int s = 0; s < SB; s++) {
         a.phi[s] = c->phi[fields[s]];
         a.gx[s] = c->gx[fields[s]]; a.gy[s] = c->gy[fields[s]]; a.gz[s] = c->gz[fields[s]];
-        a.S[s] = (s == SB - 1) ? c->SU : c->srhs[s];   // S_phi.U ends holding VARP's, as in the reference
+        a.S[s] = (s == SB - 1 && !side) ? c->SU : c->srhs[s];   // S_phi.U ends holding VARP's, as in the reference
         a.inlet_phi[s] = (types[s] == FUEL || types[s] == VARFU) ? c->k.gas_ful : 0.0;
     }
-    a.mflux = c->mflux; a.ff0 = c->ff0; a.ff1 = c->ff1; a.AV = c->AV; a.vals = c->A_vals;
-    a.dia = (double *)fine_dia_of(c, c->A_vals);
+    a.mflux = c->mflux;
+    a.ff0 = side ? c->ff0b : c->ff0; a.ff1 = side ? c->ff1b : c->ff1; a.AV = side ? c->AV2 : c->AV;
+    a.vals = c->A_vals;
+    a.dia = side ? c->A_dia3 : (double *)fine_dia_of(c, c->A_vals);
     a.type = FUEL;
     a.ev = c->k.ev; a.inlet_ev = c->k.inlet_ev; a.rdelta = c->k.rdelta; a.urf = 0.95;
     a.tphi = c->phi[TEM]; a.tgx = c->gx[TEM]; a.tgy = c->gy[TEM]; a.tgz = c->gz[TEM];
     a.tS = c->srhs[SB]; a.tvals = c->A_vals2; a.tdia = (double *)fine_dia_of(c, c->A_vals2); a.tinlet = c->k.gas_tem;
-    // the warm-started solves form b - A x0 in their own pass here: in this
-    // kernel the extra accumulators cost more (168 VGPRs, +0.2 ms) than the pass
+    return a;
+}
+
+static void batch_launch(Ctx *c, const ScalBatchArgs &a, bool tem, hipStream_t s)
+{
+    // per cell and field phi and gradient (32) read and S (8) written;
+    // per cell centre (24), volume and density (16), forder and fother
+    // (48) read, A (8) written; per face mass flux, lambda, rlencos,
+    // normal (48) read; per interior face ff0, ff1 (16) written; per
+    // boundary face centre (24) read; one matrix per equation system
+    // (8 nnz each) written
+    const int nsc = SB + (tem ? 1 : 0);
+    const Sizes z = sizes_of(c);
+    const double bytes = (40.0 * nsc + 96.0) * z.N + 48.0 * z.F + 16.0 * z.Fi + 24.0 * z.Fb +
+                         8.0 * z.nnz * (tem ? 2 : 1);
+    KTimer kt(c, s == c->stream ? MC_KC_SCALAR_BATCH : -1, bytes);
+    if (tem)
+        hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, s, geo(c), a);
+    else
+        hipLaunchKernelGGL(k_scalar_batch<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, s, geo(c), a);
+}
+
+// The solves of the batched pass's systems: TEM (its own matrix), then FUL,
+// PRO, VARF, VARP as one 4-RHS solve.  The warm-started solves form b - A x0
+// in their own pass here: in the assembly kernel the extra accumulators cost
+// more (168 VGPRs, +0.2 ms) than the pass.
+static int batch_solves(Ctx *c, const ScalBatchArgs &a, bool tem)
+{
+    static const int fields[SB] = {FUL, PRO, VARF, VARP};
     const bool warm = warm_start(c);
-    {
-        // per cell and field phi and gradient (32) read and S (8) written;
-        // per cell centre (24), volume and density (16), forder and fother
-        // (48) read, A (8) written; per face mass flux, lambda, rlencos,
-        // normal (48) read; per interior face ff0, ff1 (16) written; per
-        // boundary face centre (24) read; one matrix per equation system
-        // (8 nnz each) written
-        const int nsc = SB + (tem ? 1 : 0);
-        const Sizes z = sizes_of(c);
-        const double bytes = (40.0 * nsc + 96.0) * z.N + 48.0 * z.F + 16.0 * z.Fi + 24.0 * z.Fb +
-                             8.0 * z.nnz * (tem ? 2 : 1);
-        StageTimer t(c, MC_ST_SCA_FLUX);
-        KTimer kt(c, MC_KC_SCALAR_BATCH, bytes);
-        if (tem)
-            hipLaunchKernelGGL(k_scalar_batch<true>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
-        else
-            hipLaunchKernelGGL(k_scalar_batch<false>, dim3(walk_blocks(cell_walk(c))), dim3(256), 0, c->stream, geo(c), a);
-        MC_LAUNCH_CHECK(c);
-    }
     StageTimer ts(c, MC_ST_SCA_SOLVE);
     if (tem) {
         double *bt[1] = {c->srhs[SB]}, *xt[1] = {c->phi[TEM]};
@@ -1581,6 +1594,81 @@ static int batch_solve(Ctx *c, bool tem)
     MC_TRY(bicgstab_solve(c, c->A_vals, SB, b, x, c->opt.mom_rtol, c->opt.mom_maxit, &st, warm));
     c->st.scalar_iters += st.iterations;
     if (!st.converged) { c->err = "scalar BiCGStab did not converge"; return MC_ERR_NOCONV; }
+    return MC_OK;
+}
+
+static int batch_solve(Ctx *c, bool tem)
+{
+    const ScalBatchArgs a = batch_args(c, false);
+    {
+        StageTimer t(c, MC_ST_SCA_FLUX);
+        batch_launch(c, a, tem, c->stream);
+        MC_LAUNCH_CHECK(c);
+    }
+    return batch_solves(c, a, tem);
+}
+
+// The batched pass beside the ED solve: it reads nothing the TE / ED solves
+// write (scalar fields and gradients, mass fluxes, geometry) and writes its
+// own arrays, so it can run on the second stream from the end of ED's
+// assembly to the end of ED's solve and repair (not beside an assembly
+// kernel: TE / ED's `k_scalar` keeps the device to itself).  Unused while a
+// stage, the batch kernel or the BiCGStab kernel is timed: their events are
+// on the main stream and would take in the other stream's work.
+static bool batch_beside(const Ctx *c)
+{
+    const unsigned stages = ((1u << MC_ST_COUNT) - 1u) << MC_KC_COUNT;
+    return c->side && c->fine_nd && !(c->profile & (stages | (1u << MC_KC_SCALAR_BATCH) | (1u << MC_KC_BICG)));
+}
+
+// allocated at the first fork; cleared on the main stream, so the fork's
+// event orders the clearing before the batch kernel on the other stream
+static int batch_buffers(Ctx *c)
+{
+    if (c->A_dia3) return MC_OK;
+    {
+        MemScope ms(c, MC_MEM_MATRIX);
+        MC_TRY(dalloc(c, &c->A_dia3, (size_t)c->fine_nd * c->N));   // entries outside the pattern stay zero
+    }
+    {
+        MemScope ms(c, MC_MEM_FIELDS);
+        MC_TRY(dalloc(c, &c->AV2, c->N + 6));   // boundary slots zero, as AV's
+        MC_TRY(dalloc(c, &c->ff0b, c->F));      // boundary faces zero, as ff0's and ff1's
+        MC_TRY(dalloc(c, &c->ff1b, c->F));
+    }
+    MC_HIP(c, hipMemsetAsync(c->A_dia3, 0, (size_t)c->fine_nd * c->N * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->AV2, 0, (size_t)(c->N + 6) * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->ff0b, 0, (size_t)c->F * 8, c->stream));
+    MC_HIP(c, hipMemsetAsync(c->ff1b, 0, (size_t)c->F * 8, c->stream));
+    if (!c->ev_fork) MC_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (!c->ev_join) MC_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    return MC_OK;
+}
+
+static int batch_fork(Ctx *c)
+{
+    MC_TRY(batch_buffers(c));
+    MC_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+    MC_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    batch_launch(c, batch_args(c, true), true, c->side);
+    MC_LAUNCH_CHECK(c);
+    MC_HIP(c, hipEventRecord(c->ev_join, c->side));
+    return MC_OK;
+}
+
+// Join: the alternate arrays hold the batched systems (the last assembled, as
+// in the reference's order) and become the current ones; VARP's right-hand
+// side goes to S_U
+static int batch_join(Ctx *c)
+{
+    MC_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    std::swap(c->A_dia, c->A_dia3);
+    std::swap(c->AV, c->AV2);
+    std::swap(c->ff0, c->ff0b);
+    std::swap(c->ff1, c->ff1b);
+    const ScalBatchArgs a = batch_args(c, true);
+    MC_TRY(batch_solves(c, a, true));
+    MC_HIP(c, hipMemcpyAsync(c->SU, c->srhs[SB - 1], (size_t)c->N * 8, hipMemcpyDeviceToDevice, c->stream));
     return MC_OK;
 }
 
@@ -1812,7 +1900,7 @@ int clip_nonneg(Ctx *c, double *phi, int slot)
     return MC_OK;
 }
 
-int scalar_solve(Ctx *c, int type)
+int scalar_solve(Ctx *c, int type, bool fork_batch)
 {
     const int field = scalar_field(type);
     const bool warm = warm_start(c);
@@ -1826,6 +1914,7 @@ int scalar_solve(Ctx *c, int type)
         StageTimer t(c, turb ? MC_ST_TURB_FLUX : MC_ST_SCA_FLUX);
         MC_TRY(scalar_assemble(c, type, fused ? c->r0buf[0] : nullptr));
     }
+    if (fork_batch) MC_TRY(batch_fork(c));
     StageTimer ts(c, turb ? MC_ST_TURB_SOLVE : MC_ST_SCA_SOLVE);
     double *b[1] = {c->SU};
     double *x[1] = {c->phi[field]};
@@ -1862,8 +1951,11 @@ int simple_iteration(Ctx *c, int inner)
     MC_TRY(calculate_pressure(c));
     if (c->opt.solve_scalars) {
         MC_TRY(scalar_solve(c, TERBTE));
-        MC_TRY(scalar_solve(c, TERBED));
-        MC_TRY(batch_solve(c, true));   // TEM, then FUL, PRO, VARF, VARP
+        // TEM, then FUL, PRO, VARF, VARP: assembled beside the ED solve when
+        // nothing is being timed
+        const bool beside = batch_beside(c);
+        MC_TRY(scalar_solve(c, TERBED, beside));
+        MC_TRY(beside ? batch_join(c) : batch_solve(c, true));
     }
     c->st.simple_iterations++;
     c->global_iter++;

@@ -170,6 +170,11 @@ struct Ctx {
 
     // ---- solver workspace ----
     double *srhs[5] = {};          // right-hand sides of the batched scalar solve (4: TEM)
+    // the batched scalar pass's matrix, diagonal and face coefficients when it
+    // runs on `side` beside the ED solve (swapped with A_dia, AV, ff0, ff1 at
+    // the join); ev_fork / ev_join order it between the two streams
+    double *A_dia3 = nullptr, *AV2 = nullptr, *ff0b = nullptr, *ff1b = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double *r0buf[3] = {};         // b - A x0 formed by an assembly for the warm-started solve after it
     int *neg_list = nullptr;       // compacted negative cells (TE/ED repair)
     double *wk[24] = {};           // N-length work vectors
@@ -503,7 +508,7 @@ int momentum_solve(Ctx *c, bool r0 = false);
 int calculate_mass_flux_assemble(Ctx *c);               // mass flux + pressure matrix
 int calculate_pressure(Ctx *c);
 int scalar_assemble(Ctx *c, int type, double *r0 = nullptr);
-int scalar_solve(Ctx *c, int type);
+int scalar_solve(Ctx *c, int type, bool fork_batch = false);
 int scalar_batch_solve(Ctx *c);
 int flow_workspace(Ctx *c);
 int simple_iteration(Ctx *c, int inner);

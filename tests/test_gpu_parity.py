@@ -12,7 +12,8 @@ import pytest
 
 from minicombust_app_amd import mcflow as mc
 from oracle import pyoracle as po
-from tests.helpers import (assert_bits, assert_close, oracle_pressure_assemble, push_state)
+from tests.helpers import (CELL_ARRAYS, FACE_ARRAYS, assert_bits, assert_close, oracle_pressure_assemble,
+                           push_state)
 
 pytestmark = pytest.mark.gpu
 
@@ -462,6 +463,30 @@ def test_cell_visiting_order_does_not_change_results():
         for f in fields:
             assert_bits(fields[f], runs[0][0][f], "order %d field %s" % (k, po.FIELD_NAMES[f]))
         assert_bits(mflux, runs[0][1], "order %d mass flux" % k)
+
+
+def test_scalar_batch_beside_ed_solve_same_bits():
+    """The batched TEM / FUL / PRO / VARF / VARP assembly runs on a second
+    stream beside the ED solve unless the BiCGStab kernel class is being
+    timed: the two orders give the same bits for every field, the last
+    system's matrix, diagonal and right-hand side, and the face arrays, over
+    four SIMPLE iterations."""
+    m = 16
+    print("batch beside ED solve m=%d" % m, flush=True)
+    runs = []
+    for timed in (False, True):
+        with mc.Context(m, mode=mc.MODE_SIMPLE) as g:
+            if timed:
+                g.profile(True, classes=["bicg_spmv"])
+            for it in range(4):
+                g.simple_iteration(it)
+            arrays = {name: g.get(a).copy() for name, a in CELL_ARRAYS + FACE_ARRAYS}
+            arrays["A_vals"] = g.get(mc.ARR_A_VALS).copy()
+            runs.append(({f: g.phi(f).copy() for f in range(po.NFIELDS)}, arrays))
+    for f in runs[0][0]:
+        assert_bits(runs[0][0][f], runs[1][0][f], "beside vs in order: field %s" % po.FIELD_NAMES[f])
+    for name in runs[0][1]:
+        assert_bits(runs[0][1][name], runs[1][1][name], "beside vs in order: %s" % name)
 
 
 @pytest.mark.parametrize("guess", ["current", "worse_than_zero"])
