@@ -119,12 +119,12 @@ def test_scale_up_m200():
     """Modifier 200 (16,000,000 cells): the learned + mixed form and the FP64
     form for 3 SIMPLE iterations each; fields finite, every pressure solve at
     or below 1e-8, the two forms' fields within 1e-7 of each field's maximum,
-    and device memory within the footprint stated for this mesh (31 GB
-    without the net, 62 GB with it, of 288 GB)."""
+    and device memory within the footprint stated for this mesh (33.4 GB
+    without the net, 39.6 GB with its inference buffers, of 288 GB)."""
     m = 200
     w = learned.load_weights()
     runs = {}
-    for name, kw, budget in (("fp64", dict(psolve=mc.PSOLVE_FP64), 34e9),
+    for name, kw, budget in (("fp64", dict(psolve=mc.PSOLVE_FP64), 36e9),
                              ("learned_mixed", dict(psolve=mc.PSOLVE_MIXED, learned=1), 66e9)):
         print("m=%d %s" % (m, name), flush=True)
         with mc.Context(m, **dict(BENCH, **kw)) as g:
