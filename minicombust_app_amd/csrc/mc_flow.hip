@@ -1954,7 +1954,13 @@ int simple_iteration(Ctx *c, int inner)
         // TEM, then FUL, PRO, VARF, VARP: assembled beside the ED solve when
         // nothing is being timed
         const bool beside = batch_beside(c);
-        MC_TRY(scalar_solve(c, TERBED, beside));
+        const int rc = scalar_solve(c, TERBED, beside);
+        if (rc != MC_OK) {
+            // a failed ED solve returns with the batch kernel perhaps still
+            // running: later work on the main stream is ordered after it
+            if (beside && c->ev_join) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+            return rc;
+        }
         MC_TRY(beside ? batch_join(c) : batch_solve(c, true));
     }
     c->st.simple_iterations++;
